@@ -1,0 +1,128 @@
+/* moses2pdf.h -- C ABI of the MI355X-native adNOC_2Pb time-step engine.
+ *
+ * Drop-in for the reference's run path (MoSES_2PDF, landslide and inflow modes).  The
+ * reference has no FFI: its boundary is the C++ class RunKernels
+ * (MoSES_2PDF_landslide/adNOC_2Pb_runKernels.cuh:27-377), whose constructor parses
+ * ./par_List and the data files (adNOC_2Pb_runKernels.cu:4-9, 11-304) and whose run()
+ * allocates, runs the metric setup and the time loop and writes ./result2Pb/
+ * (adNOC_2Pb_runKernels.cu:306-356).  Each entry point below names the reference code it
+ * replaces.
+ *
+ * Conventions: every function returns 0 on success and a negative M2P_E* code on error
+ * (the library never exits); m2p_last_error() gives the message.  One context owns one
+ * HIP stream and all of its device memory.  A context is not thread-safe; distinct
+ * contexts may be driven from different host threads.  Buffers passed in are owned by
+ * the caller.  State planes are laid out as in the reference: 7 planes of nxd*nyd
+ * doubles, plane k holding u[k] at index row*nxd+col, with nxd = nx+6, nyd = ny+6
+ * (three ghost cells per side), k = 0 hs, 1 hs*us, 2 hs*vs, 3 hf, 4 hf*uf, 5 hf*vf,
+ * 6 phi.  All quantities are in the reference's scaled units (lengths x10, g = 1).
+ */
+#ifndef MOSES2PDF_H
+#define MOSES2PDF_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct m2p_ctx m2p_ctx;
+
+enum { M2P_LANDSLIDE = 0, M2P_INFLOW = 1 };
+enum { M2P_FP64 = 0 };
+
+enum {
+  M2P_OK = 0,
+  M2P_EINVAL = -1,   /* bad argument or malformed input */
+  M2P_EIO = -2,      /* file missing or unwritable */
+  M2P_EHIP = -3,     /* HIP runtime error */
+  M2P_ENOMEM = -4,   /* device allocation failed */
+  M2P_ESTATE = -5    /* call not valid in the context's state */
+};
+
+/* Run parameters.  The first seven are par_List lines 1-7 (adNOC_2Pb_runKernels.cu:21-40);
+ * the last four are constants the reference hard-codes (adNOC_2Pb_kernels.cuh:16,
+ * adNOC_2Pb_kernels.cu:1710 and 2014-2017; inflow 2060) and default to those values. */
+typedef struct m2p_params {
+  double total_sim;   /* seconds */
+  double each_out;    /* seconds */
+  double delta0;      /* basal friction angle, degrees */
+  double Cd;
+  double N_R;
+  double varTheta;
+  double phiS0;
+  double cfl;         /* reference 0.1 */
+  double rho_ratio;   /* reference 1.420/2.60 */
+  double dt_min;      /* reference 0.001 (landslide), 0.002 (inflow) */
+  double dt_max;      /* reference 2.0 */
+} m2p_params;
+
+/* Fills *p with the reference's values for `mode` (par_List values of the shipped
+ * examples, constants of the kernels). */
+void m2p_default_params(int mode, m2p_params* p);
+
+/* Replaces RunKernels::RunKernels(char* fname) (adNOC_2Pb_runKernels.cu:4-9): reads the
+ * par_List at `par_list` (9 lines landslide, 11 lines inflow; '#' starts a comment) and
+ * the files it names (relative paths resolve against the current directory, as in the
+ * reference), parses them with the reference's precision rules and uploads them.
+ * `overlay` is NULL or "key=value;key=value" over cfl, rho_ratio, dt_min, dt_max. */
+int m2p_open(const char* par_list, int mode, const char* overlay, m2p_ctx** out);
+
+/* In-memory constructor (no files).  topo[NY*NX] and depth[NY*NX] hold the grid values
+ * as a file would: line j, value i at [j*NX+i], in metres, already rounded as the
+ * reference's parser rounds them (float for topography).  depth is ignored in inflow
+ * mode.  phi (nullable, landslide only) is a per-cell solid fraction replacing phiS0 at
+ * release cells.  Inflow mode takes the hydrograph tables as parsed: t[nsamp] seconds,
+ * h/u/v[nsamp*nloc] (sample-major), loc[nloc*3] = (NO, X, Y) per location. */
+int m2p_open_grid(int mode, int NX, int NY, double cellsize, const double* topo,
+                  const double* depth, const double* phi, const m2p_params* par, int nsamp,
+                  int nloc, const double* t, const double* h, const double* u,
+                  const double* v, const double* loc, m2p_ctx** out);
+
+/* Replaces RunKernels::run() (adNOC_2Pb_runKernels.cu:306-356, loop 646-1330): runs the
+ * time loop to the last output time (at most 99999 steps) writing DEM.dat, 001.dat,
+ * Info.dat, Time.dat and NNN.dat into `result_dir` in the reference's formats.
+ * *wall_s (nullable) receives the wall time of the call. */
+int m2p_run(m2p_ctx* ctx, const char* result_dir, double* wall_s);
+
+/* Advances `nsteps` full time steps (predictor + corrector) with no output clipping and no
+ * host synchronisation inside the loop.  No reference counterpart as a call: it is the
+ * body of the kernelStep loop (adNOC_2Pb_runKernels.cu:646-1330) without the file output. */
+int m2p_step(m2p_ctx* ctx, int nsteps);
+
+/* Waits for all work queued on the context's stream. */
+int m2p_sync(m2p_ctx* ctx);
+
+/* Copies the state (7*nxd*nyd doubles, layout above) and the simulated time (scaled) and
+ * step count out.  Any pointer may be NULL. */
+int m2p_get_state(m2p_ctx* ctx, double* u, double* t, int* nt);
+
+/* Overwrites the state (7*nxd*nyd doubles, all cells including ghosts). */
+int m2p_set_state(m2p_ctx* ctx, const double* u);
+
+int m2p_dims(const m2p_ctx* ctx, int* nx, int* ny, int* nxd, int* nyd);
+
+/* Total solid and fluid volume over interior cells, sum of u[0]*Detmin and u[3]*Detmin,
+ * summed in a fixed order with compensated (double-double) arithmetic. */
+int m2p_mass(m2p_ctx* ctx, double* solid, double* fluid);
+
+/* Last computed scalars: dt of the last corrector, the float-rounded max wave speed, and
+ * the output counter `iter` (adNOC_2Pb_runKernels.cu:390). */
+int m2p_scalars(m2p_ctx* ctx, double* dt, double* dmax, int* iter);
+
+/* Copies a named nxd*nyd (or k*nxd*nyd) device field out, for verification: "cvalue",
+ * "svec" (2 planes), "Detmin", "invJ11".."invJ23", "dxdxi11".."dxdxi22", "J13dxi",
+ * "J23dxi", "J33dxi", "J13det", "J23det", "J33det", "uone45" (2 planes). Returns planes
+ * copied, or a negative code. */
+int m2p_get_field(m2p_ctx* ctx, const char* name, double* out);
+
+/* Mean device time per launch of each kernel family since the last reset, measured with
+ * HIP events on the context's stream when profiling is enabled. */
+int m2p_profile(m2p_ctx* ctx, int enable);
+int m2p_profile_get(m2p_ctx* ctx, const char* kernel, double* ms_per_launch, int* launches);
+
+const char* m2p_last_error(const m2p_ctx* ctx);
+void m2p_close(m2p_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MOSES2PDF_H */

@@ -1,0 +1,597 @@
+// m2p_engine.hip -- host side of the adNOC_2Pb engine: the m2p_ctx context, input
+// preparation, the time loop driver and the C ABI declared in include/moses2pdf.h.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/moses2pdf.h"
+#include "m2p_io.h"
+#include "m2p_kernels.h"
+
+using m2p::DevParams;
+using m2p::DevState;
+
+struct m2p_ctx {
+  int mode = 0;
+  int NX = 0, NY = 0, nxd = 0, nyd = 0;
+  long long P = 0;
+  double dx = 0, dy = 0, xll = 0, yll = 0;
+  m2p_params par{};
+  std::vector<double> topo, depth, phi;  // padded, scaled (x10)
+  std::vector<double> tf;                // output times (scaled), tf[0] = 0
+  int nsamp = 0, nloc = 0;
+  std::vector<double> table, loc, itime;
+  std::vector<int> dire;
+  hipStream_t s = nullptr;
+  double *dTopo = nullptr, *dDepth = nullptr, *dPhi = nullptr, *dM = nullptr;
+  double *dU0 = nullptr, *dU1 = nullptr, *dONE = nullptr, *dTf = nullptr, *dMass = nullptr;
+  double *dTable = nullptr, *dLoc = nullptr, *dItime = nullptr;
+  int* dDire = nullptr;
+  DevState* dSt = nullptr;
+  m2p::Grid g{};
+  m2p::Phys ph{};
+  std::string err;
+  bool prof = false;
+  std::map<std::string, std::pair<double, int>> pstats;
+};
+
+namespace {
+
+constexpr int MD = m2p::MD;
+
+int fail(m2p_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                                  \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return fail((ctx), M2P_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+// dataInitialization (L/R:216-301): x10, clamp < 0 to 0, edge copies into the ghosts,
+// transposed to [row*nxd+col] (L/R:334-341).
+std::vector<double> pad_plane(int NX, int NY, const double* raw, bool scale = true) {
+  const int nxd = NX + 2 * MD, nyd = NY + 2 * MD;
+  std::vector<double> a((size_t)nxd * nyd, 0.0);
+  auto at = [&](int i, int j) -> double& { return a[(size_t)j * nxd + i]; };
+  if (raw)
+    for (int j = 0; j < NY; j++)
+      for (int i = 0; i < NX; i++) {
+        double v = raw[(size_t)j * NX + i];
+        if (scale) v = 10.0 * v;
+        if (v < 0) v = 0;
+        at(i + MD, j + MD) = v;
+      }
+  for (int i = 0; i < MD; i++)
+    for (int j = MD; j < NY + MD; j++) {
+      at(i, j) = at(MD, j);
+      at(NX + MD + i, j) = at(NX + MD - 1, j);
+    }
+  for (int j = 0; j < MD; j++)
+    for (int i = 0; i < NX + 2 * MD; i++) {
+      at(i, j) = at(i, MD);
+      at(i, NY + MD + j) = at(i, NY + MD - 1);
+    }
+  return a;
+}
+
+// bfkt planes 0 and 1 (makeTopo1Kernel/makeTopo2Kernel, L/K:5-56), for DEM.dat.
+double coord_host(int c, int n, double d) {
+  const double xmin = 0.0;
+  if (c < MD) return (xmin + 0 * d) - (MD - c) * d;
+  if (c >= n + MD) return (xmin + (n - 1) * d) + (c - (n + MD - 1)) * d;
+  return xmin + (c - MD) * d;
+}
+
+template <class T>
+int dalloc(m2p_ctx* c, T** p, size_t n) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) return fail(c, M2P_ENOMEM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed");
+  e = hipMemsetAsync(*p, 0, n * sizeof(T), c->s);
+  if (e != hipSuccess) return fail(c, M2P_EHIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int upload(m2p_ctx* c, double* d, const std::vector<double>& h) {
+  if (h.empty()) return 0;
+  HIPCHK(c, hipMemcpyAsync(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->s));
+  return 0;
+}
+
+void release(m2p_ctx* c) {
+  void* ps[] = {c->dTopo, c->dDepth, c->dPhi, c->dM,     c->dU0,   c->dU1,  c->dONE,
+                c->dTf,   c->dMass,  c->dTable, c->dLoc, c->dItime, c->dDire, c->dSt};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  if (c->s) (void)hipStreamDestroy(c->s);
+  c->s = nullptr;
+}
+
+// Device allocation, uploads, metric setup (L/R:395-642) and the initial state.
+int setup(m2p_ctx* c) {
+  HIPCHK(c, hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+  HIPCHK(c, m2p::probe_kernels());
+  const size_t P = (size_t)c->P;
+  int rc;
+  if ((rc = dalloc(c, &c->dTopo, P)) || (rc = dalloc(c, &c->dDepth, P)) ||
+      (rc = dalloc(c, &c->dM, (size_t)m2p::MC_COUNT * P)) || (rc = dalloc(c, &c->dU0, 7 * P)) ||
+      (rc = dalloc(c, &c->dU1, 7 * P)) || (rc = dalloc(c, &c->dONE, 2 * P)) ||
+      (rc = dalloc(c, &c->dTf, c->tf.size())) || (rc = dalloc(c, &c->dMass, 4 * (size_t)m2p::mass_blocks())) ||
+      (rc = dalloc(c, &c->dSt, 1)))
+    return rc;
+  if (!c->phi.empty() && (rc = dalloc(c, &c->dPhi, P))) return rc;
+  if (c->mode == 1) {
+    if ((rc = dalloc(c, &c->dTable, c->table.size())) || (rc = dalloc(c, &c->dLoc, c->loc.size())) ||
+        (rc = dalloc(c, &c->dItime, c->itime.size())) || (rc = dalloc(c, &c->dDire, c->dire.size())))
+      return rc;
+    if ((rc = upload(c, c->dTable, c->table)) || (rc = upload(c, c->dLoc, c->loc)) ||
+        (rc = upload(c, c->dItime, c->itime)))
+      return rc;
+    HIPCHK(c, hipMemcpyAsync(c->dDire, c->dire.data(), c->dire.size() * sizeof(int),
+                             hipMemcpyHostToDevice, c->s));
+  }
+  if ((rc = upload(c, c->dTopo, c->topo)) || (rc = upload(c, c->dDepth, c->depth)) ||
+      (rc = upload(c, c->dTf, c->tf)))
+    return rc;
+  if (c->dPhi && (rc = upload(c, c->dPhi, c->phi))) return rc;
+
+  c->g.nx = c->NX;
+  c->g.ny = c->NY;
+  c->g.nxd = c->nxd;
+  c->g.nyd = c->nyd;
+  c->g.P = c->P;
+  c->g.dx = c->dx;
+  c->g.dy = c->dy;
+  c->ph.rr = c->par.rho_ratio;
+  c->ph.Drtio1 = 1.0 - c->par.rho_ratio;
+  c->ph.Cd = c->par.Cd;
+  c->ph.N_R = c->par.N_R;
+  c->ph.varTheta = c->par.varTheta;
+  c->ph.tande = std::tan(c->par.delta0 * 3.14159 / 180.);  // makeTopo11Kernel, L/K:298
+
+  DevState st{};
+  st.iter = 1;
+  HIPCHK(c, hipMemcpyAsync(c->dSt, &st, sizeof st, hipMemcpyHostToDevice, c->s));
+  m2p::launch_metric_setup(c->g, c->mode, c->dTopo, c->dDepth, c->dPhi, c->par.phiS0, c->dM,
+                           c->dU0, c->s);
+  HIPCHK(c, hipGetLastError());
+  // the first predictor's corner ghosts come from the initial state's own ghost rows
+  HIPCHK(c, hipMemcpyAsync(c->dU1, c->dU0, 7 * P * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  return 0;
+}
+
+struct Timer {
+  m2p_ctx* c;
+  const char* name;
+  hipEvent_t a = nullptr, b = nullptr;
+  Timer(m2p_ctx* c_, const char* n) : c(c_), name(n) {
+    if (c->prof) {
+      (void)hipEventCreate(&a);
+      (void)hipEventCreate(&b);
+      (void)hipEventRecord(a, c->s);
+    }
+  }
+  ~Timer() {
+    if (c->prof) {
+      (void)hipEventRecord(b, c->s);
+      (void)hipEventSynchronize(b);
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, a, b);
+      auto& e = c->pstats[name];
+      e.first += ms;
+      e.second += 1;
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+    }
+  }
+};
+
+// One pass of the kernelStep loop body (L/R:646-1330; inflow I/R:721-1424).
+void enqueue_step(m2p_ctx* c, int ntf) {
+  DevParams dp{};
+  dp.cfl = c->par.cfl;
+  dp.dt_min = c->par.dt_min;
+  dp.dt_max = c->par.dt_max;
+  dp.ntf = ntf;
+  dp.nsamp = c->mode == 1 ? c->nsamp : 0;
+  if (c->mode == 1) {
+    Timer t(c, "inflow");
+    m2p::launch_inflow(c->g, c->dU0, c->dM, c->dTable, c->dLoc, c->dDire, c->nloc, c->nsamp,
+                       c->par.phiS0, c->dSt, c->s);
+  }
+  {
+    Timer t(c, "ghost");
+    m2p::launch_ghost(c->g, c->mode, c->dU0, c->dU1, c->s);
+  }
+  {
+    Timer t(c, "cfl");
+    m2p::launch_cfl(c->g, c->ph, c->dU0, c->dM, c->dSt, c->s);
+  }
+  {
+    Timer t(c, "dt");
+    m2p::launch_dt(dp, c->dSt, c->dTf, c->dItime, c->s);
+  }
+  {
+    Timer t(c, "stage0");
+    m2p::launch_stage(0, c->g, c->ph, c->dU0, c->dU0, c->dU1, c->dONE, c->dM, c->dSt, c->s);
+  }
+  {
+    Timer t(c, "ghost");
+    m2p::launch_ghost(c->g, c->mode, c->dU1, c->dU0, c->s);
+  }
+  {
+    Timer t(c, "stage1");
+    m2p::launch_stage(1, c->g, c->ph, c->dU1, c->dU0, c->dU0, c->dONE, c->dM, c->dSt, c->s);
+  }
+}
+
+int parse_overlay(const char* ov, m2p_params& p) {
+  if (!ov) return 0;
+  std::string s(ov);
+  size_t pos = 0;
+  while (pos < s.size()) {
+    size_t end = s.find_first_of(";,", pos);
+    if (end == std::string::npos) end = s.size();
+    std::string kv = s.substr(pos, end - pos);
+    pos = end + 1;
+    if (kv.empty()) continue;
+    size_t eq = kv.find('=');
+    if (eq == std::string::npos) return -1;
+    std::string k = kv.substr(0, eq);
+    double v = std::strtod(kv.c_str() + eq + 1, nullptr);
+    if (k == "cfl") p.cfl = v;
+    else if (k == "rho_ratio") p.rho_ratio = v;
+    else if (k == "dt_min") p.dt_min = v;
+    else if (k == "dt_max") p.dt_max = v;
+    else return -1;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+void m2p_default_params(int mode, m2p_params* p) {
+  // par_List of the shipped examples (MoSES_2PDF_landslide/par_List:1-7) and the kernel
+  // constants (adNOC_2Pb_kernels.cuh:16; adNOC_2Pb_kernels.cu:1710, 2014-2017).
+  p->total_sim = mode == 0 ? 180 : 720;
+  p->each_out = mode == 0 ? 3 : 10;
+  p->delta0 = 16.0;
+  p->Cd = 6.0;
+  p->N_R = 268.;
+  p->varTheta = 5.0;
+  p->phiS0 = 0.5;
+  p->cfl = 0.1;
+  p->rho_ratio = 1.420 / 2.60;
+  p->dt_min = mode == 0 ? 0.001 : 0.002;
+  p->dt_max = 2.0;
+}
+
+int m2p_open_grid(int mode, int NX, int NY, double cellsize, const double* topo,
+                  const double* depth, const double* phi, const m2p_params* par, int nsamp,
+                  int nloc, const double* t, const double* h, const double* u, const double* v,
+                  const double* loc, m2p_ctx** out) {
+  if (!out) return M2P_EINVAL;
+  *out = nullptr;
+  if ((mode != 0 && mode != 1) || NX < 2 || NY < 2 || !topo || !par || !(cellsize > 0))
+    return M2P_EINVAL;
+  if (mode == 0 && !depth) return M2P_EINVAL;
+  if (mode == 1 && (nsamp < 1 || nloc < 1 || !t || !h || !u || !v || !loc)) return M2P_EINVAL;
+  m2p_ctx* c = new m2p_ctx();
+  c->mode = mode;
+  c->NX = NX;
+  c->NY = NY;
+  c->nxd = NX + 2 * MD;
+  c->nyd = NY + 2 * MD;
+  c->P = (long long)c->nxd * c->nyd;
+  c->dx = cellsize * 10;  // L/R:181-182
+  c->dy = cellsize * 10;
+  c->par = *par;
+  c->topo = pad_plane(NX, NY, topo);
+  c->depth = (mode == 0) ? pad_plane(NX, NY, depth) : std::vector<double>((size_t)c->P, 0.0);
+  if (phi && mode == 0) {
+    // per-cell solid fraction: padded like the depth, but not scaled
+    c->phi = pad_plane(NX, NY, phi, false);
+  }
+  // output times tf[] (L/R:382-387)
+  const int outputStep = (par->each_out > 0) ? (int)(par->total_sim / par->each_out) : 0;
+  c->tf.assign(outputStep + 1, 0.0);
+  for (int ii = 1; ii <= outputStep; ii++) c->tf[ii] = ii * par->each_out * 10;
+  if (mode == 1) {
+    // hydrograph tables (I/R:297-331, 394-428)
+    c->nsamp = nsamp;
+    c->nloc = nloc;
+    c->itime.resize(nsamp);
+    c->table.assign((size_t)3 * nsamp * nloc, 0.0);
+    for (int j = 0; j < nsamp; j++) {
+      c->itime[j] = (double)((float)t[j] * 10);
+      for (int i = 0; i < nloc; i++) {
+        const size_t q = (size_t)j * nloc + i;
+        c->table[0 * (size_t)nsamp * nloc + q] = 10.0 * h[q];
+        c->table[1 * (size_t)nsamp * nloc + q] = u[q];
+        c->table[2 * (size_t)nsamp * nloc + q] = v[q];
+      }
+    }
+    c->loc.assign(loc, loc + (size_t)3 * nloc);
+    c->dire.assign((size_t)3 * nloc, 0);
+    for (int j = 0; j < nloc; j++) {
+      const double Y = loc[j * 3 + 2];
+      c->dire[j * 3 + 0] = (int)loc[j * 3 + 0];
+      if (Y == 0 || Y == NY) {
+        c->dire[j * 3 + 1] = -5;
+        c->dire[j * 3 + 2] = 0;
+      } else {
+        c->dire[j * 3 + 1] = 0;
+        c->dire[j * 3 + 2] = -5;
+      }
+      // the injected strip must lie inside the padded grid
+      const int X0 = (int)loc[j * 3 + 1], Y0 = (int)Y;
+      const int w = c->dire[j * 3 + 1] + MD + 3, hgt = c->dire[j * 3 + 2] + MD + 3;
+      if (X0 < 0 || Y0 < 0 || X0 + w > c->nxd || Y0 + hgt > c->nyd) {
+        delete c;
+        return M2P_EINVAL;
+      }
+    }
+  }
+  int rc = setup(c);
+  if (rc) {
+    // keep the context so the caller can read the message
+    release(c);
+    *out = c;
+    return rc;
+  }
+  *out = c;
+  return 0;
+}
+
+int m2p_open(const char* par_list, int mode, const char* overlay, m2p_ctx** out) {
+  if (!out || !par_list || (mode != 0 && mode != 1)) return M2P_EINVAL;
+  *out = nullptr;
+  static thread_local m2p_ctx errctx;
+  m2p::ParList pl;
+  std::string e = m2p::read_par_list(par_list, mode, pl);
+  auto bad = [&](int code, const std::string& msg) {
+    m2p_ctx* c = new m2p_ctx();
+    c->err = msg;
+    *out = c;
+    return code;
+  };
+  if (!e.empty()) return bad(M2P_EIO, e);
+  m2p::AscGrid dem;
+  e = m2p::read_asc(pl.dem, true, dem);
+  if (!e.empty()) return bad(M2P_EIO, e);
+  m2p_params p;
+  m2p_default_params(mode, &p);
+  p.total_sim = pl.TotalSim;
+  p.each_out = pl.EachOut;
+  p.delta0 = pl.delta0;
+  p.Cd = pl.Cd;
+  p.N_R = pl.N_R;
+  p.varTheta = pl.varTheta;
+  p.phiS0 = pl.phiS0;
+  if (parse_overlay(overlay, p)) return bad(M2P_EINVAL, "bad overlay string");
+  int rc;
+  m2p_ctx* c = nullptr;
+  if (mode == 0) {
+    std::vector<double> depth;
+    e = m2p::read_asc_values(pl.init, dem.NX, dem.NY, false, depth);
+    if (!e.empty()) return bad(M2P_EIO, e);
+    rc = m2p_open_grid(0, dem.NX, dem.NY, dem.cellsize, dem.v.data(), depth.data(), nullptr, &p, 0,
+                       0, nullptr, nullptr, nullptr, nullptr, nullptr, &c);
+  } else {
+    m2p::Hydro hy;
+    e = m2p::read_hydro(pl, hy);
+    if (!e.empty()) return bad(M2P_EIO, e);
+    rc = m2p_open_grid(1, dem.NX, dem.NY, dem.cellsize, dem.v.data(), nullptr, nullptr, &p, hy.nsamp,
+                       hy.nloc, hy.t.data(), hy.h.data(), hy.u.data(), hy.v.data(), hy.loc.data(), &c);
+  }
+  (void)errctx;
+  if (!c) return bad(rc ? rc : M2P_EINVAL, "invalid grid or inflow table");
+  c->xll = dem.xll;
+  c->yll = dem.yll;
+  *out = c;
+  return rc;
+}
+
+int m2p_step(m2p_ctx* c, int nsteps) {
+  if (!c || !c->s) return M2P_ESTATE;
+  if (nsteps < 0) return fail(c, M2P_EINVAL, "negative step count");
+  for (int n = 0; n < nsteps; n++) enqueue_step(c, 0);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+int m2p_sync(m2p_ctx* c) {
+  if (!c || !c->s) return M2P_ESTATE;
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  return 0;
+}
+
+int m2p_run(m2p_ctx* c, const char* result_dir, double* wall_s) {
+  if (!c || !c->s || !result_dir) return M2P_ESTATE;
+  const auto t0 = std::chrono::steady_clock::now();
+  const std::string dir(result_dir);
+  const size_t P = (size_t)c->P;
+  m2p::OutputGrid og{c->NX, c->NY, c->nxd, c->nyd};
+  {
+    std::vector<double> bx(P), by(P), cv(P), s0(P), s1(P);
+    for (int r = 0; r < c->nyd; r++)
+      for (int q = 0; q < c->nxd; q++) {
+        bx[(size_t)r * c->nxd + q] = coord_host(q, c->NX, c->dx);
+        by[(size_t)r * c->nxd + q] = coord_host(r, c->NY, c->dy);
+      }
+    HIPCHK(c, hipMemcpy(cv.data(), c->dM + m2p::MC_C * P, P * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(s0.data(), c->dM + m2p::MC_S0 * P, P * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(s1.data(), c->dM + m2p::MC_S1 * P, P * sizeof(double), hipMemcpyDeviceToHost));
+    std::string e = m2p::write_dem(dir, og, bx, by, c->topo, cv, s0, s1);
+    if (e.empty()) e = m2p::write_init(dir, og, c->depth, c->par.phiS0);
+    if (e.empty()) e = m2p::write_info(dir, c->NX, c->NY, c->dx, c->dy, c->xll, c->yll, (int)c->tf.size());
+    if (!e.empty()) return fail(c, M2P_EIO, e);
+  }
+  const int ntf = (int)c->tf.size();
+  std::vector<double> outtime(ntf + 1, 0.0);
+  std::vector<double> u(7 * P);
+  DevState st{};
+  int nt;
+  bool stop = (ntf <= 1);
+  for (nt = 1; !stop && nt < 100000; nt++) {
+    enqueue_step(c, ntf);
+    HIPCHK(c, hipMemcpyAsync(&st, c->dSt, sizeof st, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (st.schreiben) {
+      HIPCHK(c, hipMemcpy(u.data(), c->dU0, 7 * P * sizeof(double), hipMemcpyDeviceToHost));
+      outtime[st.iter - 1] = st.step_start;
+      std::string e = m2p::write_time(dir, outtime, st.iter);
+      if (e.empty()) e = m2p::write_result(dir, st.iter, og, u.data());
+      if (!e.empty()) return fail(c, M2P_EIO, e);
+      if (st.iter == ntf) stop = true;
+      const int zero = 0;
+      HIPCHK(c, hipMemcpyAsync((char*)c->dSt + offsetof(DevState, schreiben), &zero, sizeof zero,
+                               hipMemcpyHostToDevice, c->s));
+    }
+    std::fprintf(stdout, "\r\t\tSimulation progress: %3.0f %%",
+                 (st.step_start * 0.1 / c->par.total_sim) * 100);
+    std::fflush(stdout);
+  }
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  std::printf("\nTotal time : %g sec  ", st.time / 10);
+  std::printf("\nTotal number of steps: %d\n", nt);
+  if (wall_s)
+    *wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
+int m2p_get_state(m2p_ctx* c, double* u, double* t, int* nt) {
+  if (!c || !c->s) return M2P_ESTATE;
+  DevState st{};
+  HIPCHK(c, hipMemcpyAsync(&st, c->dSt, sizeof st, hipMemcpyDeviceToHost, c->s));
+  if (u) HIPCHK(c, hipMemcpyAsync(u, c->dU0, 7 * (size_t)c->P * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  if (t) *t = st.time;
+  if (nt) *nt = st.nt;
+  return 0;
+}
+
+int m2p_set_state(m2p_ctx* c, const double* u) {
+  if (!c || !c->s) return M2P_ESTATE;
+  if (!u) return fail(c, M2P_EINVAL, "null state");
+  const size_t n = 7 * (size_t)c->P * sizeof(double);
+  HIPCHK(c, hipMemcpyAsync(c->dU0, u, n, hipMemcpyHostToDevice, c->s));
+  HIPCHK(c, hipMemcpyAsync(c->dU1, c->dU0, n, hipMemcpyDeviceToDevice, c->s));
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  return 0;
+}
+
+int m2p_dims(const m2p_ctx* c, int* nx, int* ny, int* nxd, int* nyd) {
+  if (!c) return M2P_ESTATE;
+  if (nx) *nx = c->NX;
+  if (ny) *ny = c->NY;
+  if (nxd) *nxd = c->nxd;
+  if (nyd) *nyd = c->nyd;
+  return 0;
+}
+
+int m2p_mass(m2p_ctx* c, double* solid, double* fluid) {
+  if (!c || !c->s) return M2P_ESTATE;
+  const int nb = m2p::mass_blocks();
+  std::vector<double> part(4 * (size_t)nb);
+  m2p::launch_mass(c->g, c->dU0, c->dM, c->dMass, c->s);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(part.data(), c->dMass, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  // double-double sum of the block partials, in block order
+  auto add = [](double& hi, double& lo, double x) {
+    double s = hi + x, bb = s - hi, e = (hi - (s - bb)) + (x - bb);
+    e += lo;
+    hi = s + e;
+    lo = e - (hi - s);
+  };
+  double a = 0, al = 0, b = 0, bl = 0;
+  for (int i = 0; i < nb; i++) {
+    add(a, al, part[4 * i + 0]);
+    add(a, al, part[4 * i + 1]);
+    add(b, bl, part[4 * i + 2]);
+    add(b, bl, part[4 * i + 3]);
+  }
+  if (solid) *solid = a + al;
+  if (fluid) *fluid = b + bl;
+  return 0;
+}
+
+int m2p_scalars(m2p_ctx* c, double* dt, double* dmax, int* iter) {
+  if (!c || !c->s) return M2P_ESTATE;
+  DevState st{};
+  HIPCHK(c, hipMemcpyAsync(&st, c->dSt, sizeof st, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  if (dt) *dt = st.dt_corr;
+  if (dmax) *dmax = st.dmax;
+  if (iter) *iter = st.iter;
+  return 0;
+}
+
+int m2p_get_field(m2p_ctx* c, const char* name, double* out) {
+  if (!c || !c->s) return M2P_ESTATE;
+  if (!name || !out) return fail(c, M2P_EINVAL, "null argument");
+  static const std::map<std::string, std::pair<int, int>> tab = {
+      {"cvalue", {m2p::MC_C, 1}},    {"svec", {m2p::MC_S0, 2}},     {"Detmin", {m2p::MC_DET, 1}},
+      {"invJ11", {m2p::MC_I11, 1}},  {"invJ12", {m2p::MC_I12, 1}},  {"invJ13", {m2p::MC_I13, 1}},
+      {"invJ21", {m2p::MC_I21, 1}},  {"invJ22", {m2p::MC_I22, 1}},  {"invJ23", {m2p::MC_I23, 1}},
+      {"dxdxi11", {m2p::MC_X11, 1}}, {"dxdxi12", {m2p::MC_X12, 1}}, {"dxdxi21", {m2p::MC_X21, 1}},
+      {"dxdxi22", {m2p::MC_X22, 1}}, {"J13dxi", {m2p::MC_J13X, 1}}, {"J23dxi", {m2p::MC_J23X, 1}},
+      {"J33dxi", {m2p::MC_J33X, 1}}, {"J13det", {m2p::MC_J13E, 1}}, {"J23det", {m2p::MC_J23E, 1}},
+      {"J33det", {m2p::MC_J33E, 1}}};
+  const size_t P = (size_t)c->P;
+  std::string n(name);
+  if (n == "uone45") {
+    HIPCHK(c, hipMemcpy(out, c->dONE, 2 * P * sizeof(double), hipMemcpyDeviceToHost));
+    return 2;
+  }
+  auto it = tab.find(n);
+  if (it == tab.end()) return fail(c, M2P_EINVAL, "unknown field " + n);
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  HIPCHK(c, hipMemcpy(out, c->dM + (size_t)it->second.first * P,
+                      (size_t)it->second.second * P * sizeof(double), hipMemcpyDeviceToHost));
+  return it->second.second;
+}
+
+int m2p_profile(m2p_ctx* c, int enable) {
+  if (!c) return M2P_ESTATE;
+  c->prof = enable != 0;
+  c->pstats.clear();
+  return 0;
+}
+
+int m2p_profile_get(m2p_ctx* c, const char* kernel, double* ms, int* launches) {
+  if (!c || !kernel) return M2P_ESTATE;
+  auto it = c->pstats.find(kernel);
+  if (it == c->pstats.end()) {
+    if (ms) *ms = 0;
+    if (launches) *launches = 0;
+    return 0;
+  }
+  if (ms) *ms = it->second.second ? it->second.first / it->second.second : 0.0;
+  if (launches) *launches = it->second.second;
+  return 0;
+}
+
+const char* m2p_last_error(const m2p_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+void m2p_close(m2p_ctx* c) {
+  if (!c) return;
+  release(c);
+  delete c;
+}
+
+}  // extern "C"

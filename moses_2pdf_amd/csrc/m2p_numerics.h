@@ -1,0 +1,408 @@
+// m2p_numerics.h -- per-face and per-cell numerics of the adNOC_2Pb scheme as device
+// functions, shared by the CFL pass and the stage kernels.
+//
+// Every expression keeps the operand order of the reference kernel it restates, and the
+// library is compiled with -ffp-contract=off, so results are bit-identical to a serial
+// evaluation of MoSES_2PDF_landslide/adNOC_2Pb_kernels.cu (cited as L/K) in IEEE FP64.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace m2p {
+
+constexpr int MD = 3;             // ghost width, L/KH:7
+constexpr int MN = 7;             // state components, L/KH:8
+constexpr double H0 = 1.0e-6;     // L/KH:11
+constexpr double H00 = 0.0;       // L/KH:12
+constexpr double U0 = 1.0e-6;     // L/KH:13
+
+// Metric planes kept on the device (cell centred, padded nxd*nyd each).  invJ31, invJ32
+// and invJ33 are exactly -svec0, -svec1 and cvalue (L/K:244-248), so they are not stored.
+enum MetricPlane {
+  MC_C = 0, MC_S0, MC_S1, MC_DET,
+  MC_I11, MC_I12, MC_I13, MC_I21, MC_I22, MC_I23,
+  MC_X11, MC_X12, MC_X21, MC_X22,
+  MC_J13X, MC_J23X, MC_J33X, MC_J13E, MC_J23E, MC_J33E,
+  MC_COUNT
+};
+
+struct Grid {
+  int nx, ny, nxd, nyd;
+  long long P;          // nxd*nyd
+  double dx, dy;
+};
+
+struct Phys {
+  double rr;        // density ratio (1.42/2.60 in the reference)
+  double Drtio1;    // 1 - rr
+  double Cd, N_R, varTheta;
+  double tande;     // tan(delta0*3.14159/180), one value for the whole plane (L/K:298)
+};
+
+__device__ __forceinline__ long long idx(const Grid& g, int r, int c) {
+  return (long long)r * g.nxd + c;
+}
+
+// TVD1..5Kernel (L/K:1000-1119) for one component: slope at a cell from its left/right
+// (or lower/upper) neighbours.  dA = dux[c-1], dB = dux[c].
+__device__ __forceinline__ double tvd_slope(double um, double u0, double up) {
+  const double dA = u0 - um;
+  const double dB = up - u0;
+  const double sA = (dA > 0.0) ? 1.0 : -1.0;
+  const double sB = (dB > 0.0) ? 1.0 : -1.0;
+  const double t1 = 0.5 * (sA + sB) * fmin(dA * sA, dB * sB);
+  const double t2 = 0.5 * (dA + dB);
+  const double sA2 = (t1 > 0.0) ? 1.0 : -1.0;
+  const double sB2 = (t2 > 0.0) ? 1.0 : -1.0;
+  return 0.5 * (sA2 + sB2) * fmin(t1 * sA2, t2 * sB2);
+}
+
+// Interface states of InterfacesKernel (L/K:1226-1252) for one face, components 0..5
+// (component 6 of a face state is always replaced by KeepPositivi2Kernel).
+//   dir 0: x-face between (r,c) and (r,c+1): L = W state, R = E state.
+//   dir 1: y-face between (r,c) and (r+1,c): L = S state, R = N state.
+__device__ __forceinline__ void base_states(const double* __restrict__ U, const Grid& g, int dir,
+                                            int r, int c, double L[6], double R[6]) {
+  const long long st = dir == 0 ? 1 : g.nxd;
+  const long long i = idx(g, r, c);
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const double* a = U + k * g.P;
+    const double um = a[i - st], u0 = a[i], u1 = a[i + st], u2 = a[i + 2 * st];
+    const double s0 = tvd_slope(um, u0, u1);
+    const double s1 = tvd_slope(u0, u1, u2);
+    L[k] = u0 + 0.5 * s0;
+    R[k] = u1 - 0.5 * s1;
+  }
+}
+
+// Face states after Interfaces2Kernel (L/K:1254-1337), taken as: all x-ghost overrides
+// (columns 0-2 from 3,4; columns nx+3.. from nx+2,nx+1) from the InterfacesKernel values,
+// then all y-ghost overrides (rows 0-2 from 3,4; rows ny+3.. from ny+2,ny+1) from the
+// x-overridden values.  Valid for 1 <= r,c <= N-3.
+__device__ __forceinline__ void x_overridden(const double* __restrict__ U, const Grid& g, int dir,
+                                             int r, int c, double L[6], double R[6]) {
+  if (c <= 2 || c >= g.nx + MD) {
+    const int ca = (c <= 2) ? MD : g.nx + MD - 1;
+    const int cb = (c <= 2) ? MD + 1 : g.nx + MD - 2;
+    double La[6], Ra[6], Lb[6], Rb[6];
+    base_states(U, g, dir, r, ca, La, Ra);
+    base_states(U, g, dir, r, cb, Lb, Rb);
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      L[k] = 0.5 * (La[k] + Lb[k]);
+      R[k] = 0.5 * (Ra[k] + Rb[k]);
+    }
+  } else {
+    base_states(U, g, dir, r, c, L, R);
+  }
+}
+
+__device__ __forceinline__ void final_states(const double* __restrict__ U, const Grid& g, int dir,
+                                             int r, int c, double L[6], double R[6]) {
+  if (r <= 2 || r >= g.ny + MD) {
+    const int ra = (r <= 2) ? MD : g.ny + MD - 1;
+    const int rb = (r <= 2) ? MD + 1 : g.ny + MD - 2;
+    double La[6], Ra[6], Lb[6], Rb[6];
+    x_overridden(U, g, dir, ra, c, La, Ra);
+    x_overridden(U, g, dir, rb, c, Lb, Rb);
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      L[k] = 0.5 * (La[k] + Lb[k]);
+      R[k] = 0.5 * (Ra[k] + Rb[k]);
+    }
+  } else {
+    x_overridden(U, g, dir, r, c, L, R);
+  }
+}
+
+// KeepPositivi1Kernel + KeepPositivi2Kernel (L/K:1339-1424) on one face state.
+__device__ __forceinline__ double keep_positive(double a[6]) {
+  if (a[0] <= H00) {
+    a[0] = 0.0;
+    a[1] = 0.0;
+    a[2] = 0.0;
+  }
+  if (a[3] <= H00) {
+    a[3] = 0.0;
+    a[4] = 0.0;
+    a[5] = 0.0;
+  }
+  return a[0] / (a[0] + a[3] + H0);
+}
+
+// Face-averaged metric of MeanKernel (L/K:1121-1224) plus the cell values Flux2Kernel
+// reads on either side (L/K:1506-1536).
+struct FaceMet {
+  double j11, j12, j13, j21, j22, j23, j31, j32, j33;  // invJ*_avg
+  double det, cav;                                    // Detmin_avg, cval_avg
+  double xa, xb;     // x-face: dxdxi11_avgEW, dxdxi21_avgEW; y-face: dxdxi12_avgSN, dxdxi22_avgSN
+  double sR0, sR1, cR;   // svec, cvalue of the E (x) / N (y) cell
+  double sL0, sL1, cL;   // of the W / S cell
+};
+
+__device__ __forceinline__ double avg(const double* a, long long i, long long j) {
+  return 0.5 * (a[i] + a[j]);
+}
+
+// dir 0: face (r,c)-(r,c+1); dir 1: face (r,c)-(r+1,c).  Valid for faces in [1,N-3]^2,
+// where MeanKernel writes the averages.
+__device__ __forceinline__ void face_metric(const double* __restrict__ M, const Grid& g, int dir,
+                                            int r, int c, FaceMet& f) {
+  const long long i = idx(g, r, c);
+  const long long j = i + (dir == 0 ? 1 : g.nxd);
+  const long long P = g.P;
+  const double* C = M + MC_C * P;
+  const double* S0 = M + MC_S0 * P;
+  const double* S1 = M + MC_S1 * P;
+  f.j11 = avg(M + MC_I11 * P, i, j);
+  f.j12 = avg(M + MC_I12 * P, i, j);
+  f.j13 = avg(M + MC_I13 * P, i, j);
+  f.j21 = avg(M + MC_I21 * P, i, j);
+  f.j22 = avg(M + MC_I22 * P, i, j);
+  f.j23 = avg(M + MC_I23 * P, i, j);
+  f.sL0 = S0[i];
+  f.sL1 = S1[i];
+  f.cL = C[i];
+  f.sR0 = S0[j];
+  f.sR1 = S1[j];
+  f.cR = C[j];
+  f.j31 = 0.5 * ((-f.sL0) + (-f.sR0));
+  f.j32 = 0.5 * ((-f.sL1) + (-f.sR1));
+  f.j33 = 0.5 * (f.cL + f.cR);
+  f.cav = f.j33;
+  f.det = avg(M + MC_DET * P, i, j);
+  if (dir == 0) {
+    f.xa = avg(M + MC_X11 * P, i, j);
+    f.xb = avg(M + MC_X21 * P, i, j);
+  } else {
+    f.xa = avg(M + MC_X12 * P, i, j);
+    f.xb = avg(M + MC_X22 * P, i, j);
+  }
+}
+
+// Wave speeds of one face (Flux1..6Kernel, L/K:1426-1848): returns em for the solid
+// (emS) and the fluid (emF) after the 1e-10 floor.  L/R are final states; they leave with
+// positivity applied.  Also returns the quantities the flux needs.
+struct FaceSpeed {
+  double phiL, phiR;
+  double qL, qR, qFL, qFR;     // normal contravariant speeds (q_xi for x, q_et for y)
+  double ap, apF;              // apEW/apSN, apFEW/apFSN
+  double emS, emF;
+};
+
+__device__ __forceinline__ void face_speed(int dir, double L[6], double R[6], const FaceMet& f,
+                                           const Phys& ph, FaceSpeed& o) {
+  o.phiR = keep_positive(R);
+  o.phiL = keep_positive(L);
+  // Flux1Kernel L/K:1448-1470
+  const double vxR = R[1] / (R[0] + H0), vyR = R[2] / (R[0] + H0);
+  const double vxL = L[1] / (L[0] + H0), vyL = L[2] / (L[0] + H0);
+  const double vxFR = R[4] / (R[3] + H0), vyFR = R[5] / (R[3] + H0);
+  const double vxFL = L[4] / (L[3] + H0), vyFL = L[5] / (L[3] + H0);
+  // Flux2Kernel L/K:1506-1536
+  const double wR = (f.sR0 * vxR + f.sR1 * vyR) / f.cR;
+  const double wL = (f.sL0 * vxL + f.sL1 * vyL) / f.cL;
+  const double wFR = (f.sR0 * vxFR + f.sR1 * vyFR) / f.cR;
+  const double wFL = (f.sL0 * vxFL + f.sL1 * vyFL) / f.cL;
+  // Flux3Kernel L/K:1593-1676 (only the normal component enters the flux)
+  double a1, a2, a3, an;
+  if (dir == 0) {
+    a1 = f.j11; a2 = f.j12; a3 = f.j13; an = f.j11;
+  } else {
+    a1 = f.j21; a2 = f.j22; a3 = f.j23; an = f.j22;
+  }
+  o.qR = a1 * vxR + a2 * vyR + a3 * wR;
+  o.qL = a1 * vxL + a2 * vyL + a3 * wL;
+  o.qFR = a1 * vxFR + a2 * vyFR + a3 * wFR;
+  o.qFL = a1 * vxFL + a2 * vyFL + a3 * wFL;
+  const double NpR = (R[0] + R[3]) * f.cav;
+  const double NpL = (L[0] + L[3]) * f.cav;
+  // Flux4Kernel L/K:1715-1737
+  const double apR = fabs(o.qR) + an * sqrt(NpR * ph.Drtio1);
+  const double apL = fabs(o.qL) + an * sqrt(NpL * ph.Drtio1);
+  const double apFR = fabs(o.qFR) + an * sqrt(NpR / (1.0 - o.phiR + H0));
+  const double apFL = fabs(o.qFL) + an * sqrt(NpL / (1.0 - o.phiL + H0));
+  // Flux5Kernel L/K:1762-1766 (max(E, W) / max(N, S))
+  o.ap = fmax(apR, apL);
+  o.apF = fmax(apFR, apFL);
+  // Flux6Kernel L/K:1803-1807
+  o.emS = fmax(1.0e-10, o.ap);
+  o.emF = fmax(1.0e-10, o.apF);
+}
+
+// Numerical flux of one face: Hp (6), Pp (components 1,2,4,5) and PD (4,5).
+struct FaceFlux {
+  double H[6];
+  double P1, P2, P4, P5;
+  double D4, D5;
+};
+
+// Cell values the viscous term needs on both sides of the face (Flux8Kernel
+// L/K:2182-2205): fluid momenta and depth, and the tangential limited slopes of the
+// fluid momenta.
+struct PDSide {
+  double u3, u4, u5;   // cell state
+  double t4, t5;       // slope of u4, u5 along the face (uy for x-faces, ux for y-faces)
+};
+
+__device__ __forceinline__ void face_flux(int dir, const double L[6], const double R[6],
+                                          const FaceMet& f, const FaceSpeed& s, const PDSide& pl,
+                                          const PDSide& pr, const Grid& g, const Phys& ph,
+                                          FaceFlux& o) {
+  const double M = (dir == 0) ? f.j11 * f.j11 + f.j12 * f.j12 + f.j13 * f.j13
+                              : f.j21 * f.j21 + f.j22 * f.j22 + f.j23 * f.j23;
+  // Flux6Kernel L/K:1809-1843
+  const double epsilon0 = 1.0;
+  const double kk = 0.5 * epsilon0 * f.cav * f.det * M;
+  const double czwR = R[0] * (R[0] + R[3]) * (kk)*ph.Drtio1;
+  const double czwL = L[0] * (L[0] + L[3]) * (kk)*ph.Drtio1;
+  const double czwFR = (R[0] + R[3]) * (R[0] + R[3]) * (kk);
+  const double czwFL = (L[0] + L[3]) * (L[0] + L[3]) * (kk);
+  // Flux7Kernel L/K:1883-1932
+  double FR[6], FL[6];
+  FR[0] = R[0] * s.qR * f.det;
+  FR[1] = R[1] * s.qR * f.det + czwR * f.xa;
+  FR[2] = R[2] * s.qR * f.det + czwR * f.xb;
+  FR[3] = R[3] * s.qFR * f.det;
+  FR[4] = R[4] * s.qFR * f.det + czwFR * f.xa;
+  FR[5] = R[5] * s.qFR * f.det + czwFR * f.xb;
+  FL[0] = L[0] * s.qL * f.det;
+  FL[1] = L[1] * s.qL * f.det + czwL * f.xa;
+  FL[2] = L[2] * s.qL * f.det + czwL * f.xb;
+  FL[3] = L[3] * s.qFL * f.det;
+  FL[4] = L[4] * s.qFL * f.det + czwFL * f.xa;
+  FL[5] = L[5] * s.qFL * f.det + czwFL * f.xb;
+  // Flux8Kernel L/K:2081-2128: Hp = 0.5*(F_E + F_W - a*Det*(u_E - u_W))
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const double a = (k < 3) ? s.ap : s.apF;
+    o.H[k] = 0.5 * (FR[k] + FL[k] - a * f.det * (R[k] - L[k]));
+  }
+  // Flux8Kernel L/K:2131-2179: non-conservative pressure terms
+  const double cc = 0.5 * epsilon0 * f.cav * f.det;
+  const double hh = 0.5 * ((R[0] + R[3]) + (L[0] + L[3]));
+  const double pa = (dir == 0) ? f.j11 : f.j21;
+  const double pb = (dir == 0) ? f.j12 : f.j22;
+  o.P1 = -ph.rr * (cc)*hh * hh * pa;
+  o.P2 = -ph.rr * (cc)*hh * hh * pb;
+  o.P4 = (cc)*hh * hh * pa;
+  o.P5 = (cc)*hh * hh * pb;
+  // Flux8Kernel L/K:2182-2205: fluid velocity gradients at the face
+  double dudx, dvdx, dudy, dvdy;
+  if (dir == 0) {
+    dudx = (pr.u4 / (pr.u3 + H0) - pl.u4 / (pl.u3 + H0)) / g.dx;
+    dvdx = (pr.u5 / (pr.u3 + H0) - pl.u5 / (pl.u3 + H0)) / g.dx;
+    dudy = 0.5 * (pr.t4 / (pr.u3 + H0) + pl.t4 / (pl.u3 + H0)) / g.dy;
+    dvdy = 0.5 * (pr.t5 / (pr.u3 + H0) + pl.t5 / (pl.u3 + H0)) / g.dy;
+  } else {
+    dudx = 0.5 * (pr.t4 / (pr.u3 + H0) + pl.t4 / (pl.u3 + H0)) / g.dx;
+    dvdx = 0.5 * (pr.t5 / (pr.u3 + H0) + pl.t5 / (pl.u3 + H0)) / g.dx;
+    dudy = (pr.u4 / (pr.u3 + H0) - pl.u4 / (pl.u3 + H0)) / g.dy;
+    dvdy = (pr.u5 / (pr.u3 + H0) - pl.u5 / (pl.u3 + H0)) / g.dy;
+  }
+  // Flux9Kernel L/K:2242-2273
+  const double duxidxi = f.j11 * dudx + f.j12 * dvdx - f.j13 * (f.j31 * dudx + f.j32 * dvdx) / f.j33;
+  const double dvetdxi = f.j21 * dudx + f.j22 * dvdx - f.j23 * (f.j31 * dudx + f.j32 * dvdx) / f.j33;
+  const double duxidet = f.j11 * dudy + f.j12 * dvdy - f.j13 * (f.j31 * dudy + f.j32 * dvdy) / f.j33;
+  const double dvetdet = f.j21 * dudy + f.j22 * dvdy - f.j23 * (f.j31 * dudy + f.j32 * dvdy) / f.j33;
+  // Flux10Kernel L/K:2316-2343
+  if (dir == 0) {
+    o.D4 = 2.0 * f.det * (hh) * (f.j11 * duxidxi + f.j21 * duxidet);
+    o.D5 = f.det * (hh) * (f.j12 * duxidxi + f.j22 * duxidet + f.j11 * dvetdxi + f.j21 * dvetdet);
+  } else {
+    o.D4 = f.det * (hh) * (f.j12 * duxidxi + f.j22 * duxidet + f.j11 * dvetdxi + f.j21 * dvetdet);
+    o.D5 = 2.0 * f.det * (hh) * (f.j12 * dvetdxi + f.j22 * dvetdet);
+  }
+}
+
+// Cell metric for the source terms.
+struct CellMet {
+  double c, s0, s1, det;
+  double i11, i12, i13, i21, i22, i23;
+  double j13x, j23x, j33x, j13e, j23e, j33e;
+};
+
+__device__ __forceinline__ void cell_metric(const double* __restrict__ M, long long P, long long i,
+                                            CellMet& m) {
+  m.c = M[MC_C * P + i];
+  m.s0 = M[MC_S0 * P + i];
+  m.s1 = M[MC_S1 * P + i];
+  m.det = M[MC_DET * P + i];
+  m.i11 = M[MC_I11 * P + i];
+  m.i12 = M[MC_I12 * P + i];
+  m.i13 = M[MC_I13 * P + i];
+  m.i21 = M[MC_I21 * P + i];
+  m.i22 = M[MC_I22 * P + i];
+  m.i23 = M[MC_I23 * P + i];
+  m.j13x = M[MC_J13X * P + i];
+  m.j23x = M[MC_J23X * P + i];
+  m.j33x = M[MC_J33X * P + i];
+  m.j13e = M[MC_J13E * P + i];
+  m.j23e = M[MC_J23E * P + i];
+  m.j33e = M[MC_J33E * P + i];
+}
+
+// Flux11..17Kernel (L/K:2349-2621): the source vector s (components 1,2,4,5; s0 = s3 = 0).
+__device__ __forceinline__ void cell_source(const double u[7], const CellMet& m, const Phys& ph,
+                                            double s[6]) {
+  const double h = u[0], hf = u[3];
+  double vex, vey, vexF, veyF;
+  if (h <= H00) {  // Flux11
+    vex = 0.0;
+    vey = 0.0;
+  } else {
+    vex = u[1] / (h + H0);
+    vey = u[2] / (h + H0);
+  }
+  if (hf <= H00) {
+    vexF = 0.0;
+    veyF = 0.0;
+  } else {
+    vexF = u[4] / (hf + H0);
+    veyF = u[5] / (hf + H0);
+  }
+  const double w = (m.s0 * vex + m.s1 * vey) / m.c;  // Flux12
+  const double wF = (m.s0 * vexF + m.s1 * veyF) / m.c;
+  const double usw = vex * vex + vey * vey + w * w;  // Flux13
+  double vel = sqrt(vex * vex + vey * vey + w * w);
+  double vexw, veyw;
+  if (usw <= U0 * U0) {  // Flux14
+    vexw = vex / U0;
+    veyw = vey / U0;
+    vel = U0;
+  } else {
+    vexw = vex / vel;
+    veyw = vey / vel;
+  }
+  const double q_xi = m.i11 * vex + m.i12 * vey + m.i13 * w;
+  const double q_et = m.i21 * vex + m.i22 * vey + m.i23 * w;
+  const double q_xiF = m.i11 * vexF + m.i12 * veyF + m.i13 * wF;
+  const double q_etF = m.i21 * vexF + m.i22 * veyF + m.i23 * wF;
+  // Flux15
+  const double Ac = vex * m.j13x * q_xi + vey * m.j23x * q_xi + w * m.j33x * q_xi +
+                    vex * m.j13e * q_et + vey * m.j23e * q_et + w * m.j33e * q_et;
+  const double AcF = vexF * m.j13x * q_xiF + veyF * m.j23x * q_xiF + wF * m.j33x * q_xiF +
+                     vexF * m.j13e * q_etF + veyF * m.j23e * q_etF + wF * m.j33e * q_etF;
+  // Flux16
+  const double D = m.det;
+  const double Npress1 = fmax(0.0, (D * h * (ph.Drtio1 * m.c - Ac + AcF * ph.rr)));
+  const double Npress2 = fmax(0.0, (D * h * (m.c - Ac)));
+  const double NpressF = fmax(0.0, (D * hf * (m.c - AcF)));
+  // Flux17
+  const double phi = u[6];
+  const double epsilon0 = 1.0;
+  const double Dratio = ph.rr;
+  s[0] = 0.0;
+  s[1] = -Npress2 * m.s0 - Npress1 * ph.tande * vexw +
+         Dratio * D * ph.Cd * phi * (1.0 - phi) * (h + hf) * (vexF - vex);
+  s[2] = -Npress2 * m.s1 - Npress1 * ph.tande * veyw +
+         Dratio * D * ph.Cd * phi * (1.0 - phi) * (h + hf) * (veyF - vey);
+  s[3] = 0.0;
+  s[4] = -NpressF * m.s0 - D * ph.Cd * phi * (1.0 - phi) * (h + hf) * (vexF - vex) +
+         D * (-(1.0 - phi) * (h + hf) * ph.varTheta * vexF / epsilon0) / ph.N_R;
+  s[5] = -NpressF * m.s1 - D * ph.Cd * phi * (1.0 - phi) * (h + hf) * (veyF - vey) +
+         D * (-(1.0 - phi) * (h + hf) * ph.varTheta * veyF / epsilon0) / ph.N_R;
+}
+
+}  // namespace m2p

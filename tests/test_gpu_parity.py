@@ -1,0 +1,132 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle on the same inputs.
+
+The bar (north star): <= 1e-10 relative on depths and momenta, total solid and fluid
+mass equal to 1e-12 relative.  The implementation keeps the reference's operation order
+with FMA contraction off, so the tests also report how many values differ at all; the
+expected count is 0.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from moses_2pdf_amd import Simulation
+from moses_2pdf_amd import scenarios as S
+from oracle.oracle import Oracle
+from util import compare, df004, hsiaolin, interior
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+METRIC = ["cvalue", "Detmin", "invJ11", "invJ12", "invJ13", "invJ21", "invJ22", "invJ23",
+          "dxdxi11", "dxdxi12", "dxdxi21", "dxdxi22", "J13dxi", "J23dxi", "J33dxi", "J13det",
+          "J23det", "J33det"]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def fsum_mass(u, det):
+    return (math.fsum((interior(u[0]) * interior(det)).ravel()),
+            math.fsum((interior(u[3]) * interior(det)).ravel()))
+
+
+@pytest.mark.parametrize("case", ["plane48", "hsiaolin", "df004"])
+def test_metric_parity(case):
+    sc = {"plane48": lambda: S.landslide_plane(n=48), "hsiaolin": hsiaolin, "df004": df004}[case]()
+    say("metric parity", case, sc.nx, sc.ny)
+    o = Oracle(sc)
+    with Simulation(sc) as g:
+        for name in METRIC:
+            rel, ndiff = compare(g.field(name), o.get(name), name)
+            assert ndiff == 0, (case, name, rel, ndiff)
+        rel, ndiff = compare(g.field("svec"), o.get("svec"), "svec")
+        assert ndiff == 0
+        # the initial state (makeTopo10/11) including the ghost cells
+        rel, ndiff = compare(g.state(), o.get("u"), "u0")
+        assert ndiff == 0, (rel, ndiff)
+
+
+def run_pair(sc, steps, chunks, label):
+    o = Oracle(sc)
+    det = o.get("Detmin")
+    with Simulation(sc) as g:
+        done = 0
+        for n in chunks:
+            o.step(n)
+            g.step(n)
+            done += n
+            ug, uo = g.state(), o.get("u")
+            rel, ndiff = compare(interior(ug), interior(uo))
+            dtg, dmaxg, _ = g.scalars()
+            say(f"{label}: step {done} rel {rel:.3e} differing {ndiff} dt {dtg!r} / {o.scalar('dt')!r}")
+            assert dtg == o.scalar("dt"), (done, dtg, o.scalar("dt"))
+            assert dmaxg == o.scalar("dmax")
+            assert rel <= TOL, (label, done, rel, ndiff)
+            assert np.isfinite(ug).all()
+        ms, mf = g.mass()
+        rs, rf = fsum_mass(o.get("u"), det)
+        assert abs(ms - rs) <= 1e-12 * abs(rs) and abs(mf - rf) <= 1e-12 * abs(rf), (ms, rs, mf, rf)
+        tg, ntg = g.time_steps()
+        assert ntg == done
+        assert tg == o.scalar("time")
+    return ndiff
+
+
+def test_parity_plane_small():
+    sc = S.landslide_plane(n=48)
+    assert run_pair(sc, 60, [1, 1, 3, 15, 40], "plane48") == 0
+
+
+def test_parity_plane_c1_knobs():
+    # the 256^2 configuration's knobs (CFL 0.4, density ratio 1000/2700) on a 96^2 grid
+    sc = S.landslide_plane(n=96)
+    sc.cfl = 0.4
+    sc.rho_ratio = 1000.0 / 2700.0
+    run_pair(sc, 80, [1, 9, 70], "plane96-cfl0.4")
+
+
+def test_parity_hsiaolin():
+    sc = hsiaolin()
+    run_pair(sc, 40, [1, 4, 35], "hsiaolin")
+
+
+def test_parity_df004_inflow():
+    sc = df004()
+    run_pair(sc, 120, [1, 9, 110], "df004")
+
+
+def test_parity_fractal_per_cell_phi():
+    sc = S.landslide_fractal(n=64, seed=7)
+    run_pair(sc, 40, [2, 38], "fractal64-phi")
+
+
+def test_parity_set_state_midrun():
+    """Both implementations restarted from the same perturbed mid-run state."""
+    sc = S.landslide_plane(n=40)
+    o = Oracle(sc)
+    o.step(10)
+    u = o.get("u").copy()
+    rng = np.random.default_rng(5)
+    u[1] += 0.01 * rng.standard_normal(u[1].shape) * (u[0] > 0)
+    u[5] -= 0.01 * rng.standard_normal(u[5].shape) * (u[3] > 0)
+    o.set_u(u)
+    with Simulation(sc) as g:
+        g.set_state(u)
+        o.step(10)
+        g.step(10)
+        rel, ndiff = compare(interior(g.state()), interior(o.get("u")))
+        say("set_state midrun", rel, ndiff)
+        assert rel <= TOL
+
+
+def test_c1_workload_parity_256():
+    """The 256^2 landslide configuration (35 degree plane, bump, 1e6 m^3 release,
+    phi 0.6, CFL 0.4, density ratio 1000/2700) for 100 steps."""
+    sc = S.landslide_plane(n=256, phi=0.6)
+    sc.cfl = 0.4
+    sc.rho_ratio = 1000.0 / 2700.0
+    run_pair(sc, 100, [100], "c1-256")
