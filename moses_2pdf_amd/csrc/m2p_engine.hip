@@ -39,6 +39,9 @@ struct m2p_ctx {
   std::string err;
   bool prof = false;
   std::map<std::string, std::pair<double, int>> pstats;
+  std::vector<hipEvent_t> evpool;
+  size_t evnext = 0;
+  std::vector<std::pair<const char*, std::pair<hipEvent_t, hipEvent_t>>> pending;
 };
 
 namespace {
@@ -109,6 +112,9 @@ int upload(m2p_ctx* c, double* d, const std::vector<double>& h) {
 }
 
 void release(m2p_ctx* c) {
+  for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
+  c->evpool.clear();
+  c->pending.clear();
   void* ps[] = {c->dTopo, c->dDepth, c->dPhi, c->dM,     c->dU0,   c->dU1,  c->dONE,
                 c->dTf,   c->dMass,  c->dTable, c->dLoc, c->dItime, c->dDire, c->dSt};
   for (void* p : ps)
@@ -171,31 +177,59 @@ int setup(m2p_ctx* c) {
   return 0;
 }
 
+// Per-launch HIP events on the context's stream, resolved only when read (no host
+// synchronisation between launches).
+struct ProfRec {
+  const char* name;
+  hipEvent_t a, b;
+};
+
+hipEvent_t pool_event(m2p_ctx* c);
+
 struct Timer {
   m2p_ctx* c;
   const char* name;
-  hipEvent_t a = nullptr, b = nullptr;
+  hipEvent_t a = nullptr;
   Timer(m2p_ctx* c_, const char* n) : c(c_), name(n) {
     if (c->prof) {
-      (void)hipEventCreate(&a);
-      (void)hipEventCreate(&b);
+      a = pool_event(c);
       (void)hipEventRecord(a, c->s);
     }
   }
-  ~Timer() {
-    if (c->prof) {
-      (void)hipEventRecord(b, c->s);
-      (void)hipEventSynchronize(b);
-      float ms = 0;
-      (void)hipEventElapsedTime(&ms, a, b);
-      auto& e = c->pstats[name];
-      e.first += ms;
-      e.second += 1;
-      (void)hipEventDestroy(a);
-      (void)hipEventDestroy(b);
-    }
-  }
+  ~Timer();
 };
+
+hipEvent_t pool_event(m2p_ctx* c) {
+  if (c->evnext == c->evpool.size()) {
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    c->evpool.push_back(e);
+  }
+  return c->evpool[c->evnext++];
+}
+
+Timer::~Timer() {
+  if (c->prof) {
+    hipEvent_t b = pool_event(c);
+    (void)hipEventRecord(b, c->s);
+    c->pending.push_back({name, {a, b}});
+  }
+}
+
+// Folds recorded event pairs into pstats (waits for the last one).
+void resolve_profile(m2p_ctx* c) {
+  if (c->pending.empty()) return;
+  (void)hipEventSynchronize(c->pending.back().second.second);
+  for (auto& p : c->pending) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, p.second.first, p.second.second);
+    auto& e = c->pstats[p.first];
+    e.first += ms;
+    e.second += 1;
+  }
+  c->pending.clear();
+  c->evnext = 0;
+}
 
 // One pass of the kernelStep loop body (L/R:646-1330; inflow I/R:721-1424).
 void enqueue_step(m2p_ctx* c, int ntf) {
@@ -408,7 +442,10 @@ int m2p_open(const char* par_list, int mode, const char* overlay, m2p_ctx** out)
 int m2p_step(m2p_ctx* c, int nsteps) {
   if (!c || !c->s) return M2P_ESTATE;
   if (nsteps < 0) return fail(c, M2P_EINVAL, "negative step count");
-  for (int n = 0; n < nsteps; n++) enqueue_step(c, 0);
+  for (int n = 0; n < nsteps; n++) {
+    enqueue_step(c, 0);
+    if (c->prof && c->pending.size() > 4096) resolve_profile(c);
+  }
   HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -568,6 +605,7 @@ int m2p_get_field(m2p_ctx* c, const char* name, double* out) {
 
 int m2p_profile(m2p_ctx* c, int enable) {
   if (!c) return M2P_ESTATE;
+  resolve_profile(c);
   c->prof = enable != 0;
   c->pstats.clear();
   return 0;
@@ -575,6 +613,7 @@ int m2p_profile(m2p_ctx* c, int enable) {
 
 int m2p_profile_get(m2p_ctx* c, const char* kernel, double* ms, int* launches) {
   if (!c || !kernel) return M2P_ESTATE;
+  resolve_profile(c);
   auto it = c->pstats.find(kernel);
   if (it == c->pstats.end()) {
     if (ms) *ms = 0;

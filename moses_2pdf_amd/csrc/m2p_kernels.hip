@@ -290,27 +290,28 @@ __device__ __forceinline__ double wave_max(double v) {
 // the predictor: the max over faces [1, N-3]^2 of em/dx, em/dy (solid and fluid).  Faces
 // outside that square have a zero face-averaged metric in the reference, so their em is
 // the 1e-10 floor, which never exceeds these.  Result: atomicMax on the IEEE bits of a
-// positive double.
-__global__ void __launch_bounds__(256) k_cfl(Grid g, Phys ph, const double* __restrict__ U,
-                                             const double* __restrict__ M, DevState* __restrict__ st) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  const int r = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  double m = 0.0;
-  if (r <= g.nyd - 3 && c <= g.nxd - 3) {
-#pragma unroll 1
-    for (int dir = 0; dir < 2; dir++) {
-      double L[6], R[6];
-      final_states(U, g, dir, r, c, L, R);
-      FaceMet f;
-      face_metric(M, g, dir, r, c, f);
-      FaceSpeed s;
-      face_speed(dir, L, R, f, ph, s);
-      const double d = dir == 0 ? g.dx : g.dy;
-      m = fmax(m, fmax(s.emS / d, s.emF / d));
-    }
-  }
+// positive double.  k_cfl covers the faces of the interior box [3, ny+2] x [3, nx+2]
+// (InterfacesKernel states); k_cfl_ring the rest of [1, N-3]^2, where Interfaces2Kernel
+// overrides the states.
+__device__ __forceinline__ double face_cfl(const double* __restrict__ U, const double* __restrict__ M,
+                                           const Grid& g, const Phys& ph, int dir, int r, int c,
+                                           bool edge) {
+  double L[6], R[6];
+  if (edge)
+    final_states_edge(U, g, dir, r, c, L, R);
+  else
+    base_states(U, g, dir, r, c, L, R);
+  FaceMet f;
+  face_metric(M, g, dir, r, c, f);
+  FaceSpeed s;
+  face_speed(dir, L, R, f, ph, s);
+  const double d = dir == 0 ? g.dx : g.dy;
+  return fmax(s.emS / d, s.emF / d);
+}
+
+__device__ __forceinline__ void block_max_to(double m, DevState* st) {
   m = wave_max(m);
-  __shared__ double part[8];
+  __shared__ double part[16];
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
   const int nw = (blockDim.x * blockDim.y + 63) / 64;
   if ((tid & 63) == 0) part[tid >> 6] = m;
@@ -320,6 +321,45 @@ __global__ void __launch_bounds__(256) k_cfl(Grid g, Phys ph, const double* __re
     for (int w = 1; w < nw; w++) b = fmax(b, part[w]);
     atomicMax(&st->maxbits, (unsigned long long)__double_as_longlong(b));
   }
+}
+
+__global__ void __launch_bounds__(256) k_cfl(Grid g, Phys ph, const double* __restrict__ U,
+                                             const double* __restrict__ M, DevState* __restrict__ st) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x + MD;
+  const int r = blockIdx.y * blockDim.y + threadIdx.y + MD;
+  double m = 0.0;
+  if (r <= g.ny + MD - 1 && c <= g.nx + MD - 1) {
+    m = face_cfl(U, M, g, ph, 0, r, c, false);
+    m = fmax(m, face_cfl(U, M, g, ph, 1, r, c, false));
+  }
+  block_max_to(m, st);
+}
+
+// The ring [1, N-3]^2 minus the interior box, enumerated as 4 strips: rows 1-2 and
+// ny+3..ny+4 (full width 1..N-3), then columns 1-2 and nx+3..nx+4 on rows 3..ny+2.
+__global__ void __launch_bounds__(256) k_cfl_ring(Grid g, Phys ph, const double* __restrict__ U,
+                                                  const double* __restrict__ M,
+                                                  DevState* __restrict__ st) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int w = g.nxd - 3;        // columns 1..N-3
+  const int nrow = 4 * w;         // 4 ghost face rows
+  const int ncol = 4 * g.ny;      // 4 ghost face columns on interior rows
+  double m = 0.0;
+  if (t < nrow + ncol) {
+    int r, c;
+    if (t < nrow) {
+      const int k = t / w;
+      r = (k < 2) ? 1 + k : g.ny + MD + (k - 2);
+      c = 1 + t % w;
+    } else {
+      const int q = t - nrow, k = q / g.ny;
+      r = MD + q % g.ny;
+      c = (k < 2) ? 1 + k : g.nx + MD + (k - 2);
+    }
+    m = face_cfl(U, M, g, ph, 0, r, c, true);
+    m = fmax(m, face_cfl(U, M, g, ph, 1, r, c, true));
+  }
+  block_max_to(m, st);
 }
 
 // CFL3Kernel + the host's dt control + CFL4Kernel (L/K:2006-2036, L/R:978-991; inflow
@@ -365,44 +405,141 @@ __device__ __forceinline__ PDSide pd_side(const double* __restrict__ U, const Gr
   return p;
 }
 
-__device__ __forceinline__ void face_all(const double* __restrict__ U, const double* __restrict__ M,
-                                         const Grid& g, const Phys& ph, int dir, int r, int c,
-                                         FaceFlux& o) {
-  double L[6], R[6];
-  final_states(U, g, dir, r, c, L, R);
-  FaceMet f;
-  face_metric(M, g, dir, r, c, f);
-  FaceSpeed s;
-  face_speed(dir, L, R, f, ph, s);
-  const long long i = idx(g, r, c);
-  const long long j = i + (dir == 0 ? 1 : g.nxd);
-  const PDSide pl = pd_side(U, g, i, 1 - dir);
-  const PDSide pr = pd_side(U, g, j, 1 - dir);
-  face_flux(dir, L, R, f, s, pl, pr, g, ph, o);
-}
 
-// One stage (predictor IO = 0, corrector IO = 1) for one interior cell per thread:
-// Flux8..21Kernel (L/K:2038-2726) + the crossing limiter, Flux22..24 / Flux25..28
-// (L/K:2730-2937).
+// One stage (predictor IO = 0, corrector IO = 1): Flux8..21Kernel (L/K:2038-2726) and the
+// crossing limiter, Flux22..24 / Flux25..28 (L/K:2730-2937).
 //   IO = 0: Uin = U0 = u^n (also uzero).  Writes u^{n+1/2} into Uout = U1 and the limited
 //           fluid momenta uone[4], uone[5] into ONE.
 //   IO = 1: Uin = U1, uzero = U0, old velocities from (U1[0..3], ONE).  Writes u^{n+1}
 //           into U0 in place (each thread reads and writes only its own cell of U0).
+// A block of 8 waves owns a tile of TX = 63 columns x TY = 7 rows of cells.  Each face is
+// computed once: the 64 x-faces of a tile row by one wave, then (through LDS) the 64
+// y-faces of a face row by one wave; a cell reads its two faces of each direction from
+// LDS.  The update keeps the reference's left-to-right sum, so the x terms are formed
+// first and held in registers while the y faces are computed.
+#ifndef M2P_STAGE_ATTR
+#define M2P_STAGE_ATTR __attribute__((amdgpu_waves_per_eu(4, 4)))
+#endif
+constexpr int ST_TX = 63, ST_TY = 7, ST_NQ = 12;
+typedef double (*Tile)[ST_TY + 1][64];
+
+// Phase A: InterfacesKernel states of one face into the tile (L: q 0-5, R: q 6-11).
+__device__ __forceinline__ void stage_states(Tile T, int w, int lane, const double* __restrict__ U,
+                                             const Grid& g, int dir, int r, int c) {
+  double L[6], R[6];
+  base_states(U, g, dir, r, c, L, R);
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    T[k][w][lane] = L[k];
+    T[6 + k][w][lane] = R[k];
+  }
+}
+
+// Phase B: the face's final states.  The only Interfaces2Kernel overrides a stage needs are
+// the x-face at column 2 (from the faces at columns 3, 4: lanes +1, +2) and the y-face at
+// row 2 (from rows 3, 4: tile rows +1, +2).
+__device__ __forceinline__ void load_states(Tile T, int w, int lane, bool ovr, int dir, double L[6],
+                                            double R[6]) {
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    if (!ovr) {
+      L[k] = T[k][w][lane];
+      R[k] = T[6 + k][w][lane];
+    } else if (dir == 0) {
+      L[k] = 0.5 * (T[k][w][lane + 1] + T[k][w][lane + 2]);
+      R[k] = 0.5 * (T[6 + k][w][lane + 1] + T[6 + k][w][lane + 2]);
+    } else {
+      L[k] = 0.5 * (T[k][w + 1][lane] + T[k][w + 2][lane]);
+      R[k] = 0.5 * (T[6 + k][w + 1][lane] + T[6 + k][w + 2][lane]);
+    }
+  }
+}
+
+// Phase C: fluxes of one face from its final states, into the tile.
+__device__ __forceinline__ void flux_store(Tile T, int w, int lane, const double* __restrict__ U,
+                                           const double* __restrict__ M, const Grid& g,
+                                           const Phys& ph, int dir, int r, int c, double L[6],
+                                           double R[6]) {
+  FaceMet f;
+  face_metric(M, g, dir, r, c, f);
+  FaceSpeed sp;
+  face_speed(dir, L, R, f, ph, sp);
+  const long long i = idx(g, r, c);
+  const long long j = i + (dir == 0 ? 1 : g.nxd);
+  const PDSide pl = pd_side(U, g, i, 1 - dir);
+  const PDSide pr = pd_side(U, g, j, 1 - dir);
+  FaceFlux o;
+  face_flux(dir, L, R, f, sp, pl, pr, g, ph, o);
+#pragma unroll
+  for (int k = 0; k < 6; k++) T[k][w][lane] = o.H[k];
+  T[6][w][lane] = o.P1;
+  T[7][w][lane] = o.P2;
+  T[8][w][lane] = o.P4;
+  T[9][w][lane] = o.P5;
+  T[10][w][lane] = o.D4;
+  T[11][w][lane] = o.D5;
+}
+
 template <int IO>
-__global__ void __launch_bounds__(256) k_stage(Grid g, Phys ph, const double* __restrict__ Uin,
+__global__ void __launch_bounds__(512) M2P_STAGE_ATTR k_stage(Grid g, Phys ph, const double* __restrict__ Uin,
                                                const double* Uzero, double* Uout,
                                                double* __restrict__ ONE, const double* __restrict__ M,
                                                const DevState* __restrict__ st) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x + MD;
-  const int r = blockIdx.y * blockDim.y + threadIdx.y + MD;
-  if (r > g.ny + MD - 1 || c > g.nx + MD - 1) return;
-  const long long P = g.P, i = idx(g, r, c);
+  __shared__ double F[ST_NQ][ST_TY + 1][64];
+  const int lane = threadIdx.x;
+  const int w = threadIdx.y;
+  const int c0 = blockIdx.x * ST_TX + MD;
+  const int r0 = blockIdx.y * ST_TY + MD;
+  const int xmax = g.nx + MD - 1, ymax = g.ny + MD - 1;
+  const long long P = g.P;
 
-  FaceFlux xm, xp, ym, yp;
-  face_all(Uin, M, g, ph, 0, r, c - 1, xm);
-  face_all(Uin, M, g, ph, 0, r, c, xp);
-  face_all(Uin, M, g, ph, 1, r - 1, c, ym);
-  face_all(Uin, M, g, ph, 1, r, c, yp);
+  // x faces (r0+w, c0-1+lane), faces (r, c)-(r, c+1)
+  {
+    const int r = r0 + w, c = c0 - 1 + lane;
+    const bool has = (w < ST_TY && r <= ymax && c <= xmax);
+    if (has) stage_states(F, w, lane, Uin, g, 0, r, c);
+    __syncthreads();
+    double L[6], R[6];
+    if (has) load_states(F, w, lane, c == 2, 0, L, R);
+    __syncthreads();
+    if (has) flux_store(F, w, lane, Uin, M, g, ph, 0, r, c, L, R);
+  }
+  __syncthreads();
+
+  const int r = r0 + w, c = c0 + lane;
+  const bool active = (w < ST_TY) && (lane < ST_TX) && r <= ymax && c <= xmax;
+  const long long i = idx(g, r, c);
+  const double dt = IO == 0 ? st->dt_pred : st->dt_corr;
+  const double dtdx = dt / g.dx, dtdy = dt / g.dy;
+  const double epsilon0 = 1.0;
+  double phi = 0.0, fP = 0.0, fD = 0.0;
+  double tH[6], tP[4], tD[2];
+  if (active) {
+    phi = Uin[6 * P + i];
+    fP = epsilon0 * phi;
+    fD = epsilon0 * (1.0 - phi) / ph.N_R;
+#pragma unroll
+    for (int k = 0; k < 6; k++) tH[k] = (dtdx) * (F[k][w][lane + 1] - F[k][w][lane]);
+#pragma unroll
+    for (int q = 0; q < 4; q++) tP[q] = (dtdx) * (F[6 + q][w][lane + 1] - F[6 + q][w][lane]) * (fP);
+#pragma unroll
+    for (int q = 0; q < 2; q++) tD[q] = (dtdx) * (F[10 + q][w][lane + 1] - F[10 + q][w][lane]) * (fD);
+  }
+  __syncthreads();
+
+  // y faces (r0-1+w, c0+lane), faces (r, c)-(r+1, c)
+  {
+    const int rf = r0 - 1 + w, cf = c0 + lane;
+    const bool has = (lane < ST_TX && rf <= ymax && cf <= xmax);
+    if (has) stage_states(F, w, lane, Uin, g, 1, rf, cf);
+    __syncthreads();
+    double L[6], R[6];
+    if (has) load_states(F, w, lane, rf == 2, 1, L, R);
+    __syncthreads();
+    if (has) flux_store(F, w, lane, Uin, M, g, ph, 1, rf, cf, L, R);
+  }
+  __syncthreads();
+  if (!active) return;
 
   double u[7];
 #pragma unroll
@@ -411,32 +548,25 @@ __global__ void __launch_bounds__(256) k_stage(Grid g, Phys ph, const double* __
   cell_metric(M, P, i, cm);
   double s[6];
   cell_source(u, cm, ph, s);
-
-  const double dt = IO == 0 ? st->dt_pred : st->dt_corr;
-  const double dtdx = dt / g.dx, dtdy = dt / g.dy;
-  const double epsilon0 = 1.0;
-  const double phi = u[6];
-  const double fP = epsilon0 * phi;
-  const double fD = epsilon0 * (1.0 - phi) / ph.N_R;
-  double uz[7];
+  double uz[6];
 #pragma unroll
-  for (int k = 0; k < 7; k++) uz[k] = Uzero[k * P + i];
+  for (int k = 0; k < 6; k++) uz[k] = Uzero[k * P + i];
 
   // Flux18Kernel L/K:2641-2652 (Pp and PD are stored zeros for the components they skip)
   double vnew[6];
 #pragma unroll
   for (int k = 0; k < 6; k++) {
-    const double Px_p = k == 1 ? xp.P1 : k == 2 ? xp.P2 : k == 4 ? xp.P4 : k == 5 ? xp.P5 : 0.0;
-    const double Px_m = k == 1 ? xm.P1 : k == 2 ? xm.P2 : k == 4 ? xm.P4 : k == 5 ? xm.P5 : 0.0;
-    const double Py_p = k == 1 ? yp.P1 : k == 2 ? yp.P2 : k == 4 ? yp.P4 : k == 5 ? yp.P5 : 0.0;
-    const double Py_m = k == 1 ? ym.P1 : k == 2 ? ym.P2 : k == 4 ? ym.P4 : k == 5 ? ym.P5 : 0.0;
-    const double Dx_p = k == 4 ? xp.D4 : k == 5 ? xp.D5 : 0.0;
-    const double Dx_m = k == 4 ? xm.D4 : k == 5 ? xm.D5 : 0.0;
-    const double Dy_p = k == 4 ? yp.D4 : k == 5 ? yp.D5 : 0.0;
-    const double Dy_m = k == 4 ? ym.D4 : k == 5 ? ym.D5 : 0.0;
-    vnew[k] = uz[k] * cm.det - (dtdx) * (xp.H[k] - xm.H[k]) - (dtdy) * (yp.H[k] - ym.H[k]) +
-              (dtdx) * (Px_p - Px_m) * (fP) + (dtdy) * (Py_p - Py_m) * (fP) +
-              (dtdx) * (Dx_p - Dx_m) * (fD) + (dtdy) * (Dy_p - Dy_m) * (fD) + (dt)*s[k];
+    const int q = k == 1 ? 0 : k == 2 ? 1 : k == 4 ? 2 : 3;  // P1, P2, P4, P5
+    const bool hasP = (k == 1 || k == 2 || k == 4 || k == 5);
+    const bool hasD = (k == 4 || k == 5);
+    const double yH = (dtdy) * (F[k][w + 1][lane] - F[k][w][lane]);
+    const double xP = hasP ? tP[q] : (dtdx) * (0.0 - 0.0) * (fP);
+    const double yP = hasP ? (dtdy) * (F[6 + q][w + 1][lane] - F[6 + q][w][lane]) * (fP)
+                           : (dtdy) * (0.0 - 0.0) * (fP);
+    const double xD = hasD ? tD[k - 4] : (dtdx) * (0.0 - 0.0) * (fD);
+    const double yD = hasD ? (dtdy) * (F[10 + (k - 4)][w + 1][lane] - F[10 + (k - 4)][w][lane]) * (fD)
+                           : (dtdy) * (0.0 - 0.0) * (fD);
+    vnew[k] = uz[k] * cm.det - tH[k] - yH + xP + yP + xD + yD + (dt)*s[k];
   }
   double un[7];
 #pragma unroll
@@ -566,7 +696,9 @@ void launch_ghost(const Grid& g, int mode, double* T, const double* prev, hipStr
 void launch_cfl(const Grid& g, const Phys& ph, const double* U, const double* M, DevState* st,
                 hipStream_t s) {
   const dim3 b(64, 4);
-  k_cfl<<<grid2(g.nxd - 3, g.nyd - 3, b), b, 0, s>>>(g, ph, U, M, st);
+  k_cfl<<<grid2(g.nx, g.ny, b), b, 0, s>>>(g, ph, U, M, st);
+  const int ring = 4 * (g.nxd - 3) + 4 * g.ny;
+  k_cfl_ring<<<(ring + 255) / 256, 256, 0, s>>>(g, ph, U, M, st);
 }
 
 void launch_dt(const DevParams& dp, DevState* st, const double* tf, const double* itime,
@@ -576,8 +708,8 @@ void launch_dt(const DevParams& dp, DevState* st, const double* tf, const double
 
 void launch_stage(int io, const Grid& g, const Phys& ph, const double* Uin, const double* Uzero,
                   double* Uout, double* ONE, const double* M, const DevState* st, hipStream_t s) {
-  const dim3 b(64, 4);
-  const dim3 gr = grid2(g.nx, g.ny, b);
+  const dim3 b(64, ST_TY + 1);
+  const dim3 gr((g.nx + ST_TX - 1) / ST_TX, (g.ny + ST_TY - 1) / ST_TY);
   if (io == 0)
     k_stage<0><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, st);
   else

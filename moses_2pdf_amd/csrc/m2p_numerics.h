@@ -98,8 +98,8 @@ __device__ __forceinline__ void x_overridden(const double* __restrict__ U, const
   }
 }
 
-__device__ __forceinline__ void final_states(const double* __restrict__ U, const Grid& g, int dir,
-                                             int r, int c, double L[6], double R[6]) {
+__device__ __noinline__ void final_states_edge(const double* __restrict__ U, const Grid& g,
+                                               int dir, int r, int c, double L[6], double R[6]) {
   if (r <= 2 || r >= g.ny + MD) {
     const int ra = (r <= 2) ? MD : g.ny + MD - 1;
     const int rb = (r <= 2) ? MD + 1 : g.ny + MD - 2;
@@ -114,6 +114,16 @@ __device__ __forceinline__ void final_states(const double* __restrict__ U, const
   } else {
     x_overridden(U, g, dir, r, c, L, R);
   }
+}
+
+// Face states of the face (dir, r, c): the InterfacesKernel values inside
+// [3, ny+2] x [3, nx+2], the Interfaces2Kernel overrides (out of line) on the ghost ring.
+__device__ __forceinline__ void final_states(const double* __restrict__ U, const Grid& g, int dir,
+                                             int r, int c, double L[6], double R[6]) {
+  if (r >= MD && r <= g.ny + MD - 1 && c >= MD && c <= g.nx + MD - 1)
+    base_states(U, g, dir, r, c, L, R);
+  else
+    final_states_edge(U, g, dir, r, c, L, R);
 }
 
 // KeepPositivi1Kernel + KeepPositivi2Kernel (L/K:1339-1424) on one face state.

@@ -1117,6 +1117,30 @@ int orc_step(void* h, int n) {
   return done;
 }
 
+// Runs the first half of a predictor (ghost fill, slopes, interface states, positivity,
+// local speeds and face fluxes, L/R:649-961) without updating u, so ux, uy, uE.., Val hold
+// that stage's values.  Test hook.
+void orc_faces(void* h) {
+  Orc* o = (Orc*)h;
+  o->uzeroCopy();
+  o->boundary5();
+  o->boundary6();
+  o->boundary78();
+  o->boundary9();
+  o->tvd();
+  o->interfaces();
+  o->interfaces2();
+  o->keepPositivity();
+  o->faceFluxes();
+  o->cflReduce();
+}
+
+// Applies only the inflow injection of the next step (I/K:818-887).  Test hook.
+void orc_inject(void* h) {
+  Orc* o = (Orc*)h;
+  if (o->mode == 1) o->inflowInject();
+}
+
 int orc_nxd(void* h) { return ((Orc*)h)->nxd; }
 int orc_nyd(void* h) { return ((Orc*)h)->nyd; }
 
@@ -1174,6 +1198,9 @@ int orc_get(void* h, const char* name, double* out) {
                  {"PDy", &o->PDy},
                  {"s", &o->s},
                  {"Val", &o->Val},
+                 {"dbdx", &o->dbdx},
+                 {"dbdy", &o->dbdy},
+                 {"v", &o->v},
                  {"apEW", &o->apEW},
                  {"apSN", &o->apSN}};
   for (auto& t : tab)

@@ -46,6 +46,8 @@ def lib():
         L.orc_set_u.argtypes = [ctypes.c_void_p, dp]
         L.orc_scalar.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
         L.orc_scalar.restype = ctypes.c_double
+        L.orc_faces.argtypes = [ctypes.c_void_p]
+        L.orc_inject.argtypes = [ctypes.c_void_p]
         L.orc_nxd.argtypes = [ctypes.c_void_p]
         L.orc_nyd.argtypes = [ctypes.c_void_p]
         _LIB = L
@@ -153,6 +155,13 @@ class Oracle:
         if n < 0:
             raise KeyError(name)
         return out.reshape(planes, self.nyd, self.nxd) if planes > 1 else out.reshape(self.nyd, self.nxd)
+
+    def faces(self) -> None:
+        """First half of a predictor (no update): fills ux, uy, uE.., Val, dmax."""
+        lib().orc_faces(self.h)
+
+    def inject(self) -> None:
+        lib().orc_inject(self.h)
 
     def set_u(self, u: np.ndarray) -> None:
         a = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)

@@ -1,0 +1,292 @@
+"""Pins of the CPU oracle against what the mathematics fixes (CPU only).
+
+The reference ships no tests, expected outputs or golden vectors (its data files are
+inputs only), so each part of the oracle is pinned here by a closed form, an invariant,
+a textbook special case or brute force, chosen so that a plausible transcription mistake
+(a dropped term, a wrong sign, a swapped index or operand) fails one of them.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from moses_2pdf_amd import scenarios as S
+from oracle.oracle import Oracle, f32
+from util import GOLDEN, df004, hsiaolin
+
+H0 = 1.0e-6
+
+
+def flat_lake(n=24, depth=2.0, phi=0.5):
+    topo = np.zeros((n, n))
+    d = np.full((n, n), depth)
+    return S.Scenario(mode=0, topo=topo, depth=d, cellsize=20.0, phiS0=phi, delta0=20.0)
+
+
+def mass(o):
+    u, D = o.get("u"), o.get("Detmin")
+    return (math.fsum((u[0, 3:-3, 3:-3] * D[3:-3, 3:-3]).ravel()),
+            math.fsum((u[3, 3:-3, 3:-3] * D[3:-3, 3:-3]).ravel()))
+
+
+# ---------------------------------------------------------------------------------------
+# terrain metric (makeTopo1..11, JacobKernel, MeanKernel)
+# ---------------------------------------------------------------------------------------
+
+def test_metric_on_inclined_plane_closed_forms():
+    # z = 1000 - 10*col metres: exact in float, slope -0.5 along x.  The edge copy of the
+    # topography into the ghosts halves the slope on the first/last interior column, so
+    # the slope-dependent closed forms are checked on columns 4..N-5.
+    n, dx, t = 32, 20.0, 0.5
+    topo = (1000.0 - 10.0 * np.arange(n)[None, :]).repeat(n, axis=0)
+    sc = S.Scenario(mode=0, topo=topo, depth=np.zeros((n, n)), cellsize=dx)
+    o = Oracle(sc)
+    I = (slice(3, -3), slice(4, -4))
+    bx = o.get("dbdx")[I]
+    assert np.allclose(bx, -t, rtol=1e-15)
+    assert np.allclose(o.get("dbdx")[3:-3, 3], -t / 2, rtol=1e-15)  # one-sided at the edge
+    assert np.all(o.get("dbdy")[3:-3, 3:-3] == 0.0)
+    c = o.get("cvalue")[I]
+    assert np.allclose(c, 1.0 / math.sqrt(1.0 + t * t), rtol=1e-15)
+    s = o.get("svec")
+    assert np.allclose(s[0][I], c * bx, rtol=1e-15) and np.all(s[1][I] == 0.0)
+    # Cartesian grid: dx/dxi = I; Detmin = 1/c; invJ of the surface frame
+    assert np.allclose(o.get("dxdxi11")[I], 1.0, rtol=1e-15)
+    assert np.all(o.get("dxdxi12")[I] == 0.0) and np.all(o.get("dxdxi21")[I] == 0.0)
+    assert np.allclose(o.get("Detmin")[I], 1.0 / c, rtol=1e-14)
+    sin2 = (c * bx) ** 2
+    assert np.allclose(o.get("invJ11")[I], 1.0 - sin2, rtol=1e-14)
+    assert np.allclose(o.get("invJ13")[I], c * (c * bx), rtol=1e-14)
+    assert np.allclose(o.get("invJ22")[I], 1.0, rtol=1e-15)
+    assert np.all(o.get("invJ12")[I] == 0.0) and np.all(o.get("invJ21")[I] == 0.0)
+    assert np.allclose(o.get("invJ31")[I], -(c * bx), rtol=1e-15)
+    assert np.allclose(o.get("invJ33")[I], c, rtol=1e-15)
+    I = (slice(3, -3), slice(5, -5))
+    # a plane has no curvature
+    for nm in ("J13dxi", "J23dxi", "J33dxi", "J13det", "J23det", "J33det"):
+        assert np.abs(o.get(nm)[I]).max() < 1e-12, nm
+    # ghost copies: column 0 equals column 3, row 0 equals row 3
+    cv = o.get("cvalue")
+    assert np.all(cv[:, 0] == cv[:, 3]) and np.all(cv[0, :] == cv[3, :])
+    # the reference's dxdxi21 quirk: column nx+MD+2 is never written by Boundary1Kernel
+    assert np.all(o.get("dxdxi21")[:, -1] == 0.0)
+
+
+def test_curvature_terms_on_a_parabolic_valley():
+    """b = k x^2 / 2 across x: J13dxi = d(-s1)/dx = -k c^3 and J33dxi = dc/dx = -k^2 x c^3,
+    with second-order central differences (relative error O(dx^2))."""
+    n, dx, k = 48, 20.0, 2.0e-4
+    x = (np.arange(n) - n // 2) * dx
+    topo = np.repeat((0.5 * k * x * x)[None, :], n, axis=0) + 100.0
+    sc = S.Scenario(mode=0, topo=topo.astype(np.float32).astype(np.float64),
+                    depth=np.zeros((n, n)), cellsize=dx)
+    o = Oracle(sc)
+    K = k / 10.0  # scaled lengths (x10): b and x both scale, curvature scales by 1/10
+    X = (np.arange(n + 6) - 3 - n // 2) * dx * 10.0
+    c = 1.0 / np.sqrt(1.0 + (K * X) ** 2)
+    cols = slice(8, n - 2)
+    row = n // 2
+    j13 = o.get("J13dxi")[row, cols]
+    j33 = o.get("J33dxi")[row, cols]
+    assert np.allclose(j13, (-K * c ** 3)[cols], rtol=2e-3)
+    assert np.allclose(j33, (-K * K * X * c ** 3)[cols], rtol=2e-3, atol=1e-12)
+    assert np.abs(o.get("J13det")[3:-3, 3:-3]).max() < 1e-15  # nothing varies along y
+
+
+# ---------------------------------------------------------------------------------------
+# TVD slopes (TVD1..5Kernel)
+# ---------------------------------------------------------------------------------------
+
+def minmod(a, b):
+    return np.where(a * b > 0, np.sign(a) * np.minimum(np.abs(a), np.abs(b)), 0.0)
+
+
+def test_slopes_are_textbook_minmod():
+    sc = flat_lake(n=20)
+    o = Oracle(sc)
+    rng = np.random.default_rng(3)
+    u = o.get("u").copy()
+    for k in range(7):
+        u[k] = rng.standard_normal(u[k].shape)
+        u[k][rng.random(u[k].shape) < 0.2] = 0.0      # zero differences: sign(0) = -1
+        u[k][:, 7] = u[k][:, 8]                         # equal neighbours
+    o.set_u(u)
+    o.faces()
+    ug = o.get("u")   # after the ghost fill
+    ux, uy = o.get("ux"), o.get("uy")
+    dA = ug[:, 1:-1, 1:-1] - ug[:, 1:-1, :-2]
+    dB = ug[:, 1:-1, 2:] - ug[:, 1:-1, 1:-1]
+    assert np.array_equal(ux[:, 1:-1, 1:-1], minmod(dA, dB))
+    dA = ug[:, 1:-1, 1:-1] - ug[:, :-2, 1:-1]
+    dB = ug[:, 2:, 1:-1] - ug[:, 1:-1, 1:-1]
+    assert np.array_equal(uy[:, 1:-1, 1:-1], minmod(dA, dB))
+    # Boundary9Kernel leaves the outer ring of slopes at zero
+    assert np.all(ux[:, 0, :] == 0) and np.all(ux[:, :, -1] == 0)
+
+
+# ---------------------------------------------------------------------------------------
+# interface states, positivity, fluxes and update
+# ---------------------------------------------------------------------------------------
+
+def test_lake_at_rest_stays_at_rest():
+    """Flat ground, uniform layer, no motion: depths and momenta never change, the
+    hydrostatic fluxes cancel exactly; phi is recomputed as h_s/(h_s+h_f+H0) by the first
+    update and then stays fixed."""
+    o = Oracle(flat_lake(n=24, depth=3.0))
+    u0 = o.get("u")[:, 3:-3, 3:-3].copy()
+    o.step(1)
+    phi1 = o.get("u")[6, 3:-3, 3:-3].copy()
+    o.step(29)
+    u1 = o.get("u")[:, 3:-3, 3:-3]
+    assert np.array_equal(u0[:6], u1[:6])
+    assert np.all(u1[1] == 0) and np.all(u1[2] == 0) and np.all(u1[4] == 0) and np.all(u1[5] == 0)
+    assert np.all(phi1 == u0[0] / (u0[0] + u0[3] + H0)) and np.array_equal(u1[6], phi1)
+
+
+def test_interface_states_of_linear_data_are_exact():
+    """For data linear along x, minmod gives the exact slope and uE, uW at each face are the
+    face value (u[c] + u[c+1])/2."""
+    sc = flat_lake(n=20)
+    o = Oracle(sc)
+    u = o.get("u").copy()
+    cols = np.arange(u.shape[2], dtype=np.float64)
+    for k in range(7):
+        u[k] = (1.0 + k) + 0.25 * cols[None, :]
+    o.set_u(u)
+    o.faces()
+    uE, uW = o.get("uE"), o.get("uW")
+    # faces 3..nx+1 (the right boundary face uses the extrapolated ghost cell nx+3)
+    rows, cs = slice(3, -3), slice(3, -5)
+    face = 1.0 + 0.25 * (cols[3:-5] + 0.5)
+    for k in (1, 2, 4, 5):  # momenta (h stays positive, phi is recomputed)
+        assert np.array_equal(uE[k][rows, cs], np.broadcast_to(k + face, uE[k][rows, cs].shape))
+        assert np.array_equal(uW[k][rows, cs], np.broadcast_to(k + face, uW[k][rows, cs].shape))
+
+
+def test_mass_is_conserved():
+    sc = S.landslide_plane(n=64)
+    o = Oracle(sc)
+    m0 = mass(o)
+    o.step(60)
+    m1 = mass(o)
+    assert abs(m1[0] - m0[0]) <= 1e-13 * m0[0]
+    assert abs(m1[1] - m0[1]) <= 1e-13 * m0[1]
+
+
+def test_mirror_symmetry_across_the_slope_axis():
+    n, dx = 64, 20.0
+    s = n * dx / 5120.0
+    topo = S.inclined_plane(n, n, dx, x_break=3000 * s, blend=250 * s)
+    depth = S.paraboloid(n, n, dx, 1000 * s, (n - 1) * dx / 2, 1e6 * s * s, 150 * s)
+    sc = S.Scenario(mode=0, topo=topo, depth=depth, cellsize=dx, phiS0=0.6, delta0=20.0)
+    o = Oracle(sc)
+    o.step(40)
+    u = o.get("u")[:, 3:-3, 3:-3]
+    for k in (0, 1, 3, 4, 6):
+        assert np.array_equal(u[k], u[k][::-1]), k
+    for k in (2, 5):
+        assert np.array_equal(u[k], -u[k][::-1]), k
+
+
+# ---------------------------------------------------------------------------------------
+# dt: reduceKernel (float-rounded max), CFL3Kernel clamp
+# ---------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("depth,expect", [(50.0, "free"), (1e-12, "cap"), (1e8, "floor")])
+def test_dt_closed_form_for_a_lake(depth, expect):
+    phi, cfl = 0.5, 0.1
+    o = Oracle(flat_lake(n=16, depth=depth, phi=phi))
+    o.faces()
+    H = 10.0 * depth                      # scaled total depth (phi*c + (1-phi)*c, c = 1)
+    hs = H * phi
+    phif = hs / (hs + (H - hs) + H0)      # KeepPositivi2Kernel
+    dx = 200.0
+    em_f = math.sqrt(H / (1.0 - phif + H0))   # fluid speed, q = 0, invJ11 = 1
+    em_s = math.sqrt(H * (1.0 - 1.420 / 2.60))
+    em = max(em_f, em_s, 1e-10)
+    dmax = float(np.float32(em / dx))
+    assert o.scalar("dmax") == dmax
+    dt = min(2.0, cfl / dmax)
+    dt = max(0.001, dt)
+    assert o.scalar("dtval") == dt
+    assert {"free": 0.001 < dt < 2.0, "cap": dt == 2.0, "floor": dt == 0.001}[expect]
+
+
+def test_output_times_clip_dt():
+    sc = S.landslide_plane(n=24)
+    sc.total_sim, sc.each_out = 3.0, 1.0
+    o = Oracle(sc, outputs=True)
+    hits = []
+    for _ in range(200):
+        if o.step(1) == 0:
+            break
+        if o.scalar("lastWritten"):
+            hits.append((int(o.scalar("lastWritten")), o.scalar("time")))
+    assert [h[0] for h in hits] == [2, 3, 4]
+    for it, t in hits:
+        assert abs(t - (it - 1) * 10.0) <= 1e-12
+    assert o.scalar("nstop") == 1
+
+
+# ---------------------------------------------------------------------------------------
+# velocity-crossing limiter (Flux22..28)
+# ---------------------------------------------------------------------------------------
+
+def test_crossing_limiter_prevents_relative_velocity_reversal():
+    sc = flat_lake(n=20, depth=1.0)
+    sc.Cd = 4000.0           # strong drag: the explicit update overshoots
+    o = Oracle(sc)
+    u = o.get("u").copy()
+    us, uf = 0.8, -0.8
+    u[1] = u[0] * us
+    u[4] = u[3] * uf
+    o.set_u(u)
+    o.step(1)
+    one, old = o.get("uone")[:, 3:-3, 3:-3], o.get("uzero")[:, 3:-3, 3:-3]
+    s_new = one[1] / (one[0] + H0)
+    f_new = one[4] / (one[3] + H0)
+    d_old = old[1] / (old[0] + H0) - old[4] / (old[3] + H0)
+    fired = np.abs(s_new - f_new) <= np.abs(s_new) * H0 / one[3] * 1.01
+    assert fired.any()                      # the drag overshoot did flip the sign
+    assert np.all((s_new - f_new) * d_old >= -np.abs(d_old * s_new) * 1e-5)
+
+
+# ---------------------------------------------------------------------------------------
+# inflow injection (Inflow1/2Kernel, the dire rule)
+# ---------------------------------------------------------------------------------------
+
+def test_df004_injection_strips():
+    sc = df004(phi=0.5)
+    o = Oracle(sc)
+    before = o.get("u").copy()
+    o.inject()
+    u = o.get("u")
+    c = o.get("cvalue")
+    changed = np.argwhere((u != before).any(axis=0))
+    # locations X = 43, Y = 13..20 -> 6x1 strips at columns 43..48 (padded) on rows 13..20
+    expect = {(r, col) for r in range(13, 21) for col in range(43, 49)}
+    assert {tuple(x) for x in changed} <= expect
+    H = 10.0 * f32(0.017)
+    for r in range(13, 21):
+        for col in range(43, 49):
+            assert u[0, r, col] == H * (1 - 0.5) / 0.5
+            assert u[3, r, col] == H * (1.0 - 0.5) * c[r, col]
+            assert u[6, r, col] == u[0, r, col] / (u[0, r, col] + u[3, r, col] + H0)
+
+
+# ---------------------------------------------------------------------------------------
+# self-generated golden states (tests/golden/make_golden.py)
+# ---------------------------------------------------------------------------------------
+
+def test_oracle_reproduces_golden_states():
+    import hashlib
+    import os
+    import sys
+    sys.path.insert(0, GOLDEN)
+    from make_golden import CASES, run
+    g = np.load(os.path.join(GOLDEN, "oracle_golden.npz"))
+    for name in CASES:
+        u, t, dt = run(name)
+        h = hashlib.sha256(np.ascontiguousarray(u).tobytes()).hexdigest()
+        assert h == str(g[f"{name}/sha256"]), name
+        assert t == float(g[f"{name}/time"]) and dt == float(g[f"{name}/dt"])
