@@ -68,6 +68,9 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="after the timed steps, copy the 7 state planes once device to device "
+                         "(a plain copy kernel of known size, for HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (one MI355X per box); "
@@ -99,6 +102,10 @@ def main() -> int:
 
     u = sim.state()
     import numpy as np
+    if args.copy_calib:
+        sim.set_state(u)   # H2D of 7 planes, then one D2D copy of 7*nxd*nyd doubles
+        sim.sync()
+        print(f"# copy-calib: D2D copy of {7 * sim.P * 8} bytes", file=sys.stderr, flush=True)
     ok = bool(np.isfinite(u).all())
     ms, mf = sim.mass()
 

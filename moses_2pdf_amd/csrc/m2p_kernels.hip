@@ -480,6 +480,29 @@ __device__ __forceinline__ void flux_store(Tile T, int w, int lane, const double
   T[11][w][lane] = o.D5;
 }
 
+// Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).  Block b runs on
+// XCD b % 8; it is given tile number (b % 8) * per + b / 8 in row-major tile order, so
+// each XCD sweeps a contiguous band of tile rows and vertically adjacent tiles, which
+// share halo rows, meet in the same L2.
+#ifndef M2P_XCD_REMAP
+#define M2P_XCD_REMAP 1
+#endif
+constexpr int N_XCD = 8;
+__device__ __forceinline__ bool tile_of_block(const Grid& g, int& tx, int& ty) {
+  const int ntx = (g.nx + ST_TX - 1) / ST_TX;
+  const int nty = (g.ny + ST_TY - 1) / ST_TY;
+  const int b = blockIdx.x;
+  int t = b;
+  if (M2P_XCD_REMAP) {
+    const int per = (ntx * nty + N_XCD - 1) / N_XCD;
+    t = (b % N_XCD) * per + b / N_XCD;
+  }
+  if (t >= ntx * nty) return false;
+  tx = t % ntx;
+  ty = t / ntx;
+  return true;
+}
+
 template <int IO>
 __global__ void __launch_bounds__(512) M2P_STAGE_ATTR k_stage(Grid g, Phys ph, const double* __restrict__ Uin,
                                                const double* Uzero, double* Uout,
@@ -488,8 +511,10 @@ __global__ void __launch_bounds__(512) M2P_STAGE_ATTR k_stage(Grid g, Phys ph, c
   __shared__ double F[ST_NQ][ST_TY + 1][64];
   const int lane = threadIdx.x;
   const int w = threadIdx.y;
-  const int c0 = blockIdx.x * ST_TX + MD;
-  const int r0 = blockIdx.y * ST_TY + MD;
+  int tx, ty;
+  if (!tile_of_block(g, tx, ty)) return;
+  const int c0 = tx * ST_TX + MD;
+  const int r0 = ty * ST_TY + MD;
   const int xmax = g.nx + MD - 1, ymax = g.ny + MD - 1;
   const long long P = g.P;
 
@@ -709,7 +734,8 @@ void launch_dt(const DevParams& dp, DevState* st, const double* tf, const double
 void launch_stage(int io, const Grid& g, const Phys& ph, const double* Uin, const double* Uzero,
                   double* Uout, double* ONE, const double* M, const DevState* st, hipStream_t s) {
   const dim3 b(64, ST_TY + 1);
-  const dim3 gr((g.nx + ST_TX - 1) / ST_TX, (g.ny + ST_TY - 1) / ST_TY);
+  const int ntile = ((g.nx + ST_TX - 1) / ST_TX) * ((g.ny + ST_TY - 1) / ST_TY);
+  const dim3 gr(M2P_XCD_REMAP ? ((ntile + N_XCD - 1) / N_XCD) * N_XCD : ntile);
   if (io == 0)
     k_stage<0><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, st);
   else
