@@ -335,31 +335,33 @@ __global__ void __launch_bounds__(256) k_cfl(Grid g, Phys ph, const double* __re
   block_max_to(m, st);
 }
 
-// The ring [1, N-3]^2 minus the interior box, enumerated as 4 strips: rows 1-2 and
-// ny+3..ny+4 (full width 1..N-3), then columns 1-2 and nx+3..nx+4 on rows 3..ny+2.
-__global__ void __launch_bounds__(256) k_cfl_ring(Grid g, Phys ph, const double* __restrict__ U,
-                                                  const double* __restrict__ M,
-                                                  DevState* __restrict__ st) {
+// The ring [1, N-3]^2 minus the interior box, one thread per (face index, direction).
+// Face indices are enumerated as 4 strips: rows 1-2 and ny+3..ny+4 (columns 1..N-3), then
+// columns 1-2 and nx+3..nx+4 on rows 3..ny+2.
+__global__ void __launch_bounds__(64) k_cfl_ring(Grid g, Phys ph, const double* __restrict__ U,
+                                                 const double* __restrict__ M,
+                                                 DevState* __restrict__ st) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = t >> 1, dir = t & 1;
   const int w = g.nxd - 3;        // columns 1..N-3
   const int nrow = 4 * w;         // 4 ghost face rows
   const int ncol = 4 * g.ny;      // 4 ghost face columns on interior rows
   double m = 0.0;
-  if (t < nrow + ncol) {
+  if (f < nrow + ncol) {
     int r, c;
-    if (t < nrow) {
-      const int k = t / w;
+    if (f < nrow) {
+      const int k = f / w;
       r = (k < 2) ? 1 + k : g.ny + MD + (k - 2);
-      c = 1 + t % w;
+      c = 1 + f % w;
     } else {
-      const int q = t - nrow, k = q / g.ny;
+      const int q = f - nrow, k = q / g.ny;
       r = MD + q % g.ny;
       c = (k < 2) ? 1 + k : g.nx + MD + (k - 2);
     }
-    m = face_cfl(U, M, g, ph, 0, r, c, true);
-    m = fmax(m, face_cfl(U, M, g, ph, 1, r, c, true));
+    m = face_cfl(U, M, g, ph, dir, r, c, true);
   }
-  block_max_to(m, st);
+  m = wave_max(m);
+  if (threadIdx.x == 0) atomicMax(&st->maxbits, (unsigned long long)__double_as_longlong(m));
 }
 
 // CFL3Kernel + the host's dt control + CFL4Kernel (L/K:2006-2036, L/R:978-991; inflow
@@ -420,7 +422,10 @@ __device__ __forceinline__ PDSide pd_side(const double* __restrict__ U, const Gr
 #ifndef M2P_STAGE_ATTR
 #define M2P_STAGE_ATTR __attribute__((amdgpu_waves_per_eu(4, 4)))
 #endif
-constexpr int ST_TX = 63, ST_TY = 7, ST_NQ = 12;
+#ifndef M2P_ST_TY
+#define M2P_ST_TY 7
+#endif
+constexpr int ST_TX = 63, ST_TY = M2P_ST_TY, ST_NQ = 12;
 typedef double (*Tile)[ST_TY + 1][64];
 
 // Phase A: InterfacesKernel states of one face into the tile (L: q 0-5, R: q 6-11).
@@ -504,7 +509,7 @@ __device__ __forceinline__ bool tile_of_block(const Grid& g, int& tx, int& ty) {
 }
 
 template <int IO>
-__global__ void __launch_bounds__(512) M2P_STAGE_ATTR k_stage(Grid g, Phys ph, const double* __restrict__ Uin,
+__global__ void __launch_bounds__(64 * (ST_TY + 1)) M2P_STAGE_ATTR k_stage(Grid g, Phys ph, const double* __restrict__ Uin,
                                                const double* Uzero, double* Uout,
                                                double* __restrict__ ONE, const double* __restrict__ M,
                                                const DevState* __restrict__ st) {
@@ -722,8 +727,8 @@ void launch_cfl(const Grid& g, const Phys& ph, const double* U, const double* M,
                 hipStream_t s) {
   const dim3 b(64, 4);
   k_cfl<<<grid2(g.nx, g.ny, b), b, 0, s>>>(g, ph, U, M, st);
-  const int ring = 4 * (g.nxd - 3) + 4 * g.ny;
-  k_cfl_ring<<<(ring + 255) / 256, 256, 0, s>>>(g, ph, U, M, st);
+  const int ring = 2 * (4 * (g.nxd - 3) + 4 * g.ny);
+  k_cfl_ring<<<(ring + 63) / 64, 64, 0, s>>>(g, ph, U, M, st);
 }
 
 void launch_dt(const DevParams& dp, DevState* st, const double* tf, const double* itime,

@@ -43,18 +43,21 @@ __device__ __forceinline__ long long idx(const Grid& g, int r, int c) {
   return (long long)r * g.nxd + c;
 }
 
-// TVD1..5Kernel (L/K:1000-1119) for one component: slope at a cell from its left/right
-// (or lower/upper) neighbours.  dA = dux[c-1], dB = dux[c].
+// TVD1..5Kernel (L/K:1000-1119) for one component: the slope at a cell from its two
+// neighbours, dA = dux[c-1], dB = dux[c].  The reference forms
+//   t1 = (sA+sB)/2 * min(dA*sA, dB*sB),  t2 = (dA+dB)/2,  slope = (sA'+sB')/2 * min(t1*sA', t2*sB')
+// with s = (x > 0 ? 1 : -1).  That equals minmod(dA, dB) for every input: when dA and dB
+// are both positive or both negative it is sign(dA)*min(|dA|,|dB|) exactly (|t2| >= |t1|
+// holds after rounding because rounding is monotonic), otherwise a zero.  The signs are
+// tested directly (a product dA*dB underflows for the tiny differences at a wet/dry front).
+// Only the sign of a zero slope can differ, and a zero's sign never reaches a nonzero value
+// in this scheme (no quantity is divided by a slope and no branch tests the sign of a zero).
 __device__ __forceinline__ double tvd_slope(double um, double u0, double up) {
   const double dA = u0 - um;
   const double dB = up - u0;
-  const double sA = (dA > 0.0) ? 1.0 : -1.0;
-  const double sB = (dB > 0.0) ? 1.0 : -1.0;
-  const double t1 = 0.5 * (sA + sB) * fmin(dA * sA, dB * sB);
-  const double t2 = 0.5 * (dA + dB);
-  const double sA2 = (t1 > 0.0) ? 1.0 : -1.0;
-  const double sB2 = (t2 > 0.0) ? 1.0 : -1.0;
-  return 0.5 * (sA2 + sB2) * fmin(t1 * sA2, t2 * sB2);
+  const double m = copysign(fmin(fabs(dA), fabs(dB)), dA);
+  const bool same = (dA > 0.0 && dB > 0.0) || (dA < 0.0 && dB < 0.0);
+  return same ? m : 0.0;
 }
 
 // Interface states of InterfacesKernel (L/K:1226-1252) for one face, components 0..5
@@ -98,7 +101,7 @@ __device__ __forceinline__ void x_overridden(const double* __restrict__ U, const
   }
 }
 
-__device__ __noinline__ void final_states_edge(const double* __restrict__ U, const Grid& g,
+__device__ __forceinline__ void final_states_edge(const double* __restrict__ U, const Grid& g,
                                                int dir, int r, int c, double L[6], double R[6]) {
   if (r <= 2 || r >= g.ny + MD) {
     const int ra = (r <= 2) ? MD : g.ny + MD - 1;

@@ -34,8 +34,15 @@ def _knobs(sc):
     return sc
 
 
+def canonical(a: np.ndarray) -> np.ndarray:
+    """-0.0 -> +0.0: the sign of a zero never reaches a nonzero value in this scheme, and
+    implementations may differ in it."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return np.where(a == 0.0, 0.0, a)
+
+
 def digest(a: np.ndarray) -> str:
-    return hashlib.sha256(np.ascontiguousarray(a, dtype=np.float64).tobytes()).hexdigest()
+    return hashlib.sha256(canonical(a).tobytes()).hexdigest()
 
 
 def run(name):

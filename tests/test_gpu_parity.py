@@ -134,18 +134,17 @@ def test_c1_workload_parity_256():
 
 def test_gpu_matches_committed_golden_states():
     """The HIP path against the oracle states frozen in tests/golden/oracle_golden.npz."""
-    import hashlib
     import os
     from util import GOLDEN
     sys.path.insert(0, GOLDEN)
-    from make_golden import CASES
+    from make_golden import CASES, digest
     gold = np.load(os.path.join(GOLDEN, "oracle_golden.npz"))
     for name, (make, steps) in CASES.items():
         with Simulation(make()) as g:
             g.step(steps)
             u = np.ascontiguousarray(interior(g.state()))
             t, _ = g.time_steps()
-        h = hashlib.sha256(u.tobytes()).hexdigest()
+        h = digest(u)
         say("golden", name, h[:16], str(gold[f"{name}/sha256"])[:16])
         assert h == str(gold[f"{name}/sha256"]), name
         assert t == float(gold[f"{name}/time"])

@@ -279,14 +279,13 @@ def test_df004_injection_strips():
 # ---------------------------------------------------------------------------------------
 
 def test_oracle_reproduces_golden_states():
-    import hashlib
     import os
     import sys
     sys.path.insert(0, GOLDEN)
-    from make_golden import CASES, run
+    from make_golden import CASES, digest, run
     g = np.load(os.path.join(GOLDEN, "oracle_golden.npz"))
     for name in CASES:
         u, t, dt = run(name)
-        h = hashlib.sha256(np.ascontiguousarray(u).tobytes()).hexdigest()
+        h = digest(u)
         assert h == str(g[f"{name}/sha256"]), name
         assert t == float(g[f"{name}/time"]) and dt == float(g[f"{name}/dt"])
