@@ -24,11 +24,13 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # Algorithmic HBM bytes per interior cell for each pass of one step (FP64): the state
 # planes a pass must read and write plus the topography, with the metric recomputable
 # from the topography (so the stored-metric reads are not counted).
-ALG_BYTES = {
-    "cfl": 7 * 8 + 8,                 # read u^n
-    "stage0": (7 + 7 + 2) * 8 + 8,    # read u^n, write u^{n+1/2} and 2 limiter planes
-    "stage1": (7 + 6 + 2 + 7) * 8 + 8,  # read u^{n+1/2}, u^n, limiter; write u^{n+1}
+PLANES = {
+    "cfl": 7 + 1,                 # read u^n
+    "stage0": (7 + 7 + 2) + 1,    # read u^n, write u^{n+1/2} and 2 limiter planes
+    "stage1": (7 + 6 + 2 + 7) + 1,  # read u^{n+1/2}, u^n, limiter; write u^{n+1}
 }
+ELEM = {"fp64": 8, "fp32": 4}
+ALG_BYTES = {k: v * 8 for k, v in PLANES.items()}
 
 
 def traffic_from_profiles(kernel: str, n: int):
@@ -68,6 +70,8 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--precision", choices=sorted(ELEM), default="fp64",
+                    help="fp64: the reference's arithmetic (the headline); fp32: FP32-bulk")
     ap.add_argument("--copy-calib", action="store_true",
                     help="after the timed steps, copy the 7 state planes once device to device "
                          "(a plain copy kernel of known size, for HBM counter calibration)")
@@ -82,7 +86,8 @@ def main() -> int:
 
     sc = S.landslide_fractal(n=args.n, seed=2)
     print(f"# workload ready: {sc.nx}x{sc.ny}", file=sys.stderr, flush=True)
-    sim = Simulation(sc)
+    sim = Simulation(sc, precision=args.precision)
+    alg_bytes = {k: v * ELEM[args.precision] for k, v in PLANES.items()}
     sim.step(args.warmup)
     sim.sync()
     print(f"# warmup {args.warmup} steps done", file=sys.stderr, flush=True)
@@ -112,10 +117,10 @@ def main() -> int:
     totals = {k: v[0] * v[1] for k, v in kern.items()}
     dom = max(("cfl", "stage0", "stage1"), key=lambda k: totals[k])
     avg_ms = kern[dom][0]
-    alg = ALG_BYTES[dom] * cells
+    alg = alg_bytes[dom] * cells
     achieved = alg / (avg_ms * 1e-3) / 1e9
-    traffic = traffic_from_profiles(dom, args.n)
-    step_alg = sum(ALG_BYTES.values()) * cells
+    traffic = traffic_from_profiles(dom, args.n) if args.precision == "fp64" else None
+    step_alg = sum(alg_bytes.values()) * cells
     out = {
         "metric": "cell-updates/s",
         "value": value,
@@ -127,15 +132,15 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32 (f64 dt, CFL max, mass)",
         "data": "synthetic",
-        "config": {"workload": f"landslide FP64, {sc.nx}x{sc.ny} cells at 20 m, seeded "
+        "config": {"workload": f"landslide {args.precision.upper()}, {sc.nx}x{sc.ny} cells at 20 m, seeded "
                                "diamond-square DEM (H 0.8, 1500 m relief, 3-cell box), "
                                "3 paraboloid releases, per-cell phi 0.4-0.7",
-                   "grid": [sc.nx, sc.ny], "cfl": sc.cfl, "precision": "fp64"},
+                   "grid": [sc.nx, sc.ny], "cfl": sc.cfl, "precision": args.precision},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "alg_bytes_per_cell": ALG_BYTES[dom]},
+                     "alg_bytes_per_cell": alg_bytes[dom]},
         "step_hbm_alg_GBs": step_alg * args.steps / elapsed / 1e9,
         "kernels_ms": {k: v[0] for k, v in kern.items()},
         "state_finite": ok,

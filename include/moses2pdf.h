@@ -27,7 +27,11 @@ extern "C" {
 typedef struct m2p_ctx m2p_ctx;
 
 enum { M2P_LANDSLIDE = 0, M2P_INFLOW = 1 };
-enum { M2P_FP64 = 0 };
+/* Scalar type of the time step (m2p_params.precision).  M2P_FP64 is the reference's
+ * arithmetic.  M2P_FP32 keeps the state and the per-step arithmetic in float (hardware
+ * reciprocal and square root); the metric is computed in FP64 and rounded once, and the
+ * CFL max, the dt control, the simulated time and the mass totals stay FP64. */
+enum { M2P_FP64 = 0, M2P_FP32 = 1 };
 
 enum {
   M2P_OK = 0,
@@ -53,6 +57,8 @@ typedef struct m2p_params {
   double rho_ratio;   /* reference 1.420/2.60 */
   double dt_min;      /* reference 0.001 (landslide), 0.002 (inflow) */
   double dt_max;      /* reference 2.0 */
+  int precision;      /* M2P_FP64 (default) or M2P_FP32 */
+  int reserved;       /* 0 */
 } m2p_params;
 
 /* Fills *p with the reference's values for `mode` (par_List values of the shipped
@@ -63,7 +69,8 @@ void m2p_default_params(int mode, m2p_params* p);
  * par_List at `par_list` (9 lines landslide, 11 lines inflow; '#' starts a comment) and
  * the files it names (relative paths resolve against the current directory, as in the
  * reference), parses them with the reference's precision rules and uploads them.
- * `overlay` is NULL or "key=value;key=value" over cfl, rho_ratio, dt_min, dt_max. */
+ * `overlay` is NULL or "key=value;key=value" over cfl, rho_ratio, dt_min, dt_max and
+ * precision (fp64 | fp32). */
 int m2p_open(const char* par_list, int mode, const char* overlay, m2p_ctx** out);
 
 /* In-memory constructor (no files).  topo[NY*NX] and depth[NY*NX] hold the grid values
@@ -87,6 +94,13 @@ int m2p_run(m2p_ctx* ctx, const char* result_dir, double* wall_s);
  * host synchronisation inside the loop.  No reference counterpart as a call: it is the
  * body of the kernelStep loop (adNOC_2Pb_runKernels.cu:646-1330) without the file output. */
 int m2p_step(m2p_ctx* ctx, int nsteps);
+
+/* Advances until the simulated time (scaled, as m2p_get_state reports it) reaches t_end,
+ * the last step shortened to end on it, or until max_steps steps are done; *steps_done
+ * (nullable) receives the count.  No reference counterpart: it replaces the output-time
+ * clipping of the loop (adNOC_2Pb_runKernels.cu:983-988) for a single end time, so that
+ * runs of different precision or dt can be compared at the same time. */
+int m2p_run_to(m2p_ctx* ctx, double t_end, int max_steps, int* steps_done);
 
 /* Waits for all work queued on the context's stream. */
 int m2p_sync(m2p_ctx* ctx);

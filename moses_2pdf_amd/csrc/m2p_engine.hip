@@ -2,7 +2,10 @@
 // preparation, the time loop driver and the C ABI declared in include/moses2pdf.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
+#include <climits>
+#include <cstddef>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -29,8 +32,14 @@ struct m2p_ctx {
   std::vector<double> table, loc, itime;
   std::vector<int> dire;
   hipStream_t s = nullptr;
+  int prec = M2P_FP64;     // scalar type of the state and of the per-step metric copy
+  size_t es = 8;           // its size in bytes
   double *dTopo = nullptr, *dDepth = nullptr, *dPhi = nullptr, *dM = nullptr;
-  double *dU0 = nullptr, *dU1 = nullptr, *dONE = nullptr, *dTf = nullptr, *dMass = nullptr;
+  // state planes u^n (U0), u^{n+1/2} (U1), the limited fluid momenta (ONE) and the metric
+  // the step kernels read (MT: dM itself in FP64, a float copy in FP32), in type `prec`
+  void *dU0 = nullptr, *dU1 = nullptr, *dONE = nullptr, *dMT = nullptr;
+  unsigned char* dLIM = nullptr;  // FP32 only: limiter memory flags, one byte per cell
+  double *dTf = nullptr, *dMass = nullptr;
   double *dTable = nullptr, *dLoc = nullptr, *dItime = nullptr;
   int* dDire = nullptr;
   DevState* dSt = nullptr;
@@ -115,10 +124,16 @@ void release(m2p_ctx* c) {
   for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
   c->evpool.clear();
   c->pending.clear();
-  void* ps[] = {c->dTopo, c->dDepth, c->dPhi, c->dM,     c->dU0,   c->dU1,  c->dONE,
-                c->dTf,   c->dMass,  c->dTable, c->dLoc, c->dItime, c->dDire, c->dSt};
+  if (c->dMT == c->dM) c->dMT = nullptr;
+  void* ps[] = {c->dTopo, c->dDepth, c->dPhi, c->dM,     c->dU0,   c->dU1,    c->dONE, c->dMT,
+                c->dTf,   c->dMass,  c->dTable, c->dLoc, c->dItime, c->dDire, c->dSt,  c->dLIM};
   for (void* p : ps)
     if (p) (void)hipFree(p);
+  c->dTopo = c->dDepth = c->dPhi = c->dM = c->dTf = c->dMass = c->dTable = c->dLoc = c->dItime = nullptr;
+  c->dU0 = c->dU1 = c->dONE = c->dMT = nullptr;
+  c->dLIM = nullptr;
+  c->dDire = nullptr;
+  c->dSt = nullptr;
   if (c->s) (void)hipStreamDestroy(c->s);
   c->s = nullptr;
 }
@@ -129,12 +144,26 @@ int setup(m2p_ctx* c) {
   HIPCHK(c, m2p::probe_kernels());
   const size_t P = (size_t)c->P;
   int rc;
+  double* dU64 = nullptr;  // FP64 initial state (the metric setup writes it)
   if ((rc = dalloc(c, &c->dTopo, P)) || (rc = dalloc(c, &c->dDepth, P)) ||
-      (rc = dalloc(c, &c->dM, (size_t)m2p::MC_COUNT * P)) || (rc = dalloc(c, &c->dU0, 7 * P)) ||
-      (rc = dalloc(c, &c->dU1, 7 * P)) || (rc = dalloc(c, &c->dONE, 2 * P)) ||
+      (rc = dalloc(c, &c->dM, (size_t)m2p::MC_COUNT * P)) || (rc = dalloc(c, &dU64, 7 * P)) ||
       (rc = dalloc(c, &c->dTf, c->tf.size())) || (rc = dalloc(c, &c->dMass, 4 * (size_t)m2p::mass_blocks())) ||
-      (rc = dalloc(c, &c->dSt, 1)))
+      (rc = dalloc(c, &c->dSt, 1))) {
+    if (dU64) (void)hipFree(dU64);
     return rc;
+  }
+  if (c->prec == M2P_FP64) {
+    c->dU0 = dU64;
+    c->dMT = c->dM;
+    if ((rc = dalloc(c, (double**)&c->dU1, 7 * P)) || (rc = dalloc(c, (double**)&c->dONE, 2 * P))) return rc;
+  } else {
+    if ((rc = dalloc(c, (float**)&c->dU0, 7 * P)) || (rc = dalloc(c, (float**)&c->dU1, 7 * P)) ||
+        (rc = dalloc(c, (float**)&c->dONE, 2 * P)) ||
+        (rc = dalloc(c, (float**)&c->dMT, (size_t)m2p::MC_COUNT * P)) || (rc = dalloc(c, &c->dLIM, P))) {
+      (void)hipFree(dU64);
+      return rc;
+    }
+  }
   if (!c->phi.empty() && (rc = dalloc(c, &c->dPhi, P))) return rc;
   if (c->mode == 1) {
     if ((rc = dalloc(c, &c->dTable, c->table.size())) || (rc = dalloc(c, &c->dLoc, c->loc.size())) ||
@@ -167,12 +196,53 @@ int setup(m2p_ctx* c) {
 
   DevState st{};
   st.iter = 1;
+  st.t_end = -1.0;
+  st.halt_at = INT_MAX;
   HIPCHK(c, hipMemcpyAsync(c->dSt, &st, sizeof st, hipMemcpyHostToDevice, c->s));
   m2p::launch_metric_setup(c->g, c->mode, c->dTopo, c->dDepth, c->dPhi, c->par.phiS0, c->dM,
-                           c->dU0, c->s);
+                           dU64, c->s);
+  if (c->prec == M2P_FP32) {
+    m2p::launch_convert(dU64, (float*)c->dU0, (long long)(7 * P), c->s);
+    m2p::launch_convert(c->dM, (float*)c->dMT, (long long)(m2p::MC_COUNT * P), c->s);
+  }
   HIPCHK(c, hipGetLastError());
   // the first predictor's corner ghosts come from the initial state's own ghost rows
-  HIPCHK(c, hipMemcpyAsync(c->dU1, c->dU0, 7 * P * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  HIPCHK(c, hipMemcpyAsync(c->dU1, c->dU0, 7 * P * c->es, hipMemcpyDeviceToDevice, c->s));
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  if (c->prec == M2P_FP32) (void)hipFree(dU64);
+  // the grids are only needed by the setup
+  (void)hipFree(c->dTopo);
+  (void)hipFree(c->dDepth);
+  if (c->dPhi) (void)hipFree(c->dPhi);
+  c->dTopo = c->dDepth = c->dPhi = nullptr;
+  return 0;
+}
+
+// State transfers in the reference's FP64 layout, whatever the context's precision.
+int download_state(m2p_ctx* c, void* dsrc, size_t planes, double* u) {
+  const size_t n = planes * (size_t)c->P;
+  if (c->prec == M2P_FP64) {
+    HIPCHK(c, hipMemcpyAsync(u, dsrc, n * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return 0;
+  }
+  std::vector<float> f(n);
+  HIPCHK(c, hipMemcpyAsync(f.data(), dsrc, n * sizeof(float), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  for (size_t i = 0; i < n; i++) u[i] = (double)f[i];
+  return 0;
+}
+
+int upload_state(m2p_ctx* c, void* ddst, const double* u) {
+  const size_t n = 7 * (size_t)c->P;
+  if (c->prec == M2P_FP64) {
+    HIPCHK(c, hipMemcpyAsync(ddst, u, n * sizeof(double), hipMemcpyHostToDevice, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return 0;
+  }
+  std::vector<float> f(n);
+  for (size_t i = 0; i < n; i++) f[i] = (float)u[i];
+  HIPCHK(c, hipMemcpyAsync(ddst, f.data(), n * sizeof(float), hipMemcpyHostToDevice, c->s));
   HIPCHK(c, hipStreamSynchronize(c->s));
   return 0;
 }
@@ -232,7 +302,12 @@ void resolve_profile(m2p_ctx* c) {
 }
 
 // One pass of the kernelStep loop body (L/R:646-1330; inflow I/R:721-1424).
-void enqueue_step(m2p_ctx* c, int ntf) {
+template <typename T>
+void enqueue_step_t(m2p_ctx* c, int ntf) {
+  T* U0 = (T*)c->dU0;
+  T* U1 = (T*)c->dU1;
+  T* ONE = (T*)c->dONE;
+  const T* MT = (const T*)c->dMT;
   DevParams dp{};
   dp.cfl = c->par.cfl;
   dp.dt_min = c->par.dt_min;
@@ -241,16 +316,16 @@ void enqueue_step(m2p_ctx* c, int ntf) {
   dp.nsamp = c->mode == 1 ? c->nsamp : 0;
   if (c->mode == 1) {
     Timer t(c, "inflow");
-    m2p::launch_inflow(c->g, c->dU0, c->dM, c->dTable, c->dLoc, c->dDire, c->nloc, c->nsamp,
-                       c->par.phiS0, c->dSt, c->s);
+    m2p::launch_inflow(c->g, U0, c->dM, c->dTable, c->dLoc, c->dDire, c->nloc, c->nsamp,
+                       c->par.phiS0, c->dSt, c->dLIM, c->s);
   }
   {
     Timer t(c, "ghost");
-    m2p::launch_ghost(c->g, c->mode, c->dU0, c->dU1, c->s);
+    m2p::launch_ghost(c->g, c->mode, U0, (const T*)U1, c->dSt, 0, c->s);
   }
   {
     Timer t(c, "cfl");
-    m2p::launch_cfl(c->g, c->ph, c->dU0, c->dM, c->dSt, c->s);
+    m2p::launch_cfl(c->g, c->ph, (const T*)U0, MT, c->dSt, c->s);
   }
   {
     Timer t(c, "dt");
@@ -258,16 +333,23 @@ void enqueue_step(m2p_ctx* c, int ntf) {
   }
   {
     Timer t(c, "stage0");
-    m2p::launch_stage(0, c->g, c->ph, c->dU0, c->dU0, c->dU1, c->dONE, c->dM, c->dSt, c->s);
+    m2p::launch_stage(0, c->g, c->ph, (const T*)U0, (const T*)U0, U1, ONE, MT, c->dLIM, c->dSt, c->s);
   }
   {
     Timer t(c, "ghost");
-    m2p::launch_ghost(c->g, c->mode, c->dU1, c->dU0, c->s);
+    m2p::launch_ghost(c->g, c->mode, U1, (const T*)U0, c->dSt, 1, c->s);
   }
   {
     Timer t(c, "stage1");
-    m2p::launch_stage(1, c->g, c->ph, c->dU1, c->dU0, c->dU0, c->dONE, c->dM, c->dSt, c->s);
+    m2p::launch_stage(1, c->g, c->ph, (const T*)U1, (const T*)U0, U0, ONE, MT, c->dLIM, c->dSt, c->s);
   }
+}
+
+void enqueue_step(m2p_ctx* c, int ntf) {
+  if (c->prec == M2P_FP32)
+    enqueue_step_t<float>(c, ntf);
+  else
+    enqueue_step_t<double>(c, ntf);
 }
 
 int parse_overlay(const char* ov, m2p_params& p) {
@@ -282,8 +364,14 @@ int parse_overlay(const char* ov, m2p_params& p) {
     if (kv.empty()) continue;
     size_t eq = kv.find('=');
     if (eq == std::string::npos) return -1;
-    std::string k = kv.substr(0, eq);
-    double v = std::strtod(kv.c_str() + eq + 1, nullptr);
+    std::string k = kv.substr(0, eq), val = kv.substr(eq + 1);
+    if (k == "precision") {
+      if (val == "fp64") p.precision = M2P_FP64;
+      else if (val == "fp32") p.precision = M2P_FP32;
+      else return -1;
+      continue;
+    }
+    double v = std::strtod(val.c_str(), nullptr);
     if (k == "cfl") p.cfl = v;
     else if (k == "rho_ratio") p.rho_ratio = v;
     else if (k == "dt_min") p.dt_min = v;
@@ -311,6 +399,8 @@ void m2p_default_params(int mode, m2p_params* p) {
   p->rho_ratio = 1.420 / 2.60;
   p->dt_min = mode == 0 ? 0.001 : 0.002;
   p->dt_max = 2.0;
+  p->precision = M2P_FP64;
+  p->reserved = 0;
 }
 
 int m2p_open_grid(int mode, int NX, int NY, double cellsize, const double* topo,
@@ -323,8 +413,11 @@ int m2p_open_grid(int mode, int NX, int NY, double cellsize, const double* topo,
     return M2P_EINVAL;
   if (mode == 0 && !depth) return M2P_EINVAL;
   if (mode == 1 && (nsamp < 1 || nloc < 1 || !t || !h || !u || !v || !loc)) return M2P_EINVAL;
+  if (par->precision != M2P_FP64 && par->precision != M2P_FP32) return M2P_EINVAL;
   m2p_ctx* c = new m2p_ctx();
   c->mode = mode;
+  c->prec = par->precision;
+  c->es = c->prec == M2P_FP64 ? sizeof(double) : sizeof(float);
   c->NX = NX;
   c->NY = NY;
   c->nxd = NX + 2 * MD;
@@ -450,6 +543,40 @@ int m2p_step(m2p_ctx* c, int nsteps) {
   return 0;
 }
 
+int m2p_run_to(m2p_ctx* c, double t_end, int max_steps, int* steps_done) {
+  if (!c || !c->s) return M2P_ESTATE;
+  if (!(t_end > 0) || max_steps < 0) return fail(c, M2P_EINVAL, "t_end must be > 0, max_steps >= 0");
+  DevState st{};
+  HIPCHK(c, hipMemcpyAsync(&st, c->dSt, sizeof st, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  const int nt0 = st.nt;
+  if (st.time < t_end && max_steps > 0) {
+    st.t_end = t_end;
+    st.halt_at = (max_steps > INT_MAX - nt0) ? INT_MAX : nt0 + max_steps;
+    st.skip = 0;
+    HIPCHK(c, hipMemcpyAsync(c->dSt, &st, sizeof st, hipMemcpyHostToDevice, c->s));
+    // chunks of steps; the kernels of a halted scenario return at once, so a chunk may run
+    // past the end
+    int issued = 0;
+    while (issued < max_steps) {
+      const int n = std::min(64, max_steps - issued);
+      for (int k = 0; k < n; k++) enqueue_step(c, 0);
+      issued += n;
+      HIPCHK(c, hipGetLastError());
+      HIPCHK(c, hipMemcpyAsync(&st, c->dSt, sizeof st, hipMemcpyDeviceToHost, c->s));
+      HIPCHK(c, hipStreamSynchronize(c->s));
+      if (st.skip || st.nt >= st.halt_at) break;
+    }
+    st.t_end = -1.0;
+    st.halt_at = INT_MAX;
+    st.skip = 0;
+    HIPCHK(c, hipMemcpyAsync(c->dSt, &st, sizeof st, hipMemcpyHostToDevice, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+  }
+  if (steps_done) *steps_done = st.nt - nt0;
+  return 0;
+}
+
 int m2p_sync(m2p_ctx* c) {
   if (!c || !c->s) return M2P_ESTATE;
   HIPCHK(c, hipStreamSynchronize(c->s));
@@ -488,7 +615,7 @@ int m2p_run(m2p_ctx* c, const char* result_dir, double* wall_s) {
     HIPCHK(c, hipMemcpyAsync(&st, c->dSt, sizeof st, hipMemcpyDeviceToHost, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
     if (st.schreiben) {
-      HIPCHK(c, hipMemcpy(u.data(), c->dU0, 7 * P * sizeof(double), hipMemcpyDeviceToHost));
+      if (int rc = download_state(c, c->dU0, 7, u.data())) return rc;
       outtime[st.iter - 1] = st.step_start;
       std::string e = m2p::write_time(dir, outtime, st.iter);
       if (e.empty()) e = m2p::write_result(dir, st.iter, og, u.data());
@@ -514,8 +641,9 @@ int m2p_get_state(m2p_ctx* c, double* u, double* t, int* nt) {
   if (!c || !c->s) return M2P_ESTATE;
   DevState st{};
   HIPCHK(c, hipMemcpyAsync(&st, c->dSt, sizeof st, hipMemcpyDeviceToHost, c->s));
-  if (u) HIPCHK(c, hipMemcpyAsync(u, c->dU0, 7 * (size_t)c->P * sizeof(double), hipMemcpyDeviceToHost, c->s));
   HIPCHK(c, hipStreamSynchronize(c->s));
+  if (u)
+    if (int rc = download_state(c, c->dU0, 7, u)) return rc;
   if (t) *t = st.time;
   if (nt) *nt = st.nt;
   return 0;
@@ -524,9 +652,9 @@ int m2p_get_state(m2p_ctx* c, double* u, double* t, int* nt) {
 int m2p_set_state(m2p_ctx* c, const double* u) {
   if (!c || !c->s) return M2P_ESTATE;
   if (!u) return fail(c, M2P_EINVAL, "null state");
-  const size_t n = 7 * (size_t)c->P * sizeof(double);
-  HIPCHK(c, hipMemcpyAsync(c->dU0, u, n, hipMemcpyHostToDevice, c->s));
-  HIPCHK(c, hipMemcpyAsync(c->dU1, c->dU0, n, hipMemcpyDeviceToDevice, c->s));
+  if (int rc = upload_state(c, c->dU0, u)) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->dU1, c->dU0, 7 * (size_t)c->P * c->es, hipMemcpyDeviceToDevice, c->s));
+  if (c->dLIM) HIPCHK(c, hipMemsetAsync(c->dLIM, 0, (size_t)c->P, c->s));
   HIPCHK(c, hipStreamSynchronize(c->s));
   return 0;
 }
@@ -544,7 +672,10 @@ int m2p_mass(m2p_ctx* c, double* solid, double* fluid) {
   if (!c || !c->s) return M2P_ESTATE;
   const int nb = m2p::mass_blocks();
   std::vector<double> part(4 * (size_t)nb);
-  m2p::launch_mass(c->g, c->dU0, c->dM, c->dMass, c->s);
+  if (c->prec == M2P_FP32)
+    m2p::launch_mass(c->g, (const float*)c->dU0, c->dM, c->dMass, c->s);
+  else
+    m2p::launch_mass(c->g, (const double*)c->dU0, c->dM, c->dMass, c->s);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(part.data(), c->dMass, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->s));
   HIPCHK(c, hipStreamSynchronize(c->s));
@@ -592,7 +723,7 @@ int m2p_get_field(m2p_ctx* c, const char* name, double* out) {
   const size_t P = (size_t)c->P;
   std::string n(name);
   if (n == "uone45") {
-    HIPCHK(c, hipMemcpy(out, c->dONE, 2 * P * sizeof(double), hipMemcpyDeviceToHost));
+    if (int rc = download_state(c, c->dONE, 2, out)) return rc;
     return 2;
   }
   auto it = tab.find(n);

@@ -16,14 +16,27 @@ struct DevState {
   double dt_corr;     // dtval   (io = 1)
   double dmax;        // float-rounded max wave speed
   double step_start;  // TotalTime at the start of the last step
-  unsigned long long maxbits;  // running max (IEEE bits of a positive double)
+  double t_end;       // > 0: the step that reaches it is shortened to end on it, then the
+                      //      scenario halts
+  double vstop2;      // > 0: halt once the squared max flow speed falls below it (after
+                      //      it has been reached once)
+  unsigned long long maxbits;   // running max wave speed (IEEE bits of a positive double)
+  unsigned long long vmaxbits;  // squared max flow speed of the last corrector (IEEE bits)
   int iter;           // next output index (L/R:390)
   int schreiben;      // output due after this step
   int nstop;
   int inflowCount;
   int nt;             // steps done
-  int pad;
+  int halt_at;        // step count after which every kernel of this scenario returns at once
+  int skip;           // halted (set by the dt kernel for the kernels after it in a step)
+  int moved;          // the flow speed has reached the stop threshold at least once
 };
+
+// A scenario is halted for the kernels that run before the dt kernel of a step
+// (inflow, ghost fill, CFL) once it has done halt_at steps, and for those after it once
+// the dt kernel has set `skip`.
+__device__ __forceinline__ bool halted_pre(const DevState* st) { return st->nt >= st->halt_at; }
+__device__ __forceinline__ bool halted_post(const DevState* st) { return st->skip != 0; }
 
 struct DevParams {
   double cfl, dt_min, dt_max;
@@ -34,18 +47,25 @@ struct DevParams {
 void launch_metric_setup(const Grid& g, int mode, const double* topo, const double* depth,
                          const double* phicell, double phiS0, double* M, double* U,
                          hipStream_t s);
-void launch_inflow(const Grid& g, double* U, const double* M, const double* table,
-                   const double* loc, const int* dire, int nloc, int nsamp, double phiS0,
-                   const DevState* st, hipStream_t s);
-void launch_ghost(const Grid& g, int mode, double* T, const double* prev, hipStream_t s);
-void launch_cfl(const Grid& g, const Phys& ph, const double* U, const double* M, DevState* st,
-                hipStream_t s);
+template <typename T>
+void launch_convert(const double* a, T* b, long long n, hipStream_t s);
+template <typename T>
+void launch_inflow(const Grid& g, T* U, const double* M, const double* table, const double* loc,
+                   const int* dire, int nloc, int nsamp, double phiS0, const DevState* st,
+                   unsigned char* LIM, hipStream_t s);
+template <typename T>
+void launch_ghost(const Grid& g, int mode, T* Tg, const T* prev, const DevState* st, int post,
+                  hipStream_t s);
+template <typename T>
+void launch_cfl(const Grid& g, const Phys& ph, const T* U, const T* M, DevState* st, hipStream_t s);
 void launch_dt(const DevParams& dp, DevState* st, const double* tf, const double* itime,
                hipStream_t s);
-void launch_stage(int io, const Grid& g, const Phys& ph, const double* Uin, const double* Uzero,
-                  double* Uout, double* ONE, const double* M, const DevState* st, hipStream_t s);
+template <typename T>
+void launch_stage(int io, const Grid& g, const Phys& ph, const T* Uin, const T* Uzero, T* Uout,
+                  T* ONE, const T* M, unsigned char* LIM, DevState* st, hipStream_t s);
 int mass_blocks();
+template <typename T>
+void launch_mass(const Grid& g, const T* U, const double* M, double* out, hipStream_t s);
 hipError_t probe_kernels();
-void launch_mass(const Grid& g, const double* U, const double* M, double* out, hipStream_t s);
 
 }  // namespace m2p

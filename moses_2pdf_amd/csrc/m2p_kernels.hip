@@ -1,14 +1,18 @@
 // m2p_kernels.hip -- HIP kernels of the adNOC_2Pb engine for gfx950.
 //
 // The reference launches ~85 kernels and synchronises after each of them per time step
-// (MoSES_2PDF_landslide/adNOC_2Pb_runKernels.cu:646-1330).  Here one step is five
+// (MoSES_2PDF_landslide/adNOC_2Pb_runKernels.cu:646-1330).  Here one step is a handful of
 // launches on one stream, with dt kept on the device:
 //   ghost(U0) -> cfl(U0) -> dt -> stage<0>(U0 -> U1) -> ghost(U1) -> stage<1>(U1,U0 -> U0)
 // (+ the inflow injection first in inflow mode).  A stage kernel fuses the reference's
 // TVD slopes, interface states, positivity, local speeds, central-upwind fluxes, pressure
 // and viscous terms, sources, update, wet/dry clipping and the velocity-crossing limiter
-// (L/K:1000-2937) for one interior cell per thread.
+// (L/K:1000-2937).  The per-step kernels are templated on the state's scalar type (double:
+// the plain path; float: the FP32-bulk variant); the metric setup is always FP64.  Tile
+// bodies are device functions so the batched-ensemble kernels reuse them.
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "m2p_kernels.h"
 #include "m2p_numerics.h"
@@ -151,13 +155,13 @@ __global__ void k_jacob(Grid g, double* __restrict__ M) {
 // ======================================================================================
 
 // Inflow1Kernel + Inflow2Kernel (I/K:818-887).  In the reference every thread of the
-// grid repeats the whole loop; one thread does it here, in the same order.
-__global__ void k_inflow(Grid g, double* __restrict__ U, const double* __restrict__ M,
-                         const double* __restrict__ table, const double* __restrict__ loc,
-                         const int* __restrict__ dire, int nloc, int nsamp, double phiS0,
-                         const DevState* __restrict__ st) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  const int ic = st->inflowCount;
+// grid repeats the whole loop; one thread does it here, in the same order (FP64
+// arithmetic, stored in the state's type).
+template <typename T>
+__device__ __forceinline__ void inflow_body(const Grid& g, T* __restrict__ U, const double* __restrict__ M,
+                                            const double* __restrict__ table, const double* __restrict__ loc,
+                                            const int* __restrict__ dire, int nloc, int nsamp,
+                                            double phiS0, int ic, unsigned char* __restrict__ LIM) {
   const long long P = g.P;
   for (int i = 0; i < nloc; i++) {
     const int direX = dire[i * 3 + 1], direY = dire[i * 3 + 2];
@@ -166,8 +170,8 @@ __global__ void k_inflow(Grid g, double* __restrict__ U, const double* __restric
     for (int mm = 0; mm < direX + MD + 3; mm++)
       for (int nn = 0; nn < direY + MD + 3; nn++) {
         const long long c = idx(g, yloc + nn, xloc + mm);
-        U[0 * P + c] = Hin * (1 - phiS0) / phiS0;
-        U[3 * P + c] = Hin * (1.0 - phiS0) * M[MC_C * P + c];
+        U[0 * P + c] = (T)(Hin * (1 - phiS0) / phiS0);
+        U[3 * P + c] = (T)(Hin * (1.0 - phiS0) * M[MC_C * P + c]);
       }
   }
   for (int i = 0; i < nloc; i++) {
@@ -178,13 +182,24 @@ __global__ void k_inflow(Grid g, double* __restrict__ U, const double* __restric
     for (int mm = 0; mm < direX + MD + 3; mm++)
       for (int nn = 0; nn < direY + MD + 3; nn++) {
         const long long c = idx(g, yloc + nn, xloc + mm);
-        U[1 * P + c] = Uin * (U[0 * P + c] + H0);
-        U[2 * P + c] = Vin * (U[0 * P + c] + H0);
-        U[4 * P + c] = Uin * (U[3 * P + c] + H0);
-        U[5 * P + c] = Vin * (U[3 * P + c] + H0);
-        U[6 * P + c] = U[0 * P + c] / (U[0 * P + c] + U[3 * P + c] + H0);
+        const double u0 = (double)U[0 * P + c], u3 = (double)U[3 * P + c];
+        U[1 * P + c] = (T)(Uin * (u0 + H0));
+        U[2 * P + c] = (T)(Vin * (u0 + H0));
+        U[4 * P + c] = (T)(Uin * (u3 + H0));
+        U[5 * P + c] = (T)(Vin * (u3 + H0));
+        U[6 * P + c] = (T)(u0 / (u0 + u3 + H0));
+        if (LIM) LIM[c] = 0;
       }
   }
+}
+
+template <typename T>
+__global__ void k_inflow(Grid g, T* __restrict__ U, const double* __restrict__ M,
+                         const double* __restrict__ table, const double* __restrict__ loc,
+                         const int* __restrict__ dire, int nloc, int nsamp, double phiS0,
+                         const DevState* __restrict__ st, unsigned char* __restrict__ LIM) {
+  if (blockIdx.x != 0 || threadIdx.x != 0 || halted_pre(st)) return;
+  inflow_body(g, U, M, table, loc, dire, nloc, nsamp, phiS0, st->inflowCount, LIM);
 }
 
 // Boundary5..8Kernel (L/K:853-956; inflow I/K:936-997) as one launch.  Thread (k, line):
@@ -196,32 +211,32 @@ __global__ void k_inflow(Grid g, double* __restrict__ U, const double* __restric
 //     columns [3, nx+2] only.
 //   inflow: x-ghosts on rows [3, ny+2]; y-ghosts on every column, the corner columns from
 //     the x-ghost values of rows 3,4 / ny+1,ny+2 (Boundary6 covers every column).
-__device__ __forceinline__ void extrap3(double a, double b, double& g2, double& g1, double& g0) {
-  g2 = 0.5 * (a + b);   // u[2] = (u[3]+u[4])/2
-  g1 = 0.5 * (g2 + a);  // u[1] = (u[2]+u[3])/2
-  g0 = 0.5 * (g1 + g2); // u[0] = (u[1]+u[2])/2
+template <typename T>
+__device__ __forceinline__ void extrap3(T a, T b, T& g2, T& g1, T& g0) {
+  g2 = T(0.5) * (a + b);   // u[2] = (u[3]+u[4])/2
+  g1 = T(0.5) * (g2 + a);  // u[1] = (u[2]+u[3])/2
+  g0 = T(0.5) * (g1 + g2); // u[0] = (u[1]+u[2])/2
 }
 
-__global__ void k_ghost(Grid g, int mode, double* __restrict__ T, const double* __restrict__ prev) {
-  const int line = blockIdx.x * blockDim.x + threadIdx.x;
-  const int k = blockIdx.y;
-  if (line >= g.nyd + g.nxd) return;
+template <typename T>
+__device__ __forceinline__ void ghost_line(const Grid& g, int mode, T* __restrict__ Tg,
+                                           const T* __restrict__ prev, int line, int k) {
   const long long P = g.P;
-  double* A = T + k * P;
+  T* A = Tg + k * P;
   const bool mom = (k == 1 || k == 2 || k == 4 || k == 5);
   const int nx = g.nx, ny = g.ny;
   if (line < g.nyd) {
     const int r = line;
     const bool ghost_row = (r < MD || r >= ny + MD);
     if (ghost_row && mode == 1) return;
-    const double* src = (ghost_row ? prev + k * P : A) + (long long)r * g.nxd;
-    double* row = A + (long long)r * g.nxd;
+    const T* src = (ghost_row ? prev + k * P : A) + (long long)r * g.nxd;
+    T* row = A + (long long)r * g.nxd;
     if (mom) {
-      row[0] = row[1] = row[2] = 0.0;
-      row[nx + MD] = row[nx + MD + 1] = row[nx + MD + 2] = 0.0;
+      row[0] = row[1] = row[2] = T(0);
+      row[nx + MD] = row[nx + MD + 1] = row[nx + MD + 2] = T(0);
       return;
     }
-    double g2, g1, g0, h3, h4, h5;
+    T g2, g1, g0, h3, h4, h5;
     extrap3(src[MD], src[MD + 1], g2, g1, g0);
     extrap3(src[nx + MD - 1], src[nx + MD - 2], h3, h4, h5);
     row[2] = g2;
@@ -235,13 +250,13 @@ __global__ void k_ghost(Grid g, int mode, double* __restrict__ T, const double* 
     const bool ghost_col = (c < MD || c >= nx + MD);
     if (ghost_col && mode == 0) return;
     const long long n = g.nxd;
-    double* col = A + c;
+    T* col = A + c;
     if (mom) {
-      col[0] = col[n] = col[2 * n] = 0.0;
-      col[(ny + MD) * n] = col[(ny + MD + 1) * n] = col[(ny + MD + 2) * n] = 0.0;
+      col[0] = col[n] = col[2 * n] = T(0);
+      col[(ny + MD) * n] = col[(ny + MD + 1) * n] = col[(ny + MD + 2) * n] = T(0);
       return;
     }
-    double a3, a4, b2, b1;
+    T a3, a4, b2, b1;
     if (!ghost_col) {
       a3 = col[MD * n];
       a4 = col[(MD + 1) * n];
@@ -249,11 +264,11 @@ __global__ void k_ghost(Grid g, int mode, double* __restrict__ T, const double* 
       b1 = col[(ny + MD - 2) * n];
     } else {
       // x-ghost value of column c on rows 3, 4, ny+2, ny+1 (what Boundary5 just wrote)
-      double v[4];
+      T v[4];
       const int rows[4] = {MD, MD + 1, ny + MD - 1, ny + MD - 2};
       for (int q = 0; q < 4; q++) {
-        const double* rr = A + (long long)rows[q] * n;
-        double e2, e1, e0;
+        const T* rr = A + (long long)rows[q] * n;
+        T e2, e1, e0;
         if (c < MD) {
           extrap3(rr[MD], rr[MD + 1], e2, e1, e0);
           v[q] = (c == 2) ? e2 : (c == 1) ? e1 : e0;
@@ -267,7 +282,7 @@ __global__ void k_ghost(Grid g, int mode, double* __restrict__ T, const double* 
       b2 = v[2];
       b1 = v[3];
     }
-    double g2, g1, g0, h3, h4, h5;
+    T g2, g1, g0, h3, h4, h5;
     extrap3(a3, a4, g2, g1, g0);
     extrap3(b2, b1, h3, h4, h5);
     col[2 * n] = g2;
@@ -277,6 +292,14 @@ __global__ void k_ghost(Grid g, int mode, double* __restrict__ T, const double* 
     col[(ny + MD + 1) * n] = h4;
     col[(ny + MD + 2) * n] = h5;
   }
+}
+
+template <typename T>
+__global__ void k_ghost(Grid g, int mode, T* __restrict__ Tg, const T* __restrict__ prev,
+                        const DevState* __restrict__ st, int post) {
+  const int line = blockIdx.x * blockDim.x + threadIdx.x;
+  if (line >= g.nyd + g.nxd || (post ? halted_post(st) : halted_pre(st))) return;
+  ghost_line(g, mode, Tg, prev, line, (int)blockIdx.y);
 }
 
 // Max of one wave (64 lanes) with fmax semantics (NaN ignored, as fmaxf in reduceKernel).
@@ -292,21 +315,22 @@ __device__ __forceinline__ double wave_max(double v) {
 // the 1e-10 floor, which never exceeds these.  Result: atomicMax on the IEEE bits of a
 // positive double.  k_cfl covers the faces of the interior box [3, ny+2] x [3, nx+2]
 // (InterfacesKernel states); k_cfl_ring the rest of [1, N-3]^2, where Interfaces2Kernel
-// overrides the states.
-__device__ __forceinline__ double face_cfl(const double* __restrict__ U, const double* __restrict__ M,
+// overrides the states.  Speeds are formed in T; the quotient by dx and the max in FP64.
+template <typename T>
+__device__ __forceinline__ double face_cfl(const T* __restrict__ U, const T* __restrict__ M,
                                            const Grid& g, const Phys& ph, int dir, int r, int c,
                                            bool edge) {
-  double L[6], R[6];
+  T L[6], R[6];
   if (edge)
     final_states_edge(U, g, dir, r, c, L, R);
   else
     base_states(U, g, dir, r, c, L, R);
-  FaceMet f;
+  FaceMet<T> f;
   face_metric(M, g, dir, r, c, f);
-  FaceSpeed s;
+  FaceSpeed<T> s;
   face_speed(dir, L, R, f, ph, s);
   const double d = dir == 0 ? g.dx : g.dy;
-  return fmax(s.emS / d, s.emF / d);
+  return fmax((double)s.emS / d, (double)s.emF / d);
 }
 
 __device__ __forceinline__ void block_max_to(double m, DevState* st) {
@@ -323,53 +347,87 @@ __device__ __forceinline__ void block_max_to(double m, DevState* st) {
   }
 }
 
-__global__ void __launch_bounds__(256) k_cfl(Grid g, Phys ph, const double* __restrict__ U,
-                                             const double* __restrict__ M, DevState* __restrict__ st) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x + MD;
-  const int r = blockIdx.y * blockDim.y + threadIdx.y + MD;
+// one 64x4 block of interior faces (both directions) starting at box column bx*64, row by*4
+template <typename T>
+__device__ __forceinline__ double cfl_box(const T* __restrict__ U, const T* __restrict__ M,
+                                          const Grid& g, const Phys& ph, int bx, int by) {
+  const int c = bx * 64 + threadIdx.x + MD;
+  const int r = by * 4 + threadIdx.y + MD;
   double m = 0.0;
   if (r <= g.ny + MD - 1 && c <= g.nx + MD - 1) {
     m = face_cfl(U, M, g, ph, 0, r, c, false);
     m = fmax(m, face_cfl(U, M, g, ph, 1, r, c, false));
   }
-  block_max_to(m, st);
+  return m;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_cfl(Grid g, Phys ph, const T* __restrict__ U,
+                                             const T* __restrict__ M, DevState* __restrict__ st) {
+  if (halted_pre(st)) return;
+  block_max_to(cfl_box(U, M, g, ph, blockIdx.x, blockIdx.y), st);
 }
 
 // The ring [1, N-3]^2 minus the interior box, one thread per (face index, direction).
 // Face indices are enumerated as 4 strips: rows 1-2 and ny+3..ny+4 (columns 1..N-3), then
 // columns 1-2 and nx+3..nx+4 on rows 3..ny+2.
-__global__ void __launch_bounds__(64) k_cfl_ring(Grid g, Phys ph, const double* __restrict__ U,
-                                                 const double* __restrict__ M,
-                                                 DevState* __restrict__ st) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__host__ __device__ __forceinline__ int ring_threads(const Grid& g) {
+  return 2 * (4 * (g.nxd - 3) + 4 * g.ny);
+}
+
+template <typename T>
+__device__ __forceinline__ double cfl_ring_face(const T* __restrict__ U, const T* __restrict__ M,
+                                                const Grid& g, const Phys& ph, int t) {
   const int f = t >> 1, dir = t & 1;
   const int w = g.nxd - 3;        // columns 1..N-3
   const int nrow = 4 * w;         // 4 ghost face rows
   const int ncol = 4 * g.ny;      // 4 ghost face columns on interior rows
-  double m = 0.0;
-  if (f < nrow + ncol) {
-    int r, c;
-    if (f < nrow) {
-      const int k = f / w;
-      r = (k < 2) ? 1 + k : g.ny + MD + (k - 2);
-      c = 1 + f % w;
-    } else {
-      const int q = f - nrow, k = q / g.ny;
-      r = MD + q % g.ny;
-      c = (k < 2) ? 1 + k : g.nx + MD + (k - 2);
-    }
-    m = face_cfl(U, M, g, ph, dir, r, c, true);
+  if (f >= nrow + ncol) return 0.0;
+  int r, c;
+  if (f < nrow) {
+    const int k = f / w;
+    r = (k < 2) ? 1 + k : g.ny + MD + (k - 2);
+    c = 1 + f % w;
+  } else {
+    const int q = f - nrow, k = q / g.ny;
+    r = MD + q % g.ny;
+    c = (k < 2) ? 1 + k : g.nx + MD + (k - 2);
   }
+  return face_cfl(U, M, g, ph, dir, r, c, true);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) k_cfl_ring(Grid g, Phys ph, const T* __restrict__ U,
+                                                 const T* __restrict__ M, DevState* __restrict__ st) {
+  if (halted_pre(st)) return;
+  double m = cfl_ring_face(U, M, g, ph, (int)(blockIdx.x * blockDim.x + threadIdx.x));
   m = wave_max(m);
   if (threadIdx.x == 0) atomicMax(&st->maxbits, (unsigned long long)__double_as_longlong(m));
 }
 
 // CFL3Kernel + the host's dt control + CFL4Kernel (L/K:2006-2036, L/R:978-991; inflow
 // sample advance I/R:1078-1081), on the device.  The max is rounded to float as the
-// reference's fmaxf reduction does (L/K:1988, 1996).
-__global__ void k_dt(DevParams dp, DevState* __restrict__ st, const double* __restrict__ tf,
-                     const double* __restrict__ itime) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+// reference's fmaxf reduction does (L/K:1988, 1996).  Also the halting rules, which have
+// no reference counterpart: a step budget (halt_at), an end time (t_end: the step that
+// reaches it is shortened to end on it) and a stop speed (vstop2: once the squared max
+// flow speed of a corrector has reached it and later falls below it, the next step is the
+// last).
+__device__ __forceinline__ void dt_body(const DevParams& dp, DevState* __restrict__ st,
+                                        const double* __restrict__ tf,
+                                        const double* __restrict__ itime) {
+  if (st->skip) return;
+  if (st->nt >= st->halt_at) {
+    st->skip = 1;
+    return;
+  }
+  if (st->vstop2 > 0.0 && st->nt > 0) {
+    const double v2 = __longlong_as_double((long long)st->vmaxbits);
+    if (v2 >= st->vstop2)
+      st->moved = 1;
+    else if (st->moved && st->halt_at > st->nt + 1)
+      st->halt_at = st->nt + 1;
+  }
+  st->vmaxbits = 0ull;
   const double mx = __longlong_as_double((long long)st->maxbits);
   const double dmax = (double)(float)mx;
   double dtval = dp.cfl / dmax;
@@ -385,6 +443,10 @@ __global__ void k_dt(DevParams dp, DevState* __restrict__ st, const double* __re
   }
   if (dp.nsamp > 0 && T >= itime[st->inflowCount] && st->inflowCount < (dp.nsamp - 1))
     st->inflowCount = st->inflowCount + 1;
+  if (st->t_end > 0.0 && (T + dtval) >= st->t_end) {
+    dtval = st->t_end - T;
+    if (st->halt_at > st->nt + 1) st->halt_at = st->nt + 1;
+  }
   st->dtval = dtval;
   st->dt_pred = dtval / (2.0 - 0);
   st->dt_corr = dtval / (2.0 - 1);
@@ -393,12 +455,19 @@ __global__ void k_dt(DevParams dp, DevState* __restrict__ st, const double* __re
   st->nt = st->nt + 1;
 }
 
+__global__ void k_dt(DevParams dp, DevState* __restrict__ st, const double* __restrict__ tf,
+                     const double* __restrict__ itime) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  dt_body(dp, st, tf, itime);
+}
+
 // Slope-carrying cell values for the viscous term: u3, u4, u5 and the limited slopes of
 // u4, u5 along direction `tdir` (0: x, 1: y) at cell i.
-__device__ __forceinline__ PDSide pd_side(const double* __restrict__ U, const Grid& g, long long i,
-                                          int tdir) {
+template <typename T>
+__device__ __forceinline__ PDSide<T> pd_side(const T* __restrict__ U, const Grid& g, long long i,
+                                             int tdir) {
   const long long P = g.P, st = tdir == 0 ? 1 : g.nxd;
-  PDSide p;
+  PDSide<T> p;
   p.u3 = U[3 * P + i];
   p.u4 = U[4 * P + i];
   p.u5 = U[5 * P + i];
@@ -406,7 +475,6 @@ __device__ __forceinline__ PDSide pd_side(const double* __restrict__ U, const Gr
   p.t5 = tvd_slope(U[5 * P + i - st], p.u5, U[5 * P + i + st]);
   return p;
 }
-
 
 // One stage (predictor IO = 0, corrector IO = 1): Flux8..21Kernel (L/K:2038-2726) and the
 // crossing limiter, Flux22..24 / Flux25..28 (L/K:2730-2937).
@@ -419,105 +487,84 @@ __device__ __forceinline__ PDSide pd_side(const double* __restrict__ U, const Gr
 // y-faces of a face row by one wave; a cell reads its two faces of each direction from
 // LDS.  The update keeps the reference's left-to-right sum, so the x terms are formed
 // first and held in registers while the y faces are computed.
-#ifndef M2P_STAGE_ATTR
-#define M2P_STAGE_ATTR __attribute__((amdgpu_waves_per_eu(4, 4)))
-#endif
 #ifndef M2P_ST_TY
 #define M2P_ST_TY 7
 #endif
 constexpr int ST_TX = 63, ST_TY = M2P_ST_TY, ST_NQ = 12;
-typedef double (*Tile)[ST_TY + 1][64];
 
 // Phase A: InterfacesKernel states of one face into the tile (L: q 0-5, R: q 6-11).
-__device__ __forceinline__ void stage_states(Tile T, int w, int lane, const double* __restrict__ U,
-                                             const Grid& g, int dir, int r, int c) {
-  double L[6], R[6];
+template <typename T>
+__device__ __forceinline__ void stage_states(T (*Tl)[ST_TY + 1][64], int w, int lane,
+                                             const T* __restrict__ U, const Grid& g, int dir, int r,
+                                             int c) {
+  T L[6], R[6];
   base_states(U, g, dir, r, c, L, R);
 #pragma unroll
   for (int k = 0; k < 6; k++) {
-    T[k][w][lane] = L[k];
-    T[6 + k][w][lane] = R[k];
+    Tl[k][w][lane] = L[k];
+    Tl[6 + k][w][lane] = R[k];
   }
 }
 
 // Phase B: the face's final states.  The only Interfaces2Kernel overrides a stage needs are
 // the x-face at column 2 (from the faces at columns 3, 4: lanes +1, +2) and the y-face at
 // row 2 (from rows 3, 4: tile rows +1, +2).
-__device__ __forceinline__ void load_states(Tile T, int w, int lane, bool ovr, int dir, double L[6],
-                                            double R[6]) {
+template <typename T>
+__device__ __forceinline__ void load_states(T (*Tl)[ST_TY + 1][64], int w, int lane, bool ovr,
+                                            int dir, T L[6], T R[6]) {
 #pragma unroll
   for (int k = 0; k < 6; k++) {
     if (!ovr) {
-      L[k] = T[k][w][lane];
-      R[k] = T[6 + k][w][lane];
+      L[k] = Tl[k][w][lane];
+      R[k] = Tl[6 + k][w][lane];
     } else if (dir == 0) {
-      L[k] = 0.5 * (T[k][w][lane + 1] + T[k][w][lane + 2]);
-      R[k] = 0.5 * (T[6 + k][w][lane + 1] + T[6 + k][w][lane + 2]);
+      L[k] = T(0.5) * (Tl[k][w][lane + 1] + Tl[k][w][lane + 2]);
+      R[k] = T(0.5) * (Tl[6 + k][w][lane + 1] + Tl[6 + k][w][lane + 2]);
     } else {
-      L[k] = 0.5 * (T[k][w + 1][lane] + T[k][w + 2][lane]);
-      R[k] = 0.5 * (T[6 + k][w + 1][lane] + T[6 + k][w + 2][lane]);
+      L[k] = T(0.5) * (Tl[k][w + 1][lane] + Tl[k][w + 2][lane]);
+      R[k] = T(0.5) * (Tl[6 + k][w + 1][lane] + Tl[6 + k][w + 2][lane]);
     }
   }
 }
 
 // Phase C: fluxes of one face from its final states, into the tile.
-__device__ __forceinline__ void flux_store(Tile T, int w, int lane, const double* __restrict__ U,
-                                           const double* __restrict__ M, const Grid& g,
-                                           const Phys& ph, int dir, int r, int c, double L[6],
-                                           double R[6]) {
-  FaceMet f;
+template <typename T>
+__device__ __forceinline__ void flux_store(T (*Tl)[ST_TY + 1][64], int w, int lane,
+                                           const T* __restrict__ U, const T* __restrict__ M,
+                                           const Grid& g, const Phys& ph, int dir, int r, int c,
+                                           T L[6], T R[6]) {
+  FaceMet<T> f;
   face_metric(M, g, dir, r, c, f);
-  FaceSpeed sp;
+  FaceSpeed<T> sp;
   face_speed(dir, L, R, f, ph, sp);
   const long long i = idx(g, r, c);
   const long long j = i + (dir == 0 ? 1 : g.nxd);
-  const PDSide pl = pd_side(U, g, i, 1 - dir);
-  const PDSide pr = pd_side(U, g, j, 1 - dir);
-  FaceFlux o;
+  const PDSide<T> pl = pd_side(U, g, i, 1 - dir);
+  const PDSide<T> pr = pd_side(U, g, j, 1 - dir);
+  FaceFlux<T> o;
   face_flux(dir, L, R, f, sp, pl, pr, g, ph, o);
 #pragma unroll
-  for (int k = 0; k < 6; k++) T[k][w][lane] = o.H[k];
-  T[6][w][lane] = o.P1;
-  T[7][w][lane] = o.P2;
-  T[8][w][lane] = o.P4;
-  T[9][w][lane] = o.P5;
-  T[10][w][lane] = o.D4;
-  T[11][w][lane] = o.D5;
+  for (int k = 0; k < 6; k++) Tl[k][w][lane] = o.H[k];
+  Tl[6][w][lane] = o.P1;
+  Tl[7][w][lane] = o.P2;
+  Tl[8][w][lane] = o.P4;
+  Tl[9][w][lane] = o.P5;
+  Tl[10][w][lane] = o.D4;
+  Tl[11][w][lane] = o.D5;
 }
 
-// Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).  Block b runs on
-// XCD b % 8; it is given tile number (b % 8) * per + b / 8 in row-major tile order, so
-// each XCD sweeps a contiguous band of tile rows and vertically adjacent tiles, which
-// share halo rows, meet in the same L2.
-#ifndef M2P_XCD_REMAP
-#define M2P_XCD_REMAP 1
-#endif
-constexpr int N_XCD = 8;
-__device__ __forceinline__ bool tile_of_block(const Grid& g, int& tx, int& ty) {
-  const int ntx = (g.nx + ST_TX - 1) / ST_TX;
-  const int nty = (g.ny + ST_TY - 1) / ST_TY;
-  const int b = blockIdx.x;
-  int t = b;
-  if (M2P_XCD_REMAP) {
-    const int per = (ntx * nty + N_XCD - 1) / N_XCD;
-    t = (b % N_XCD) * per + b / N_XCD;
-  }
-  if (t >= ntx * nty) return false;
-  tx = t % ntx;
-  ty = t / ntx;
-  return true;
-}
+// Returns, for IO = 1, the larger squared flow speed (solid, fluid) of the cell's new state
+// if its flow depth exceeds WET_SPEED (0 otherwise, and for IO = 0).
+constexpr double WET_SPEED = 1.0e-2;  // scaled depth (1 mm)
 
-template <int IO>
-__global__ void __launch_bounds__(64 * (ST_TY + 1)) M2P_STAGE_ATTR k_stage(Grid g, Phys ph, const double* __restrict__ Uin,
-                                               const double* Uzero, double* Uout,
-                                               double* __restrict__ ONE, const double* __restrict__ M,
-                                               const DevState* __restrict__ st) {
-  __shared__ double F[ST_NQ][ST_TY + 1][64];
+template <int IO, typename T>
+__device__ __forceinline__ double stage_tile(T (*F)[ST_TY + 1][64], const Grid& g, const Phys& ph,
+                                           const T* __restrict__ Uin, const T* Uzero, T* Uout,
+                                           T* __restrict__ ONE, const T* __restrict__ M,
+                                           unsigned char* __restrict__ LIM, double dt_d, int tx,
+                                           int ty) {
   const int lane = threadIdx.x;
   const int w = threadIdx.y;
-  int tx, ty;
-  if (!tile_of_block(g, tx, ty)) return;
   const int c0 = tx * ST_TX + MD;
   const int r0 = ty * ST_TY + MD;
   const int xmax = g.nx + MD - 1, ymax = g.ny + MD - 1;
@@ -529,7 +576,7 @@ __global__ void __launch_bounds__(64 * (ST_TY + 1)) M2P_STAGE_ATTR k_stage(Grid 
     const bool has = (w < ST_TY && r <= ymax && c <= xmax);
     if (has) stage_states(F, w, lane, Uin, g, 0, r, c);
     __syncthreads();
-    double L[6], R[6];
+    T L[6], R[6];
     if (has) load_states(F, w, lane, c == 2, 0, L, R);
     __syncthreads();
     if (has) flux_store(F, w, lane, Uin, M, g, ph, 0, r, c, L, R);
@@ -539,15 +586,15 @@ __global__ void __launch_bounds__(64 * (ST_TY + 1)) M2P_STAGE_ATTR k_stage(Grid 
   const int r = r0 + w, c = c0 + lane;
   const bool active = (w < ST_TY) && (lane < ST_TX) && r <= ymax && c <= xmax;
   const long long i = idx(g, r, c);
-  const double dt = IO == 0 ? st->dt_pred : st->dt_corr;
-  const double dtdx = dt / g.dx, dtdy = dt / g.dy;
-  const double epsilon0 = 1.0;
-  double phi = 0.0, fP = 0.0, fD = 0.0;
-  double tH[6], tP[4], tD[2];
+  const T dt = T(dt_d);
+  const T dtdx = qdiv(dt, T(g.dx)), dtdy = qdiv(dt, T(g.dy));
+  const T epsilon0 = T(1.0);
+  T phi = T(0), fP = T(0), fD = T(0);
+  T tH[6], tP[4], tD[2];
   if (active) {
     phi = Uin[6 * P + i];
     fP = epsilon0 * phi;
-    fD = epsilon0 * (1.0 - phi) / ph.N_R;
+    fD = qdiv(epsilon0 * (T(1.0) - phi), T(ph.N_R));
 #pragma unroll
     for (int k = 0; k < 6; k++) tH[k] = (dtdx) * (F[k][w][lane + 1] - F[k][w][lane]);
 #pragma unroll
@@ -563,84 +610,161 @@ __global__ void __launch_bounds__(64 * (ST_TY + 1)) M2P_STAGE_ATTR k_stage(Grid 
     const bool has = (lane < ST_TX && rf <= ymax && cf <= xmax);
     if (has) stage_states(F, w, lane, Uin, g, 1, rf, cf);
     __syncthreads();
-    double L[6], R[6];
+    T L[6], R[6];
     if (has) load_states(F, w, lane, rf == 2, 1, L, R);
     __syncthreads();
     if (has) flux_store(F, w, lane, Uin, M, g, ph, 1, rf, cf, L, R);
   }
   __syncthreads();
-  if (!active) return;
+  if (!active) return 0.0;
 
-  double u[7];
+  T u[7];
 #pragma unroll
   for (int k = 0; k < 7; k++) u[k] = Uin[k * P + i];
-  CellMet cm;
+  CellMet<T> cm;
   cell_metric(M, P, i, cm);
-  double s[6];
+  T s[6];
   cell_source(u, cm, ph, s);
-  double uz[6];
+  T uz[6];
 #pragma unroll
   for (int k = 0; k < 6; k++) uz[k] = Uzero[k * P + i];
 
   // Flux18Kernel L/K:2641-2652 (Pp and PD are stored zeros for the components they skip)
-  double vnew[6];
+  T vnew[6];
 #pragma unroll
   for (int k = 0; k < 6; k++) {
     const int q = k == 1 ? 0 : k == 2 ? 1 : k == 4 ? 2 : 3;  // P1, P2, P4, P5
     const bool hasP = (k == 1 || k == 2 || k == 4 || k == 5);
     const bool hasD = (k == 4 || k == 5);
-    const double yH = (dtdy) * (F[k][w + 1][lane] - F[k][w][lane]);
-    const double xP = hasP ? tP[q] : (dtdx) * (0.0 - 0.0) * (fP);
-    const double yP = hasP ? (dtdy) * (F[6 + q][w + 1][lane] - F[6 + q][w][lane]) * (fP)
-                           : (dtdy) * (0.0 - 0.0) * (fP);
-    const double xD = hasD ? tD[k - 4] : (dtdx) * (0.0 - 0.0) * (fD);
-    const double yD = hasD ? (dtdy) * (F[10 + (k - 4)][w + 1][lane] - F[10 + (k - 4)][w][lane]) * (fD)
-                           : (dtdy) * (0.0 - 0.0) * (fD);
+    const T zero = T(0.0);
+    const T yH = (dtdy) * (F[k][w + 1][lane] - F[k][w][lane]);
+    const T xP = hasP ? tP[q] : (dtdx) * (zero - zero) * (fP);
+    const T yP = hasP ? (dtdy) * (F[6 + q][w + 1][lane] - F[6 + q][w][lane]) * (fP)
+                      : (dtdy) * (zero - zero) * (fP);
+    const T xD = hasD ? tD[k - 4] : (dtdx) * (zero - zero) * (fD);
+    const T yD = hasD ? (dtdy) * (F[10 + (k - 4)][w + 1][lane] - F[10 + (k - 4)][w][lane]) * (fD)
+                      : (dtdy) * (zero - zero) * (fD);
     vnew[k] = uz[k] * cm.det - tH[k] - yH + xP + yP + xD + yD + (dt)*s[k];
   }
-  double un[7];
+  const T h0 = T(H0);
+  T un[7];
 #pragma unroll
-  for (int k = 0; k < 6; k++) un[k] = vnew[k] / cm.det;  // Flux19
-  if (un[0] <= H00) {                                     // Flux20
-    un[0] = 0.0;
-    un[1] = 0.0;
-    un[2] = 0.0;
+  for (int k = 0; k < 6; k++) un[k] = qdiv(vnew[k], cm.det);  // Flux19
+  if (un[0] <= T(H00)) {                                     // Flux20
+    un[0] = T(0);
+    un[1] = T(0);
+    un[2] = T(0);
   }
-  if (un[3] <= H00) {
-    un[3] = 0.0;
-    un[4] = 0.0;
-    un[5] = 0.0;
+  if (un[3] <= T(H00)) {
+    un[3] = T(0);
+    un[4] = T(0);
+    un[5] = T(0);
   }
-  un[6] = un[0] / (un[0] + un[3] + H0);  // Flux21
+  un[6] = qdiv(un[0], un[0] + un[3] + h0);  // Flux21
 
   // velocity-crossing limiter (Flux23/24, Flux26/27)
-  double o0, o1, o2, o3, o4, o5;
+  T o0, o1, o2, o3, o4, o5;
   if (IO == 0) {
     o0 = uz[0]; o1 = uz[1]; o2 = uz[2]; o3 = uz[3]; o4 = uz[4]; o5 = uz[5];
   } else {
     o0 = u[0]; o1 = u[1]; o2 = u[2]; o3 = u[3]; o4 = ONE[i]; o5 = ONE[P + i];
   }
-  const double usxnew = un[1] / (un[0] + H0);
-  const double ufxnew = un[4] / (un[3] + H0);
-  const double usxold = o1 / (o0 + H0);
-  const double ufxold = o4 / (o3 + H0);
-  const double usynew = un[2] / (un[0] + H0);
-  const double ufynew = un[5] / (un[3] + H0);
-  const double usyold = o2 / (o0 + H0);
-  const double ufyold = o5 / (o3 + H0);
-  const double lim4 = ((usxnew - ufxnew) * (usxold - ufxold) >= 0.0) ? un[4] : usxnew * un[3];
-  const double lim5 = ((usynew - ufynew) * (usyold - ufyold) >= 0.0) ? un[5] : usynew * un[3];
+  const T usxnew = qdiv(un[1], un[0] + h0);
+  const T ufxnew = qdiv(un[4], un[3] + h0);
+  const T usxold = qdiv(o1, o0 + h0);
+  const T ufxold = qdiv(o4, o3 + h0);
+  const T usynew = qdiv(un[2], un[0] + h0);
+  const T ufynew = qdiv(un[5], un[3] + h0);
+  const T usyold = qdiv(o2, o0 + h0);
+  const T ufyold = qdiv(o5, o3 + h0);
+  T lim4, lim5;
+  if constexpr (std::is_same<T, double>::value) {
+    lim4 = ((usxnew - ufxnew) * (usxold - ufxold) >= T(0.0)) ? un[4] : usxnew * un[3];
+    lim5 = ((usynew - ufynew) * (usyold - ufyold) >= T(0.0)) ? un[5] : usynew * un[3];
+  } else {
+    // Limiter memory (FP32 only).  When a stage limits, the stored fluid momentum is
+    // us*hf, and in FP64 the next stage's old relative velocity is us*H0/(hf+H0): about
+    // 1e-8 of us, with the sign of us.  Float storage rounds that to noise, so the flag
+    // of the stored state says to take the sign of us instead.  Bits 0-1: flags of the
+    // stage-1 output (U0, read by stage 0); bits 2-3: flags of ONE (read by stage 1).
+    const unsigned f = IO == 0 ? (LIM[i] & 3u) : ((LIM[i] >> 2) & 3u);
+    const T dxo = (f & 1u) ? usxold : (usxold - ufxold);
+    const T dyo = (f & 2u) ? usyold : (usyold - ufyold);
+    const bool kx = (usxnew - ufxnew) * dxo >= T(0.0);
+    const bool ky = (usynew - ufynew) * dyo >= T(0.0);
+    lim4 = kx ? un[4] : usxnew * un[3];
+    lim5 = ky ? un[5] : usynew * un[3];
+    const unsigned nb = (kx ? 0u : 1u) | (ky ? 0u : 2u);
+    LIM[i] = (unsigned char)(IO == 0 ? ((f & 3u) | (nb << 2)) : nb);
+  }
   if (IO == 0) {
 #pragma unroll
     for (int k = 0; k < 7; k++) Uout[k * P + i] = un[k];
     ONE[i] = lim4;
     ONE[P + i] = lim5;
+    return 0.0;
   } else {
     un[4] = lim4;
     un[5] = lim5;
 #pragma unroll
     for (int k = 0; k < 7; k++) Uout[k * P + i] = un[k];
+    if ((double)un[0] + (double)un[3] <= WET_SPEED) return 0.0;
+    const double ufx = (double)qdiv(lim4, un[3] + h0), ufy = (double)qdiv(lim5, un[3] + h0);
+    const double sx = (double)usxnew, sy = (double)usynew;
+    return fmax(sx * sx + sy * sy, ufx * ufx + ufy * ufy);
   }
+}
+
+// Max over the block of a per-thread value into a DevState field (atomicMax on the IEEE
+// bits of a non-negative double).  Every thread of the block must call it.
+__device__ __forceinline__ void block_max_bits(double m, unsigned long long* dst) {
+  m = wave_max(m);
+  __shared__ double part[16];
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  const int nw = (blockDim.x * blockDim.y + 63) / 64;
+  if ((tid & 63) == 0) part[tid >> 6] = m;
+  __syncthreads();
+  if (tid == 0) {
+    double b = part[0];
+    for (int w = 1; w < nw; w++) b = fmax(b, part[w]);
+    if (b > 0.0) atomicMax(dst, (unsigned long long)__double_as_longlong(b));
+  }
+}
+
+// Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).  Block b runs on
+// XCD b % 8; it is given tile number (b % 8) * per + b / 8 in row-major tile order, so
+// each XCD sweeps a contiguous band of tile rows and vertically adjacent tiles, which
+// share halo rows, meet in the same L2.
+constexpr int N_XCD = 8;
+__host__ __device__ __forceinline__ int stage_tiles(const Grid& g) {
+  return ((g.nx + ST_TX - 1) / ST_TX) * ((g.ny + ST_TY - 1) / ST_TY);
+}
+__device__ __forceinline__ bool tile_of(const Grid& g, int t, int& tx, int& ty) {
+  const int ntx = (g.nx + ST_TX - 1) / ST_TX;
+  if (t >= stage_tiles(g)) return false;
+  tx = t % ntx;
+  ty = t / ntx;
+  return true;
+}
+
+#ifndef M2P_STAGE_ATTR
+#define M2P_STAGE_ATTR __attribute__((amdgpu_waves_per_eu(4, 4)))
+#endif
+
+template <int IO, typename T>
+__global__ void __launch_bounds__(64 * (ST_TY + 1)) M2P_STAGE_ATTR
+    k_stage(Grid g, Phys ph, const T* __restrict__ Uin, const T* Uzero, T* Uout,
+            T* __restrict__ ONE, const T* __restrict__ M, unsigned char* __restrict__ LIM,
+            DevState* __restrict__ st) {
+  __shared__ T F[ST_NQ][ST_TY + 1][64];
+  if (halted_post(st)) return;
+  const int b = blockIdx.x;
+  const int per = (stage_tiles(g) + N_XCD - 1) / N_XCD;
+  int tx, ty;
+  if (!tile_of(g, (b % N_XCD) * per + b / N_XCD, tx, ty)) return;
+  const double v2 = stage_tile<IO, T>(F, g, ph, Uin, Uzero, Uout, ONE, M, LIM,
+                                      IO == 0 ? st->dt_pred : st->dt_corr, tx, ty);
+  if (IO == 1 && st->vstop2 > 0.0) block_max_bits(v2, &st->vmaxbits);
 }
 
 // Sum of u0*Det and u3*Det over interior cells with double-double accumulation; one
@@ -657,7 +781,8 @@ __device__ __forceinline__ void dd_add(double& hi, double& lo, double x) {
   two_sum(s, e, hi, lo);
 }
 
-__global__ void k_mass(Grid g, const double* __restrict__ U, const double* __restrict__ M,
+template <typename T>
+__global__ void k_mass(Grid g, const T* __restrict__ U, const double* __restrict__ M,
                        double* __restrict__ out) {
   const int tid = threadIdx.x;
   double sh = 0, sl = 0, fh = 0, fl = 0;
@@ -667,8 +792,8 @@ __global__ void k_mass(Grid g, const double* __restrict__ U, const double* __res
     const int r = (int)(q / g.nx) + MD, c = (int)(q % g.nx) + MD;
     const long long i = idx(g, r, c);
     const double d = M[MC_DET * g.P + i];
-    dd_add(sh, sl, U[i] * d);
-    dd_add(fh, fl, U[3 * g.P + i] * d);
+    dd_add(sh, sl, (double)U[i] * d);
+    dd_add(fh, fl, (double)U[3 * g.P + i] * d);
   }
   __shared__ double buf[4][256];
   buf[0][tid] = sh;
@@ -689,6 +814,14 @@ __global__ void k_mass(Grid g, const double* __restrict__ U, const double* __res
     out[blockIdx.x * 4 + 2] = b;
     out[blockIdx.x * 4 + 3] = bl;
   }
+}
+
+// FP64 -> T copy of n values (the FP32 variant's metric and initial state).
+template <typename T>
+__global__ void k_convert(const double* __restrict__ a, T* __restrict__ b, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    b[i] = (T)a[i];
 }
 
 // ======================================================================================
@@ -712,23 +845,31 @@ void launch_metric_setup(const Grid& g, int mode, const double* topo, const doub
   k_ghost_copy_y<<<dim3((g.nxd + 63) / 64, MC_COUNT - MC_J13X), 64, 0, s>>>(g, M, MC_J13X, MC_COUNT);
 }
 
-void launch_inflow(const Grid& g, double* U, const double* M, const double* table,
-                   const double* loc, const int* dire, int nloc, int nsamp, double phiS0,
-                   const DevState* st, hipStream_t s) {
-  k_inflow<<<1, 64, 0, s>>>(g, U, M, table, loc, dire, nloc, nsamp, phiS0, st);
+template <typename T>
+void launch_convert(const double* a, T* b, long long n, hipStream_t s) {
+  k_convert<T><<<1024, 256, 0, s>>>(a, b, n);
+}
+template void launch_convert<float>(const double*, float*, long long, hipStream_t);
+
+template <typename T>
+void launch_inflow(const Grid& g, T* U, const double* M, const double* table, const double* loc,
+                   const int* dire, int nloc, int nsamp, double phiS0, const DevState* st,
+                   unsigned char* LIM, hipStream_t s) {
+  k_inflow<T><<<1, 64, 0, s>>>(g, U, M, table, loc, dire, nloc, nsamp, phiS0, st, LIM);
 }
 
-void launch_ghost(const Grid& g, int mode, double* T, const double* prev, hipStream_t s) {
+template <typename T>
+void launch_ghost(const Grid& g, int mode, T* Tg, const T* prev, const DevState* st, int post,
+                  hipStream_t s) {
   const int lines = g.nyd + g.nxd;
-  k_ghost<<<dim3((lines + 127) / 128, MN), 128, 0, s>>>(g, mode, T, prev);
+  k_ghost<T><<<dim3((lines + 127) / 128, MN), 128, 0, s>>>(g, mode, Tg, prev, st, post);
 }
 
-void launch_cfl(const Grid& g, const Phys& ph, const double* U, const double* M, DevState* st,
-                hipStream_t s) {
+template <typename T>
+void launch_cfl(const Grid& g, const Phys& ph, const T* U, const T* M, DevState* st, hipStream_t s) {
   const dim3 b(64, 4);
-  k_cfl<<<grid2(g.nx, g.ny, b), b, 0, s>>>(g, ph, U, M, st);
-  const int ring = 2 * (4 * (g.nxd - 3) + 4 * g.ny);
-  k_cfl_ring<<<(ring + 63) / 64, 64, 0, s>>>(g, ph, U, M, st);
+  k_cfl<T><<<grid2(g.nx, g.ny, b), b, 0, s>>>(g, ph, U, M, st);
+  k_cfl_ring<T><<<(ring_threads(g) + 63) / 64, 64, 0, s>>>(g, ph, U, M, st);
 }
 
 void launch_dt(const DevParams& dp, DevState* st, const double* tf, const double* itime,
@@ -736,30 +877,43 @@ void launch_dt(const DevParams& dp, DevState* st, const double* tf, const double
   k_dt<<<1, 64, 0, s>>>(dp, st, tf, itime);
 }
 
-void launch_stage(int io, const Grid& g, const Phys& ph, const double* Uin, const double* Uzero,
-                  double* Uout, double* ONE, const double* M, const DevState* st, hipStream_t s) {
+template <typename T>
+void launch_stage(int io, const Grid& g, const Phys& ph, const T* Uin, const T* Uzero, T* Uout,
+                  T* ONE, const T* M, unsigned char* LIM, DevState* st, hipStream_t s) {
   const dim3 b(64, ST_TY + 1);
-  const int ntile = ((g.nx + ST_TX - 1) / ST_TX) * ((g.ny + ST_TY - 1) / ST_TY);
-  const dim3 gr(M2P_XCD_REMAP ? ((ntile + N_XCD - 1) / N_XCD) * N_XCD : ntile);
+  const dim3 gr(((stage_tiles(g) + N_XCD - 1) / N_XCD) * N_XCD);
   if (io == 0)
-    k_stage<0><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, st);
+    k_stage<0, T><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st);
   else
-    k_stage<1><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, st);
+    k_stage<1, T><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st);
 }
 
 int mass_blocks() { return 512; }
 
-void launch_mass(const Grid& g, const double* U, const double* M, double* out, hipStream_t s) {
-  k_mass<<<mass_blocks(), 256, 0, s>>>(g, U, M, out);
+template <typename T>
+void launch_mass(const Grid& g, const T* U, const double* M, double* out, hipStream_t s) {
+  k_mass<T><<<mass_blocks(), 256, 0, s>>>(g, U, M, out);
 }
 
-}  // namespace m2p
+#define M2P_INSTANTIATE(T)                                                                    \
+  template void launch_inflow<T>(const Grid&, T*, const double*, const double*, const double*, \
+                                 const int*, int, int, double, const DevState*,                \
+                                 unsigned char*, hipStream_t);                                 \
+  template void launch_ghost<T>(const Grid&, int, T*, const T*, const DevState*, int,         \
+                                hipStream_t);                                                 \
+  template void launch_cfl<T>(const Grid&, const Phys&, const T*, const T*, DevState*,         \
+                              hipStream_t);                                                    \
+  template void launch_stage<T>(int, const Grid&, const Phys&, const T*, const T*, T*, T*,     \
+                                const T*, unsigned char*, DevState*, hipStream_t);             \
+  template void launch_mass<T>(const Grid&, const T*, const double*, double*, hipStream_t);
+M2P_INSTANTIATE(double)
+M2P_INSTANTIATE(float)
 
-namespace m2p {
 // Confirms the library holds a code object for the current device (a missing gfx950 image
 // otherwise surfaces only at the first launch).
 hipError_t probe_kernels() {
   hipFuncAttributes a;
-  return hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_stage<0>));
+  return hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_stage<0, double>));
 }
+
 }  // namespace m2p

@@ -1,9 +1,14 @@
 // m2p_numerics.h -- per-face and per-cell numerics of the adNOC_2Pb scheme as device
-// functions, shared by the CFL pass and the stage kernels.
+// functions, shared by the CFL pass and the stage kernels, templated on the scalar type.
 //
-// Every expression keeps the operand order of the reference kernel it restates, and the
-// library is compiled with -ffp-contract=off, so results are bit-identical to a serial
-// evaluation of MoSES_2PDF_landslide/adNOC_2Pb_kernels.cu (cited as L/K) in IEEE FP64.
+// T = double is the plain path: every expression keeps the operand order of the
+// reference kernel it restates and the library is compiled with -ffp-contract=off, so
+// results are bit-identical to a serial IEEE FP64 evaluation of
+// MoSES_2PDF_landslide/adNOC_2Pb_kernels.cu (cited as L/K).
+// T = float is the FP32-bulk variant: the same expressions in single precision, with the
+// hardware reciprocal and square root (v_rcp_f32, v_sqrt_f32) in place of the correctly
+// rounded sequences.  Its metric comes from the FP64 setup; dt, the CFL maximum and the
+// mass totals stay FP64.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -43,6 +48,12 @@ __device__ __forceinline__ long long idx(const Grid& g, int r, int c) {
   return (long long)r * g.nxd + c;
 }
 
+// quotient and square root: IEEE for FP64, the hardware approximations for FP32
+__device__ __forceinline__ double qdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float qdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double qsqrt(double a) { return sqrt(a); }
+__device__ __forceinline__ float qsqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
+
 // TVD1..5Kernel (L/K:1000-1119) for one component: the slope at a cell from its two
 // neighbours, dA = dux[c-1], dB = dux[c].  The reference forms
 //   t1 = (sA+sB)/2 * min(dA*sA, dB*sB),  t2 = (dA+dB)/2,  slope = (sA'+sB')/2 * min(t1*sA', t2*sB')
@@ -52,30 +63,32 @@ __device__ __forceinline__ long long idx(const Grid& g, int r, int c) {
 // tested directly (a product dA*dB underflows for the tiny differences at a wet/dry front).
 // Only the sign of a zero slope can differ, and a zero's sign never reaches a nonzero value
 // in this scheme (no quantity is divided by a slope and no branch tests the sign of a zero).
-__device__ __forceinline__ double tvd_slope(double um, double u0, double up) {
-  const double dA = u0 - um;
-  const double dB = up - u0;
-  const double m = copysign(fmin(fabs(dA), fabs(dB)), dA);
-  const bool same = (dA > 0.0 && dB > 0.0) || (dA < 0.0 && dB < 0.0);
-  return same ? m : 0.0;
+template <typename T>
+__device__ __forceinline__ T tvd_slope(T um, T u0, T up) {
+  const T dA = u0 - um;
+  const T dB = up - u0;
+  const T m = copysign(fmin(fabs(dA), fabs(dB)), dA);
+  const bool same = (dA > T(0) && dB > T(0)) || (dA < T(0) && dB < T(0));
+  return same ? m : T(0);
 }
 
 // Interface states of InterfacesKernel (L/K:1226-1252) for one face, components 0..5
 // (component 6 of a face state is always replaced by KeepPositivi2Kernel).
 //   dir 0: x-face between (r,c) and (r,c+1): L = W state, R = E state.
 //   dir 1: y-face between (r,c) and (r+1,c): L = S state, R = N state.
-__device__ __forceinline__ void base_states(const double* __restrict__ U, const Grid& g, int dir,
-                                            int r, int c, double L[6], double R[6]) {
+template <typename T>
+__device__ __forceinline__ void base_states(const T* __restrict__ U, const Grid& g, int dir, int r,
+                                            int c, T L[6], T R[6]) {
   const long long st = dir == 0 ? 1 : g.nxd;
   const long long i = idx(g, r, c);
 #pragma unroll
   for (int k = 0; k < 6; k++) {
-    const double* a = U + k * g.P;
-    const double um = a[i - st], u0 = a[i], u1 = a[i + st], u2 = a[i + 2 * st];
-    const double s0 = tvd_slope(um, u0, u1);
-    const double s1 = tvd_slope(u0, u1, u2);
-    L[k] = u0 + 0.5 * s0;
-    R[k] = u1 - 0.5 * s1;
+    const T* a = U + k * g.P;
+    const T um = a[i - st], u0 = a[i], u1 = a[i + st], u2 = a[i + 2 * st];
+    const T s0 = tvd_slope(um, u0, u1);
+    const T s1 = tvd_slope(u0, u1, u2);
+    L[k] = u0 + T(0.5) * s0;
+    R[k] = u1 - T(0.5) * s1;
   }
 }
 
@@ -83,91 +96,87 @@ __device__ __forceinline__ void base_states(const double* __restrict__ U, const 
 // (columns 0-2 from 3,4; columns nx+3.. from nx+2,nx+1) from the InterfacesKernel values,
 // then all y-ghost overrides (rows 0-2 from 3,4; rows ny+3.. from ny+2,ny+1) from the
 // x-overridden values.  Valid for 1 <= r,c <= N-3.
-__device__ __forceinline__ void x_overridden(const double* __restrict__ U, const Grid& g, int dir,
-                                             int r, int c, double L[6], double R[6]) {
+template <typename T>
+__device__ __forceinline__ void x_overridden(const T* __restrict__ U, const Grid& g, int dir, int r,
+                                             int c, T L[6], T R[6]) {
   if (c <= 2 || c >= g.nx + MD) {
     const int ca = (c <= 2) ? MD : g.nx + MD - 1;
     const int cb = (c <= 2) ? MD + 1 : g.nx + MD - 2;
-    double La[6], Ra[6], Lb[6], Rb[6];
+    T La[6], Ra[6], Lb[6], Rb[6];
     base_states(U, g, dir, r, ca, La, Ra);
     base_states(U, g, dir, r, cb, Lb, Rb);
 #pragma unroll
     for (int k = 0; k < 6; k++) {
-      L[k] = 0.5 * (La[k] + Lb[k]);
-      R[k] = 0.5 * (Ra[k] + Rb[k]);
+      L[k] = T(0.5) * (La[k] + Lb[k]);
+      R[k] = T(0.5) * (Ra[k] + Rb[k]);
     }
   } else {
     base_states(U, g, dir, r, c, L, R);
   }
 }
 
-__device__ __forceinline__ void final_states_edge(const double* __restrict__ U, const Grid& g,
-                                               int dir, int r, int c, double L[6], double R[6]) {
+template <typename T>
+__device__ __forceinline__ void final_states_edge(const T* __restrict__ U, const Grid& g, int dir,
+                                                  int r, int c, T L[6], T R[6]) {
   if (r <= 2 || r >= g.ny + MD) {
     const int ra = (r <= 2) ? MD : g.ny + MD - 1;
     const int rb = (r <= 2) ? MD + 1 : g.ny + MD - 2;
-    double La[6], Ra[6], Lb[6], Rb[6];
+    T La[6], Ra[6], Lb[6], Rb[6];
     x_overridden(U, g, dir, ra, c, La, Ra);
     x_overridden(U, g, dir, rb, c, Lb, Rb);
 #pragma unroll
     for (int k = 0; k < 6; k++) {
-      L[k] = 0.5 * (La[k] + Lb[k]);
-      R[k] = 0.5 * (Ra[k] + Rb[k]);
+      L[k] = T(0.5) * (La[k] + Lb[k]);
+      R[k] = T(0.5) * (Ra[k] + Rb[k]);
     }
   } else {
     x_overridden(U, g, dir, r, c, L, R);
   }
 }
 
-// Face states of the face (dir, r, c): the InterfacesKernel values inside
-// [3, ny+2] x [3, nx+2], the Interfaces2Kernel overrides (out of line) on the ghost ring.
-__device__ __forceinline__ void final_states(const double* __restrict__ U, const Grid& g, int dir,
-                                             int r, int c, double L[6], double R[6]) {
-  if (r >= MD && r <= g.ny + MD - 1 && c >= MD && c <= g.nx + MD - 1)
-    base_states(U, g, dir, r, c, L, R);
-  else
-    final_states_edge(U, g, dir, r, c, L, R);
-}
-
 // KeepPositivi1Kernel + KeepPositivi2Kernel (L/K:1339-1424) on one face state.
-__device__ __forceinline__ double keep_positive(double a[6]) {
-  if (a[0] <= H00) {
-    a[0] = 0.0;
-    a[1] = 0.0;
-    a[2] = 0.0;
+template <typename T>
+__device__ __forceinline__ T keep_positive(T a[6]) {
+  if (a[0] <= T(H00)) {
+    a[0] = T(0);
+    a[1] = T(0);
+    a[2] = T(0);
   }
-  if (a[3] <= H00) {
-    a[3] = 0.0;
-    a[4] = 0.0;
-    a[5] = 0.0;
+  if (a[3] <= T(H00)) {
+    a[3] = T(0);
+    a[4] = T(0);
+    a[5] = T(0);
   }
-  return a[0] / (a[0] + a[3] + H0);
+  return qdiv(a[0], a[0] + a[3] + T(H0));
 }
 
 // Face-averaged metric of MeanKernel (L/K:1121-1224) plus the cell values Flux2Kernel
 // reads on either side (L/K:1506-1536).
+template <typename T>
 struct FaceMet {
-  double j11, j12, j13, j21, j22, j23, j31, j32, j33;  // invJ*_avg
-  double det, cav;                                    // Detmin_avg, cval_avg
-  double xa, xb;     // x-face: dxdxi11_avgEW, dxdxi21_avgEW; y-face: dxdxi12_avgSN, dxdxi22_avgSN
-  double sR0, sR1, cR;   // svec, cvalue of the E (x) / N (y) cell
-  double sL0, sL1, cL;   // of the W / S cell
+  T j11, j12, j13, j21, j22, j23, j31, j32, j33;  // invJ*_avg
+  T det, cav;                                    // Detmin_avg, cval_avg
+  T xa, xb;      // x-face: dxdxi11_avgEW, dxdxi21_avgEW; y-face: dxdxi12_avgSN, dxdxi22_avgSN
+  T sR0, sR1, cR;   // svec, cvalue of the E (x) / N (y) cell
+  T sL0, sL1, cL;   // of the W / S cell
 };
 
-__device__ __forceinline__ double avg(const double* a, long long i, long long j) {
-  return 0.5 * (a[i] + a[j]);
+template <typename T>
+__device__ __forceinline__ T avg(const T* a, long long i, long long j) {
+  return T(0.5) * (a[i] + a[j]);
 }
 
 // dir 0: face (r,c)-(r,c+1); dir 1: face (r,c)-(r+1,c).  Valid for faces in [1,N-3]^2,
 // where MeanKernel writes the averages.
-__device__ __forceinline__ void face_metric(const double* __restrict__ M, const Grid& g, int dir,
-                                            int r, int c, FaceMet& f) {
+template <typename T>
+__device__ __forceinline__ void face_metric(const T* __restrict__ M, const Grid& g, int dir, int r,
+                                            int c, FaceMet<T>& f) {
   const long long i = idx(g, r, c);
   const long long j = i + (dir == 0 ? 1 : g.nxd);
   const long long P = g.P;
-  const double* C = M + MC_C * P;
-  const double* S0 = M + MC_S0 * P;
-  const double* S1 = M + MC_S1 * P;
+  const T* C = M + MC_C * P;
+  const T* S0 = M + MC_S0 * P;
+  const T* S1 = M + MC_S1 * P;
   f.j11 = avg(M + MC_I11 * P, i, j);
   f.j12 = avg(M + MC_I12 * P, i, j);
   f.j13 = avg(M + MC_I13 * P, i, j);
@@ -180,9 +189,9 @@ __device__ __forceinline__ void face_metric(const double* __restrict__ M, const 
   f.sR0 = S0[j];
   f.sR1 = S1[j];
   f.cR = C[j];
-  f.j31 = 0.5 * ((-f.sL0) + (-f.sR0));
-  f.j32 = 0.5 * ((-f.sL1) + (-f.sR1));
-  f.j33 = 0.5 * (f.cL + f.cR);
+  f.j31 = T(0.5) * ((-f.sL0) + (-f.sR0));
+  f.j32 = T(0.5) * ((-f.sL1) + (-f.sR1));
+  f.j33 = T(0.5) * (f.cL + f.cR);
   f.cav = f.j33;
   f.det = avg(M + MC_DET * P, i, j);
   if (dir == 0) {
@@ -194,32 +203,35 @@ __device__ __forceinline__ void face_metric(const double* __restrict__ M, const 
   }
 }
 
-// Wave speeds of one face (Flux1..6Kernel, L/K:1426-1848): returns em for the solid
-// (emS) and the fluid (emF) after the 1e-10 floor.  L/R are final states; they leave with
-// positivity applied.  Also returns the quantities the flux needs.
+// Wave speeds of one face (Flux1..6Kernel, L/K:1426-1848): em for the solid (emS) and the
+// fluid (emF) after the 1e-10 floor.  L/R are final states; they leave with positivity
+// applied.  Also returns the quantities the flux needs.
+template <typename T>
 struct FaceSpeed {
-  double phiL, phiR;
-  double qL, qR, qFL, qFR;     // normal contravariant speeds (q_xi for x, q_et for y)
-  double ap, apF;              // apEW/apSN, apFEW/apFSN
-  double emS, emF;
+  T phiL, phiR;
+  T qL, qR, qFL, qFR;     // normal contravariant speeds (q_xi for x, q_et for y)
+  T ap, apF;              // apEW/apSN, apFEW/apFSN
+  T emS, emF;
 };
 
-__device__ __forceinline__ void face_speed(int dir, double L[6], double R[6], const FaceMet& f,
-                                           const Phys& ph, FaceSpeed& o) {
+template <typename T>
+__device__ __forceinline__ void face_speed(int dir, T L[6], T R[6], const FaceMet<T>& f,
+                                           const Phys& ph, FaceSpeed<T>& o) {
   o.phiR = keep_positive(R);
   o.phiL = keep_positive(L);
+  const T h0 = T(H0);
   // Flux1Kernel L/K:1448-1470
-  const double vxR = R[1] / (R[0] + H0), vyR = R[2] / (R[0] + H0);
-  const double vxL = L[1] / (L[0] + H0), vyL = L[2] / (L[0] + H0);
-  const double vxFR = R[4] / (R[3] + H0), vyFR = R[5] / (R[3] + H0);
-  const double vxFL = L[4] / (L[3] + H0), vyFL = L[5] / (L[3] + H0);
+  const T vxR = qdiv(R[1], R[0] + h0), vyR = qdiv(R[2], R[0] + h0);
+  const T vxL = qdiv(L[1], L[0] + h0), vyL = qdiv(L[2], L[0] + h0);
+  const T vxFR = qdiv(R[4], R[3] + h0), vyFR = qdiv(R[5], R[3] + h0);
+  const T vxFL = qdiv(L[4], L[3] + h0), vyFL = qdiv(L[5], L[3] + h0);
   // Flux2Kernel L/K:1506-1536
-  const double wR = (f.sR0 * vxR + f.sR1 * vyR) / f.cR;
-  const double wL = (f.sL0 * vxL + f.sL1 * vyL) / f.cL;
-  const double wFR = (f.sR0 * vxFR + f.sR1 * vyFR) / f.cR;
-  const double wFL = (f.sL0 * vxFL + f.sL1 * vyFL) / f.cL;
+  const T wR = qdiv(f.sR0 * vxR + f.sR1 * vyR, f.cR);
+  const T wL = qdiv(f.sL0 * vxL + f.sL1 * vyL, f.cL);
+  const T wFR = qdiv(f.sR0 * vxFR + f.sR1 * vyFR, f.cR);
+  const T wFL = qdiv(f.sL0 * vxFL + f.sL1 * vyFL, f.cL);
   // Flux3Kernel L/K:1593-1676 (only the normal component enters the flux)
-  double a1, a2, a3, an;
+  T a1, a2, a3, an;
   if (dir == 0) {
     a1 = f.j11; a2 = f.j12; a3 = f.j13; an = f.j11;
   } else {
@@ -229,51 +241,57 @@ __device__ __forceinline__ void face_speed(int dir, double L[6], double R[6], co
   o.qL = a1 * vxL + a2 * vyL + a3 * wL;
   o.qFR = a1 * vxFR + a2 * vyFR + a3 * wFR;
   o.qFL = a1 * vxFL + a2 * vyFL + a3 * wFL;
-  const double NpR = (R[0] + R[3]) * f.cav;
-  const double NpL = (L[0] + L[3]) * f.cav;
+  const T NpR = (R[0] + R[3]) * f.cav;
+  const T NpL = (L[0] + L[3]) * f.cav;
+  const T Drtio1 = T(ph.Drtio1);
   // Flux4Kernel L/K:1715-1737
-  const double apR = fabs(o.qR) + an * sqrt(NpR * ph.Drtio1);
-  const double apL = fabs(o.qL) + an * sqrt(NpL * ph.Drtio1);
-  const double apFR = fabs(o.qFR) + an * sqrt(NpR / (1.0 - o.phiR + H0));
-  const double apFL = fabs(o.qFL) + an * sqrt(NpL / (1.0 - o.phiL + H0));
+  const T apR = fabs(o.qR) + an * qsqrt(NpR * Drtio1);
+  const T apL = fabs(o.qL) + an * qsqrt(NpL * Drtio1);
+  const T apFR = fabs(o.qFR) + an * qsqrt(qdiv(NpR, T(1.0) - o.phiR + h0));
+  const T apFL = fabs(o.qFL) + an * qsqrt(qdiv(NpL, T(1.0) - o.phiL + h0));
   // Flux5Kernel L/K:1762-1766 (max(E, W) / max(N, S))
   o.ap = fmax(apR, apL);
   o.apF = fmax(apFR, apFL);
   // Flux6Kernel L/K:1803-1807
-  o.emS = fmax(1.0e-10, o.ap);
-  o.emF = fmax(1.0e-10, o.apF);
+  o.emS = fmax(T(1.0e-10), o.ap);
+  o.emF = fmax(T(1.0e-10), o.apF);
 }
 
 // Numerical flux of one face: Hp (6), Pp (components 1,2,4,5) and PD (4,5).
+template <typename T>
 struct FaceFlux {
-  double H[6];
-  double P1, P2, P4, P5;
-  double D4, D5;
+  T H[6];
+  T P1, P2, P4, P5;
+  T D4, D5;
 };
 
 // Cell values the viscous term needs on both sides of the face (Flux8Kernel
 // L/K:2182-2205): fluid momenta and depth, and the tangential limited slopes of the
 // fluid momenta.
+template <typename T>
 struct PDSide {
-  double u3, u4, u5;   // cell state
-  double t4, t5;       // slope of u4, u5 along the face (uy for x-faces, ux for y-faces)
+  T u3, u4, u5;   // cell state
+  T t4, t5;       // slope of u4, u5 along the face (uy for x-faces, ux for y-faces)
 };
 
-__device__ __forceinline__ void face_flux(int dir, const double L[6], const double R[6],
-                                          const FaceMet& f, const FaceSpeed& s, const PDSide& pl,
-                                          const PDSide& pr, const Grid& g, const Phys& ph,
-                                          FaceFlux& o) {
-  const double M = (dir == 0) ? f.j11 * f.j11 + f.j12 * f.j12 + f.j13 * f.j13
-                              : f.j21 * f.j21 + f.j22 * f.j22 + f.j23 * f.j23;
+template <typename T>
+__device__ __forceinline__ void face_flux(int dir, const T L[6], const T R[6], const FaceMet<T>& f,
+                                          const FaceSpeed<T>& s, const PDSide<T>& pl,
+                                          const PDSide<T>& pr, const Grid& g, const Phys& ph,
+                                          FaceFlux<T>& o) {
+  const T M = (dir == 0) ? f.j11 * f.j11 + f.j12 * f.j12 + f.j13 * f.j13
+                         : f.j21 * f.j21 + f.j22 * f.j22 + f.j23 * f.j23;
+  const T Drtio1 = T(ph.Drtio1);
+  const T h0 = T(H0);
   // Flux6Kernel L/K:1809-1843
-  const double epsilon0 = 1.0;
-  const double kk = 0.5 * epsilon0 * f.cav * f.det * M;
-  const double czwR = R[0] * (R[0] + R[3]) * (kk)*ph.Drtio1;
-  const double czwL = L[0] * (L[0] + L[3]) * (kk)*ph.Drtio1;
-  const double czwFR = (R[0] + R[3]) * (R[0] + R[3]) * (kk);
-  const double czwFL = (L[0] + L[3]) * (L[0] + L[3]) * (kk);
+  const T epsilon0 = T(1.0);
+  const T kk = T(0.5) * epsilon0 * f.cav * f.det * M;
+  const T czwR = R[0] * (R[0] + R[3]) * (kk)*Drtio1;
+  const T czwL = L[0] * (L[0] + L[3]) * (kk)*Drtio1;
+  const T czwFR = (R[0] + R[3]) * (R[0] + R[3]) * (kk);
+  const T czwFL = (L[0] + L[3]) * (L[0] + L[3]) * (kk);
   // Flux7Kernel L/K:1883-1932
-  double FR[6], FL[6];
+  T FR[6], FL[6];
   FR[0] = R[0] * s.qR * f.det;
   FR[1] = R[1] * s.qR * f.det + czwR * f.xa;
   FR[2] = R[2] * s.qR * f.det + czwR * f.xb;
@@ -289,55 +307,59 @@ __device__ __forceinline__ void face_flux(int dir, const double L[6], const doub
   // Flux8Kernel L/K:2081-2128: Hp = 0.5*(F_E + F_W - a*Det*(u_E - u_W))
 #pragma unroll
   for (int k = 0; k < 6; k++) {
-    const double a = (k < 3) ? s.ap : s.apF;
-    o.H[k] = 0.5 * (FR[k] + FL[k] - a * f.det * (R[k] - L[k]));
+    const T a = (k < 3) ? s.ap : s.apF;
+    o.H[k] = T(0.5) * (FR[k] + FL[k] - a * f.det * (R[k] - L[k]));
   }
   // Flux8Kernel L/K:2131-2179: non-conservative pressure terms
-  const double cc = 0.5 * epsilon0 * f.cav * f.det;
-  const double hh = 0.5 * ((R[0] + R[3]) + (L[0] + L[3]));
-  const double pa = (dir == 0) ? f.j11 : f.j21;
-  const double pb = (dir == 0) ? f.j12 : f.j22;
-  o.P1 = -ph.rr * (cc)*hh * hh * pa;
-  o.P2 = -ph.rr * (cc)*hh * hh * pb;
+  const T cc = T(0.5) * epsilon0 * f.cav * f.det;
+  const T hh = T(0.5) * ((R[0] + R[3]) + (L[0] + L[3]));
+  const T pa = (dir == 0) ? f.j11 : f.j21;
+  const T pb = (dir == 0) ? f.j12 : f.j22;
+  const T rr = T(ph.rr);
+  o.P1 = -rr * (cc)*hh * hh * pa;
+  o.P2 = -rr * (cc)*hh * hh * pb;
   o.P4 = (cc)*hh * hh * pa;
   o.P5 = (cc)*hh * hh * pb;
   // Flux8Kernel L/K:2182-2205: fluid velocity gradients at the face
-  double dudx, dvdx, dudy, dvdy;
+  const T dx = T(g.dx), dy = T(g.dy);
+  T dudx, dvdx, dudy, dvdy;
   if (dir == 0) {
-    dudx = (pr.u4 / (pr.u3 + H0) - pl.u4 / (pl.u3 + H0)) / g.dx;
-    dvdx = (pr.u5 / (pr.u3 + H0) - pl.u5 / (pl.u3 + H0)) / g.dx;
-    dudy = 0.5 * (pr.t4 / (pr.u3 + H0) + pl.t4 / (pl.u3 + H0)) / g.dy;
-    dvdy = 0.5 * (pr.t5 / (pr.u3 + H0) + pl.t5 / (pl.u3 + H0)) / g.dy;
+    dudx = qdiv(qdiv(pr.u4, pr.u3 + h0) - qdiv(pl.u4, pl.u3 + h0), dx);
+    dvdx = qdiv(qdiv(pr.u5, pr.u3 + h0) - qdiv(pl.u5, pl.u3 + h0), dx);
+    dudy = qdiv(T(0.5) * (qdiv(pr.t4, pr.u3 + h0) + qdiv(pl.t4, pl.u3 + h0)), dy);
+    dvdy = qdiv(T(0.5) * (qdiv(pr.t5, pr.u3 + h0) + qdiv(pl.t5, pl.u3 + h0)), dy);
   } else {
-    dudx = 0.5 * (pr.t4 / (pr.u3 + H0) + pl.t4 / (pl.u3 + H0)) / g.dx;
-    dvdx = 0.5 * (pr.t5 / (pr.u3 + H0) + pl.t5 / (pl.u3 + H0)) / g.dx;
-    dudy = (pr.u4 / (pr.u3 + H0) - pl.u4 / (pl.u3 + H0)) / g.dy;
-    dvdy = (pr.u5 / (pr.u3 + H0) - pl.u5 / (pl.u3 + H0)) / g.dy;
+    dudx = qdiv(T(0.5) * (qdiv(pr.t4, pr.u3 + h0) + qdiv(pl.t4, pl.u3 + h0)), dx);
+    dvdx = qdiv(T(0.5) * (qdiv(pr.t5, pr.u3 + h0) + qdiv(pl.t5, pl.u3 + h0)), dx);
+    dudy = qdiv(qdiv(pr.u4, pr.u3 + h0) - qdiv(pl.u4, pl.u3 + h0), dy);
+    dvdy = qdiv(qdiv(pr.u5, pr.u3 + h0) - qdiv(pl.u5, pl.u3 + h0), dy);
   }
   // Flux9Kernel L/K:2242-2273
-  const double duxidxi = f.j11 * dudx + f.j12 * dvdx - f.j13 * (f.j31 * dudx + f.j32 * dvdx) / f.j33;
-  const double dvetdxi = f.j21 * dudx + f.j22 * dvdx - f.j23 * (f.j31 * dudx + f.j32 * dvdx) / f.j33;
-  const double duxidet = f.j11 * dudy + f.j12 * dvdy - f.j13 * (f.j31 * dudy + f.j32 * dvdy) / f.j33;
-  const double dvetdet = f.j21 * dudy + f.j22 * dvdy - f.j23 * (f.j31 * dudy + f.j32 * dvdy) / f.j33;
+  const T duxidxi = f.j11 * dudx + f.j12 * dvdx - qdiv(f.j13 * (f.j31 * dudx + f.j32 * dvdx), f.j33);
+  const T dvetdxi = f.j21 * dudx + f.j22 * dvdx - qdiv(f.j23 * (f.j31 * dudx + f.j32 * dvdx), f.j33);
+  const T duxidet = f.j11 * dudy + f.j12 * dvdy - qdiv(f.j13 * (f.j31 * dudy + f.j32 * dvdy), f.j33);
+  const T dvetdet = f.j21 * dudy + f.j22 * dvdy - qdiv(f.j23 * (f.j31 * dudy + f.j32 * dvdy), f.j33);
   // Flux10Kernel L/K:2316-2343
   if (dir == 0) {
-    o.D4 = 2.0 * f.det * (hh) * (f.j11 * duxidxi + f.j21 * duxidet);
+    o.D4 = T(2.0) * f.det * (hh) * (f.j11 * duxidxi + f.j21 * duxidet);
     o.D5 = f.det * (hh) * (f.j12 * duxidxi + f.j22 * duxidet + f.j11 * dvetdxi + f.j21 * dvetdet);
   } else {
     o.D4 = f.det * (hh) * (f.j12 * duxidxi + f.j22 * duxidet + f.j11 * dvetdxi + f.j21 * dvetdet);
-    o.D5 = 2.0 * f.det * (hh) * (f.j12 * dvetdxi + f.j22 * dvetdet);
+    o.D5 = T(2.0) * f.det * (hh) * (f.j12 * dvetdxi + f.j22 * dvetdet);
   }
 }
 
 // Cell metric for the source terms.
+template <typename T>
 struct CellMet {
-  double c, s0, s1, det;
-  double i11, i12, i13, i21, i22, i23;
-  double j13x, j23x, j33x, j13e, j23e, j33e;
+  T c, s0, s1, det;
+  T i11, i12, i13, i21, i22, i23;
+  T j13x, j23x, j33x, j13e, j23e, j33e;
 };
 
-__device__ __forceinline__ void cell_metric(const double* __restrict__ M, long long P, long long i,
-                                            CellMet& m) {
+template <typename T>
+__device__ __forceinline__ void cell_metric(const T* __restrict__ M, long long P, long long i,
+                                            CellMet<T>& m) {
   m.c = M[MC_C * P + i];
   m.s0 = M[MC_S0 * P + i];
   m.s1 = M[MC_S1 * P + i];
@@ -357,65 +379,69 @@ __device__ __forceinline__ void cell_metric(const double* __restrict__ M, long l
 }
 
 // Flux11..17Kernel (L/K:2349-2621): the source vector s (components 1,2,4,5; s0 = s3 = 0).
-__device__ __forceinline__ void cell_source(const double u[7], const CellMet& m, const Phys& ph,
-                                            double s[6]) {
-  const double h = u[0], hf = u[3];
-  double vex, vey, vexF, veyF;
-  if (h <= H00) {  // Flux11
-    vex = 0.0;
-    vey = 0.0;
+template <typename T>
+__device__ __forceinline__ void cell_source(const T u[7], const CellMet<T>& m, const Phys& ph,
+                                            T s[6]) {
+  const T h = u[0], hf = u[3];
+  const T h0 = T(H0), u0 = T(U0);
+  T vex, vey, vexF, veyF;
+  if (h <= T(H00)) {  // Flux11
+    vex = T(0);
+    vey = T(0);
   } else {
-    vex = u[1] / (h + H0);
-    vey = u[2] / (h + H0);
+    vex = qdiv(u[1], h + h0);
+    vey = qdiv(u[2], h + h0);
   }
-  if (hf <= H00) {
-    vexF = 0.0;
-    veyF = 0.0;
+  if (hf <= T(H00)) {
+    vexF = T(0);
+    veyF = T(0);
   } else {
-    vexF = u[4] / (hf + H0);
-    veyF = u[5] / (hf + H0);
+    vexF = qdiv(u[4], hf + h0);
+    veyF = qdiv(u[5], hf + h0);
   }
-  const double w = (m.s0 * vex + m.s1 * vey) / m.c;  // Flux12
-  const double wF = (m.s0 * vexF + m.s1 * veyF) / m.c;
-  const double usw = vex * vex + vey * vey + w * w;  // Flux13
-  double vel = sqrt(vex * vex + vey * vey + w * w);
-  double vexw, veyw;
-  if (usw <= U0 * U0) {  // Flux14
-    vexw = vex / U0;
-    veyw = vey / U0;
-    vel = U0;
+  const T w = qdiv(m.s0 * vex + m.s1 * vey, m.c);  // Flux12
+  const T wF = qdiv(m.s0 * vexF + m.s1 * veyF, m.c);
+  const T usw = vex * vex + vey * vey + w * w;  // Flux13
+  T vel = qsqrt(vex * vex + vey * vey + w * w);
+  T vexw, veyw;
+  if (usw <= u0 * u0) {  // Flux14
+    vexw = qdiv(vex, u0);
+    veyw = qdiv(vey, u0);
+    vel = u0;
   } else {
-    vexw = vex / vel;
-    veyw = vey / vel;
+    vexw = qdiv(vex, vel);
+    veyw = qdiv(vey, vel);
   }
-  const double q_xi = m.i11 * vex + m.i12 * vey + m.i13 * w;
-  const double q_et = m.i21 * vex + m.i22 * vey + m.i23 * w;
-  const double q_xiF = m.i11 * vexF + m.i12 * veyF + m.i13 * wF;
-  const double q_etF = m.i21 * vexF + m.i22 * veyF + m.i23 * wF;
+  const T q_xi = m.i11 * vex + m.i12 * vey + m.i13 * w;
+  const T q_et = m.i21 * vex + m.i22 * vey + m.i23 * w;
+  const T q_xiF = m.i11 * vexF + m.i12 * veyF + m.i13 * wF;
+  const T q_etF = m.i21 * vexF + m.i22 * veyF + m.i23 * wF;
   // Flux15
-  const double Ac = vex * m.j13x * q_xi + vey * m.j23x * q_xi + w * m.j33x * q_xi +
-                    vex * m.j13e * q_et + vey * m.j23e * q_et + w * m.j33e * q_et;
-  const double AcF = vexF * m.j13x * q_xiF + veyF * m.j23x * q_xiF + wF * m.j33x * q_xiF +
-                     vexF * m.j13e * q_etF + veyF * m.j23e * q_etF + wF * m.j33e * q_etF;
+  const T Ac = vex * m.j13x * q_xi + vey * m.j23x * q_xi + w * m.j33x * q_xi +
+               vex * m.j13e * q_et + vey * m.j23e * q_et + w * m.j33e * q_et;
+  const T AcF = vexF * m.j13x * q_xiF + veyF * m.j23x * q_xiF + wF * m.j33x * q_xiF +
+                vexF * m.j13e * q_etF + veyF * m.j23e * q_etF + wF * m.j33e * q_etF;
   // Flux16
-  const double D = m.det;
-  const double Npress1 = fmax(0.0, (D * h * (ph.Drtio1 * m.c - Ac + AcF * ph.rr)));
-  const double Npress2 = fmax(0.0, (D * h * (m.c - Ac)));
-  const double NpressF = fmax(0.0, (D * hf * (m.c - AcF)));
+  const T D = m.det;
+  const T Drtio1 = T(ph.Drtio1), rr = T(ph.rr), Cd = T(ph.Cd), tande = T(ph.tande);
+  const T Npress1 = fmax(T(0), (D * h * (Drtio1 * m.c - Ac + AcF * rr)));
+  const T Npress2 = fmax(T(0), (D * h * (m.c - Ac)));
+  const T NpressF = fmax(T(0), (D * hf * (m.c - AcF)));
   // Flux17
-  const double phi = u[6];
-  const double epsilon0 = 1.0;
-  const double Dratio = ph.rr;
-  s[0] = 0.0;
-  s[1] = -Npress2 * m.s0 - Npress1 * ph.tande * vexw +
-         Dratio * D * ph.Cd * phi * (1.0 - phi) * (h + hf) * (vexF - vex);
-  s[2] = -Npress2 * m.s1 - Npress1 * ph.tande * veyw +
-         Dratio * D * ph.Cd * phi * (1.0 - phi) * (h + hf) * (veyF - vey);
-  s[3] = 0.0;
-  s[4] = -NpressF * m.s0 - D * ph.Cd * phi * (1.0 - phi) * (h + hf) * (vexF - vex) +
-         D * (-(1.0 - phi) * (h + hf) * ph.varTheta * vexF / epsilon0) / ph.N_R;
-  s[5] = -NpressF * m.s1 - D * ph.Cd * phi * (1.0 - phi) * (h + hf) * (veyF - vey) +
-         D * (-(1.0 - phi) * (h + hf) * ph.varTheta * veyF / epsilon0) / ph.N_R;
+  const T phi = u[6];
+  const T epsilon0 = T(1.0);
+  const T Dratio = rr;
+  const T varTheta = T(ph.varTheta), N_R = T(ph.N_R);
+  s[0] = T(0);
+  s[1] = -Npress2 * m.s0 - Npress1 * tande * vexw +
+         Dratio * D * Cd * phi * (T(1.0) - phi) * (h + hf) * (vexF - vex);
+  s[2] = -Npress2 * m.s1 - Npress1 * tande * veyw +
+         Dratio * D * Cd * phi * (T(1.0) - phi) * (h + hf) * (veyF - vey);
+  s[3] = T(0);
+  s[4] = -NpressF * m.s0 - D * Cd * phi * (T(1.0) - phi) * (h + hf) * (vexF - vex) +
+         qdiv(D * qdiv(-(T(1.0) - phi) * (h + hf) * varTheta * vexF, epsilon0), N_R);
+  s[5] = -NpressF * m.s1 - D * Cd * phi * (T(1.0) - phi) * (h + hf) * (veyF - vey) +
+         qdiv(D * qdiv(-(T(1.0) - phi) * (h + hf) * varTheta * veyF, epsilon0), N_R);
 }
 
 }  // namespace m2p

@@ -20,8 +20,11 @@ _LIB = None
 
 M2P_LANDSLIDE = 0
 M2P_INFLOW = 1
+M2P_FP64 = 0
+M2P_FP32 = 1
+PRECISIONS = {"fp64": M2P_FP64, "fp32": M2P_FP32}
 
-EXPORTS = ["m2p_default_params", "m2p_open", "m2p_open_grid", "m2p_run", "m2p_step", "m2p_sync",
+EXPORTS = ["m2p_default_params", "m2p_open", "m2p_open_grid", "m2p_run", "m2p_step", "m2p_run_to", "m2p_sync",
            "m2p_get_state", "m2p_set_state", "m2p_dims", "m2p_mass", "m2p_scalars",
            "m2p_get_field", "m2p_profile", "m2p_profile_get", "m2p_last_error", "m2p_close"]
 
@@ -33,7 +36,8 @@ class M2PError(RuntimeError):
 class Params(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
                 ("total_sim", "each_out", "delta0", "Cd", "N_R", "varTheta", "phiS0", "cfl",
-                 "rho_ratio", "dt_min", "dt_max")]
+                 "rho_ratio", "dt_min", "dt_max")] + [("precision", ctypes.c_int),
+                                                       ("reserved", ctypes.c_int)]
 
 
 def lib():
@@ -56,6 +60,7 @@ def lib():
                                 dp, dp, ctypes.POINTER(vp)]
     L.m2p_run.argtypes = [vp, ctypes.c_char_p, dp]
     L.m2p_step.argtypes = [vp, ctypes.c_int]
+    L.m2p_run_to.argtypes = [vp, ctypes.c_double, ctypes.c_int, ip]
     L.m2p_sync.argtypes = [vp]
     L.m2p_get_state.argtypes = [vp, dp, dp, ip]
     L.m2p_set_state.argtypes = [vp, dp]
@@ -90,13 +95,20 @@ def default_params(mode: int) -> Params:
 
 
 class Simulation:
-    """One scenario resident on the GPU."""
+    """One scenario resident on the GPU.  ``precision`` is "fp64" (the reference's
+    arithmetic) or "fp32" (FP32-bulk: float state and per-step arithmetic, FP64 metric
+    setup, CFL max, dt control and mass totals)."""
 
     def __init__(self, sc=None, *, par_list: Optional[str] = None, mode: Optional[int] = None,
-                 overlay: Optional[str] = None):
+                 overlay: Optional[str] = None, precision: str = "fp64"):
         L = lib()
         self._h = ctypes.c_void_p()
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
+        self.precision = precision
         if par_list is not None:
+            if precision != "fp64":
+                overlay = ((overlay + ";") if overlay else "") + "precision=" + precision
             rc = L.m2p_open(par_list.encode(), int(mode), overlay.encode() if overlay else None,
                             ctypes.byref(self._h))
         else:
@@ -115,6 +127,7 @@ class Simulation:
         p.total_sim, p.each_out = float(sc.total_sim), float(sc.each_out)
         p.delta0, p.Cd, p.N_R, p.varTheta, p.phiS0 = sc.delta0, sc.Cd, sc.N_R, sc.varTheta, sc.phiS0
         p.cfl, p.rho_ratio, p.dt_min, p.dt_max = sc.cfl, sc.rho_ratio, sc.dt_floor(), sc.dt_max
+        p.precision = PRECISIONS[self.precision]
         f32 = lambda a: None if a is None else np.ascontiguousarray(
             np.asarray(a, np.float64).astype(np.float32).astype(np.float64))
         topo = f32(sc.topo)
@@ -146,6 +159,14 @@ class Simulation:
 
     def step(self, n: int = 1) -> None:
         self._check(lib().m2p_step(self._h, int(n)), "m2p_step")
+
+    def run_to(self, t_end: float, max_steps: int = 1 << 30) -> int:
+        """Steps until the scaled simulated time reaches ``t_end`` (the last step shortened
+        to end on it) or ``max_steps`` steps are done; returns the steps taken."""
+        n = ctypes.c_int()
+        self._check(lib().m2p_run_to(self._h, float(t_end), int(max_steps), ctypes.byref(n)),
+                    "m2p_run_to")
+        return n.value
 
     def sync(self) -> None:
         self._check(lib().m2p_sync(self._h), "m2p_sync")

@@ -170,12 +170,13 @@ def landslide_plane(n: int = 256, seed: int = 1, dx: float = 20.0, phi: float = 
                     delta0=20.0, Cd=REF_CD, N_R=REF_NR, varTheta=REF_VARTHETA)
 
 
-def landslide_fractal(n: int = 2048, seed: int = 2, dx: float = 20.0) -> Scenario:
-    """Fractal DEM (diamond-square, Hurst 0.8, relief 1500 m, 3-cell box filter) with three
-    seeded paraboloid releases of 1e5..5e6 m^3 and per-release solid fraction in
+def landslide_fractal(n: int = 2048, seed: int = 2, dx: float = 20.0,
+                      relief: float = 1500.0) -> Scenario:
+    """Fractal DEM (diamond-square, Hurst 0.8, ``relief`` metres, 3-cell box filter) with
+    three seeded paraboloid releases of 1e5..5e6 m^3 and per-release solid fraction in
     [0.4, 0.7] (the per-cell phi extension)."""
     rng = np.random.default_rng(seed)
-    topo = fractal_dem(n, n, 1500.0, 0.8, seed)
+    topo = fractal_dem(n, n, relief, 0.8, seed)
     depth = np.zeros_like(topo)
     phi = np.full_like(topo, 0.5)
     hi = np.argwhere(topo > np.median(topo))
@@ -191,6 +192,68 @@ def landslide_fractal(n: int = 2048, seed: int = 2, dx: float = 20.0) -> Scenari
         phi = np.where(mask & (p > 0), ph, phi)
     return Scenario(mode=0, topo=topo, depth=depth, phi=phi, cellsize=dx, phiS0=0.5,
                     delta0=20.0)
+
+
+def self_affine_relief(n: int, dx: float = 20.0, relief_2048: float = 1500.0,
+                       hurst: float = 0.8) -> float:
+    """Relief of an n-cell square that has the slope statistics of a 2048-cell one with
+    ``relief_2048`` metres: relief scales as extent^H for a self-affine surface."""
+    return relief_2048 * (n * dx / (2048 * 20.0)) ** hurst
+
+
+@dataclasses.dataclass
+class MemberParams:
+    """One ensemble member: grid side and the sampled physical parameters."""
+    n: int
+    seed: int
+    delta0: float          # basal friction angle [deg]
+    internal: float        # internal friction angle [deg] (sampled; this model has none)
+    volume: float          # release volume [m^3]
+    phi: float             # solid volume fraction of the release
+
+
+def latin_hypercube(count: int, seed: int, ranges) -> np.ndarray:
+    """``count`` samples of len(ranges) variables, one per stratum of each, strata paired
+    by independent random permutations; each range is (lo, hi, 'lin' | 'log')."""
+    rng = np.random.default_rng(seed)
+    out = np.empty((count, len(ranges)))
+    for k, (lo, hi, kind) in enumerate(ranges):
+        u = (rng.permutation(count) + rng.uniform(size=count)) / count
+        if kind == "log":
+            out[:, k] = np.exp(math.log(lo) + u * (math.log(hi) - math.log(lo)))
+        else:
+            out[:, k] = lo + u * (hi - lo)
+    return out
+
+
+def ensemble_members(count: int = 256, seed: int = 4, sizes=(128, 256, 512, 1024),
+                     weights=(4, 3, 2, 1)):
+    """The batched-ensemble population: sides drawn from ``sizes`` with ``weights``, basal
+    friction 15-30 deg, internal friction 30-40 deg, release volume 1e5-5e6 m^3 (log),
+    solid fraction 0.4-0.7, by Latin hypercube."""
+    rng = np.random.default_rng(seed)
+    w = np.asarray(weights, np.float64)
+    ns = rng.choice(np.asarray(sizes), size=count, p=w / w.sum())
+    lhs = latin_hypercube(count, seed + 1, [(15.0, 30.0, "lin"), (30.0, 40.0, "lin"),
+                                            (1e5, 5e6, "log"), (0.4, 0.7, "lin")])
+    return [MemberParams(int(ns[k]), seed * 100003 + k, *map(float, lhs[k])) for k in range(count)]
+
+
+def ensemble_scenario(m: MemberParams, dx: float = 20.0) -> Scenario:
+    """Landslide of one ensemble member: seeded fractal DEM with self-affine relief
+    (the slope statistics of the 2048-cell, 1500 m DEM), one paraboloid release of
+    ``m.volume`` at a seeded high point (20 m peak depth, radius capped at a quarter of
+    the domain), solid fraction ``m.phi``."""
+    rng = np.random.default_rng(m.seed)
+    n = m.n
+    topo = fractal_dem(n, n, self_affine_relief(n, dx), 0.8, m.seed)
+    radius = min(math.sqrt(2.0 * m.volume / (math.pi * 20.0)), 0.25 * n * dx)
+    rc = int(math.ceil(radius / dx)) + 2
+    inner = topo[rc:n - rc, rc:n - rc]
+    cand = np.argwhere(inner >= np.quantile(inner, 0.75)) + rc
+    j, i = cand[rng.integers(len(cand))]
+    depth = paraboloid(n, n, dx, i * dx, j * dx, m.volume, radius)
+    return Scenario(mode=0, topo=topo, depth=depth, cellsize=dx, phiS0=m.phi, delta0=m.delta0)
 
 
 def inflow_valley(nx: int = 1024, ny: int = 4096, seed: int = 3, dx: float = 20.0,

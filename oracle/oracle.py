@@ -107,7 +107,7 @@ def output_times(total_sim: float, each_out: float) -> np.ndarray:
 class Oracle:
     """One simulation on the CPU oracle."""
 
-    def __init__(self, sc, outputs: bool = False):
+    def __init__(self, sc, outputs: bool = False, tf=None):
         L = lib()
         self.sc = sc
         topo = pad(f32(sc.topo))
@@ -118,8 +118,12 @@ class Oracle:
         dx = float(np.float32(sc.cellsize)) * 10
         par = np.array([sc.delta0, sc.Cd, sc.N_R, sc.varTheta, sc.phiS0, sc.cfl, sc.rho_ratio,
                         sc.dt_floor(), sc.dt_max], dtype=np.float64)
-        tf = output_times(sc.total_sim, sc.each_out) if outputs else np.zeros(1)
-        ntf = len(tf) if outputs else 0
+        if tf is not None:  # explicit clipping times (tf[0] = 0, as output_times gives)
+            tf = np.ascontiguousarray(tf, dtype=np.float64)
+            ntf = len(tf)
+        else:
+            tf = output_times(sc.total_sim, sc.each_out) if outputs else np.zeros(1)
+            ntf = len(tf) if outputs else 0
         if sc.mode == 1:
             table, loc, dire, itime = inflow_tables(sc)
             nsamp, nloc = sc.inflow_h.shape

@@ -123,6 +123,29 @@ def test_parity_set_state_midrun():
         assert rel <= TOL
 
 
+def test_run_to_matches_oracle_clipping():
+    """m2p_run_to(t_end): the step that reaches t_end is shortened to end on it (the
+    oracle's output-time clipping with tf = [0, t_end]) and nothing runs after it."""
+    sc = S.landslide_plane(n=48)
+    o0 = Oracle(sc)
+    o0.step(7)
+    t_end = 0.5 * (o0.scalar("time") + o0.scalar("time") - o0.scalar("dt"))  # inside step 7
+    o = Oracle(sc, tf=np.array([0.0, t_end, 1e30]))   # (a last entry it never reaches)
+    o.step(7)
+    assert o.scalar("time") == t_end
+    with Simulation(sc) as g:
+        n = g.run_to(t_end, max_steps=1000)
+        tg, ntg = g.time_steps()
+        assert (n, ntg, tg) == (7, 7, t_end)
+        rel, ndiff = compare(interior(g.state()), interior(o.get("u")))
+        assert ndiff == 0, (rel, ndiff)
+        g.step(3)                 # plain stepping resumes after run_to
+        o.step(3)
+        rel, ndiff = compare(interior(g.state()), interior(o.get("u")))
+        assert ndiff == 0 and g.time_steps()[1] == 10
+        assert g.run_to(t_end) == 0   # already past it
+
+
 def test_c1_workload_parity_256():
     """The 256^2 landslide configuration (35 degree plane, bump, 1e6 m^3 release,
     phi 0.6, CFL 0.4, density ratio 1000/2700) for 100 steps."""
