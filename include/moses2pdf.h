@@ -134,7 +134,61 @@ int m2p_profile(m2p_ctx* ctx, int enable);
 int m2p_profile_get(m2p_ctx* ctx, const char* kernel, double* ms_per_launch, int* launches);
 
 const char* m2p_last_error(const m2p_ctx* ctx);
+
+/* Closes the context (removing it from its batch first) and frees its device memory. */
 void m2p_close(m2p_ctx* ctx);
+
+/* ---- Halting rules and status (no reference counterpart) -------------------------- */
+
+/* Sets when a scenario halts, counted from now: t_end (scaled time; <= 0: none; the step
+ * that reaches it is shortened to end on it), vstop (flow speed, scaled units = m/s;
+ * <= 0: none; the scenario takes one more step after the first corrector whose largest
+ * flow speed over cells deeper than 1 mm falls below vstop, once some corrector has
+ * reached vstop), max_steps (< 0: none).  The kernels of a halted scenario return at
+ * once, so it can stay in a batch or a captured graph.  Clears an earlier halt. */
+int m2p_set_stop(m2p_ctx* ctx, double t_end, double vstop, int max_steps);
+
+/* Simulated time (scaled), steps done and whether the scenario has halted. */
+int m2p_status(m2p_ctx* ctx, double* t, int* nt, int* halted);
+
+/* Device bytes a scenario of this shape holds once open (*steady) and at most while it
+ * is being set up (*peak), for admission by HBM footprint. */
+int m2p_footprint(int mode, int NX, int NY, int precision, int nsamp, int nloc,
+                  long long* steady, long long* peak);
+
+/* ---- Batched ensembles (no reference counterpart) --------------------------------- */
+
+/* A batch steps many scenarios of one precision, of any grid sizes, with one launch per
+ * phase of the time step for all of them (inflow, ghost fill, CFL box and ring, dt,
+ * predictor, ghost fill, corrector), captured as a HIP graph of `graph_steps` steps.
+ * Each member keeps its own buffers, dt and halting rules; its results are bit-identical
+ * to stepping it alone with m2p_step.  While a scenario is in a batch, m2p_step,
+ * m2p_run, m2p_run_to and m2p_set_state on it return M2P_ESTATE; the read functions work
+ * (they order after the batch's queued steps). */
+typedef struct m2p_batch m2p_batch;
+
+int m2p_batch_create(int precision, m2p_batch** out);
+/* Returns the member's index (>= 0) or an error code.  The caller keeps ownership. */
+int m2p_batch_add(m2p_batch* b, m2p_ctx* ctx);
+int m2p_batch_remove(m2p_batch* b, m2p_ctx* ctx);
+int m2p_batch_size(const m2p_batch* b);
+/* Enqueues nsteps steps of every member (asynchronous; m2p_batch_sync waits). */
+int m2p_batch_step(m2p_batch* b, int nsteps);
+int m2p_batch_sync(m2p_batch* b);
+/* Autotuner: times every compiled stage-kernel variant (tile height x waves per SIMD)
+ * under each block schedule (XCD banding on/off, member order as added / largest first)
+ * for trial_steps steps on the batch itself, restores every member's state, and keeps
+ * the fastest.  A text table of the timings goes to report (nullable). */
+int m2p_batch_tune(m2p_batch* b, int trial_steps, char* report, int report_len);
+/* Manual schedule: keys "ty", "waves", "remap", "order", "graph_steps" (0: no graph).
+ * The (ty, waves) pair must be a compiled variant; m2p_batch_step reports M2P_EINVAL
+ * otherwise. */
+int m2p_batch_set(m2p_batch* b, const char* key, int value);
+int m2p_batch_get(const m2p_batch* b, const char* key);
+const char* m2p_batch_last_error(const m2p_batch* b);
+/* Detaches the members (they keep their state and can be stepped alone) and frees the
+ * batch. */
+void m2p_batch_destroy(m2p_batch* b);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,62 @@
+// m2p_batch.h -- batched ensembles: many scenarios of uneven grids stepped by one set of
+// launches per time step (internal to the library).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "m2p_kernels.h"
+
+namespace m2p {
+
+// One scenario of a batch as the batched kernels see it: its grid, physics, buffers and
+// halting state, and where its blocks start in each batched launch.
+struct Member {
+  Grid g;
+  Phys ph;
+  DevParams dp;
+  int mode, nloc, nsamp, nbx;   // nbx: CFL box blocks per row
+  double phiS0;
+  void* U0;
+  void* U1;
+  void* ONE;
+  const void* MT;
+  const double* M;
+  unsigned char* LIM;
+  DevState* st;
+  const double* tf;
+  const double* itime;
+  const double* table;
+  const double* loc;
+  const int* dire;
+  int ghost0, box0, ring0, stage0;   // first block of this member in each launch
+};
+
+// Launch description of a batch: the member table and, per launch, the member of each
+// block (blocks of one member are contiguous).
+struct BatchDev {
+  const Member* mem;
+  const int* mapGhost;
+  const int* mapBox;
+  const int* mapRing;
+  const int* mapStage;
+  int nmem, nGhost, nBox, nRing, nStage;
+  int remap;   // stage blocks dealt to the 8 XCDs in contiguous bands
+};
+
+// Tile shapes and occupancy caps with a compiled batched stage kernel, per scalar type.
+struct StageVariant {
+  int ty, w;
+};
+int stage_variants(int prec, const StageVariant** out);
+
+constexpr int GHOST_LINES = 128;   // ghost-fill lines per block
+int ring_blocks(const Grid& g);
+int box_blocks_x(const Grid& g);
+int box_blocks_y(const Grid& g);
+
+// Enqueues `nsteps` time steps of every member (halted members return at once).
+template <typename T>
+hipError_t launch_batch_steps(const BatchDev& B, int ty, int w, bool any_inflow, int nsteps,
+                              hipStream_t s);
+
+}  // namespace m2p

@@ -1,0 +1,417 @@
+// m2p_batch.hip -- batched ensembles behind the C ABI (m2p_batch_*): scenarios of uneven
+// grids, each an m2p_ctx with its own buffers and DevState, stepped together by one set of
+// launches per time step, captured in a HIP graph, with the stage kernel's tile shape,
+// occupancy cap and block schedule picked by an autotuner that times the candidates on
+// the batch itself.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstdio>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/moses2pdf.h"
+#include "m2p_batch.h"
+#include "m2p_ctx.h"
+#include "m2p_tiles.h"
+
+namespace m2p {
+
+static const StageVariant kVar64[] = {{7, 4}, {3, 4}, {7, 3}};
+static const StageVariant kVar32[] = {{7, 6}, {7, 4}, {3, 6}, {15, 4}};
+
+int stage_variants(int prec, const StageVariant** out) {
+  if (prec == M2P_FP32) {
+    *out = kVar32;
+    return (int)(sizeof kVar32 / sizeof kVar32[0]);
+  }
+  *out = kVar64;
+  return (int)(sizeof kVar64 / sizeof kVar64[0]);
+}
+
+int ring_blocks(const Grid& g) { return (ring_threads(g) + 63) / 64; }
+int box_blocks_x(const Grid& g) { return (g.nx + 63) / 64; }
+int box_blocks_y(const Grid& g) { return (g.ny + 3) / 4; }
+
+}  // namespace m2p
+
+struct m2p_batch {
+  int prec = M2P_FP64;
+  hipStream_t s = nullptr;
+  std::vector<m2p_ctx*> mem;   // in order of addition
+  // schedule
+  int ty = 7, w = 4, remap = 1, order = 1, graph_steps = 16;
+  bool dirty = true;
+  // device tables
+  m2p::Member* dMem = nullptr;
+  size_t capMem = 0;
+  int* dMap = nullptr;
+  size_t capMap = 0;
+  m2p::BatchDev B{};
+  bool any_inflow = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_n = 0;
+  std::string err, report;
+};
+
+hipStream_t m2p_batch_stream(m2p_batch* b) { return b->s; }
+
+namespace {
+
+int bfail(m2p_batch* b, int code, const std::string& msg) {
+  if (b) b->err = msg;
+  return code;
+}
+
+#define BCHK(b, call)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return bfail((b), M2P_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+void drop_graph(m2p_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->graph_n = 0;
+}
+
+bool variant_ok(int prec, int ty, int w) {
+  const m2p::StageVariant* v;
+  const int n = m2p::stage_variants(prec, &v);
+  for (int i = 0; i < n; i++)
+    if (v[i].ty == ty && v[i].w == w) return true;
+  return false;
+}
+
+// Member table and block maps for the current members and schedule.
+int rebuild(m2p_batch* b) {
+  drop_graph(b);
+  if (!variant_ok(b->prec, b->ty, b->w))
+    return bfail(b, M2P_EINVAL, "no stage kernel compiled for ty=" + std::to_string(b->ty) +
+                                    " waves=" + std::to_string(b->w));
+  const int n = (int)b->mem.size();
+  std::vector<int> ord(n);
+  std::iota(ord.begin(), ord.end(), 0);
+  if (b->order == 1)  // largest grids first: the long tiles of big members start early
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](int a, int c) { return b->mem[a]->P > b->mem[c]->P; });
+  std::vector<m2p::Member> M(n);
+  std::vector<int> mg, mb, mr, ms;
+  b->any_inflow = false;
+  for (int k = 0; k < n; k++) {
+    m2p_ctx* c = b->mem[ord[k]];
+    m2p::Member& m = M[k];
+    m = m2p::Member{};
+    m.g = c->g;
+    m.ph = c->ph;
+    m.dp.cfl = c->par.cfl;
+    m.dp.dt_min = c->par.dt_min;
+    m.dp.dt_max = c->par.dt_max;
+    m.dp.ntf = 0;
+    m.dp.nsamp = c->mode == 1 ? c->nsamp : 0;
+    m.mode = c->mode;
+    m.nloc = c->nloc;
+    m.nsamp = c->nsamp;
+    m.nbx = m2p::box_blocks_x(c->g);
+    m.phiS0 = c->par.phiS0;
+    m.U0 = c->dU0;
+    m.U1 = c->dU1;
+    m.ONE = c->dONE;
+    m.MT = c->dMT;
+    m.M = c->dM;
+    m.LIM = c->dLIM;
+    m.st = c->dSt;
+    m.tf = c->dTf;
+    m.itime = c->dItime;
+    m.table = c->dTable;
+    m.loc = c->dLoc;
+    m.dire = c->dDire;
+    if (c->mode == 1) b->any_inflow = true;
+    m.ghost0 = (int)mg.size();
+    mg.insert(mg.end(), (c->nyd + c->nxd + m2p::GHOST_LINES - 1) / m2p::GHOST_LINES, k);
+    m.box0 = (int)mb.size();
+    mb.insert(mb.end(), (size_t)m.nbx * m2p::box_blocks_y(c->g), k);
+    m.ring0 = (int)mr.size();
+    mr.insert(mr.end(), m2p::ring_blocks(c->g), k);
+    m.stage0 = (int)ms.size();
+    ms.insert(ms.end(), m2p::stage_tiles(c->g, b->ty), k);
+  }
+  const size_t nmap = mg.size() + mb.size() + mr.size() + ms.size();
+  if (ms.size() > (size_t)INT_MAX / 2) return bfail(b, M2P_EINVAL, "batch too large");
+  if ((size_t)n > b->capMem) {
+    if (b->dMem) (void)hipFree(b->dMem);
+    b->dMem = nullptr;
+    b->capMem = std::max<size_t>(n, 2 * b->capMem);
+    BCHK(b, hipMalloc((void**)&b->dMem, b->capMem * sizeof(m2p::Member)));
+  }
+  if (nmap > b->capMap) {
+    if (b->dMap) (void)hipFree(b->dMap);
+    b->dMap = nullptr;
+    b->capMap = std::max(nmap, 2 * b->capMap);
+    BCHK(b, hipMalloc((void**)&b->dMap, b->capMap * sizeof(int)));
+  }
+  std::vector<int> all;
+  all.reserve(nmap);
+  for (auto* v : {&mg, &mb, &mr, &ms}) all.insert(all.end(), v->begin(), v->end());
+  if (n) BCHK(b, hipMemcpyAsync(b->dMem, M.data(), n * sizeof(m2p::Member), hipMemcpyHostToDevice, b->s));
+  if (nmap) BCHK(b, hipMemcpyAsync(b->dMap, all.data(), nmap * sizeof(int), hipMemcpyHostToDevice, b->s));
+  BCHK(b, hipStreamSynchronize(b->s));
+  m2p::BatchDev& B = b->B;
+  B.mem = b->dMem;
+  B.mapGhost = b->dMap;
+  B.mapBox = b->dMap + mg.size();
+  B.mapRing = b->dMap + mg.size() + mb.size();
+  B.mapStage = b->dMap + mg.size() + mb.size() + mr.size();
+  B.nmem = n;
+  B.nGhost = (int)mg.size();
+  B.nBox = (int)mb.size();
+  B.nRing = (int)mr.size();
+  B.nStage = (int)ms.size();
+  B.remap = b->remap;
+  b->dirty = false;
+  return 0;
+}
+
+hipError_t enqueue(m2p_batch* b, int nsteps) {
+  if (b->prec == M2P_FP32)
+    return m2p::launch_batch_steps<float>(b->B, b->ty, b->w, b->any_inflow, nsteps, b->s);
+  return m2p::launch_batch_steps<double>(b->B, b->ty, b->w, b->any_inflow, nsteps, b->s);
+}
+
+int ensure_graph(m2p_batch* b) {
+  if (b->exec && b->graph_n == b->graph_steps) return 0;
+  drop_graph(b);
+  BCHK(b, hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
+  const hipError_t e = enqueue(b, b->graph_steps);
+  hipGraph_t g = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(b->s, &g);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return bfail(b, M2P_EHIP, std::string("graph capture: ") +
+                                  hipGetErrorString(e != hipSuccess ? e : e2));
+  }
+  b->graph = g;
+  BCHK(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  b->graph_n = b->graph_steps;
+  return 0;
+}
+
+int step_batch(m2p_batch* b, int nsteps, bool use_graph) {
+  if (b->dirty)
+    if (int rc = rebuild(b)) return rc;
+  if (b->mem.empty() || nsteps <= 0) return 0;
+  int left = nsteps;
+  if (use_graph && b->graph_steps > 0 && left >= b->graph_steps) {
+    if (int rc = ensure_graph(b)) return rc;
+    for (; left >= b->graph_steps; left -= b->graph_steps) BCHK(b, hipGraphLaunch(b->exec, b->s));
+  }
+  if (left) BCHK(b, enqueue(b, left));
+  return 0;
+}
+
+// Saved member state for the autotuner's trial steps.
+struct Snapshot {
+  std::vector<void*> buf;
+};
+
+size_t snap_bytes(const m2p_ctx* c) {
+  return (16 * (size_t)c->P) * c->es + (c->dLIM ? (size_t)c->P : 0) + sizeof(DevState);
+}
+
+int save(m2p_batch* b, Snapshot& sn, bool restore) {
+  const hipMemcpyKind k = hipMemcpyDeviceToDevice;
+  for (size_t i = 0; i < b->mem.size(); i++) {
+    m2p_ctx* c = b->mem[i];
+    if (!restore) {
+      void* p = nullptr;
+      BCHK(b, hipMalloc(&p, snap_bytes(c)));
+      sn.buf.push_back(p);
+    }
+    char* p = (char*)sn.buf[i];
+    const size_t P = (size_t)c->P, es = c->es;
+    struct Part {
+      void* dev;
+      size_t n;
+    } parts[] = {{c->dU0, 7 * P * es}, {c->dU1, 7 * P * es}, {c->dONE, 2 * P * es},
+                 {c->dLIM, c->dLIM ? P : 0}, {c->dSt, sizeof(DevState)}};
+    for (auto& q : parts) {
+      if (q.n == 0) continue;
+      if (restore)
+        BCHK(b, hipMemcpyAsync(q.dev, p, q.n, k, b->s));
+      else
+        BCHK(b, hipMemcpyAsync(p, q.dev, q.n, k, b->s));
+      p += q.n;
+    }
+  }
+  BCHK(b, hipStreamSynchronize(b->s));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int m2p_batch_create(int precision, m2p_batch** out) {
+  if (!out || (precision != M2P_FP64 && precision != M2P_FP32)) return M2P_EINVAL;
+  m2p_batch* b = new m2p_batch();
+  b->prec = precision;
+  b->w = precision == M2P_FP32 ? 6 : 4;
+  const hipError_t e = hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking);
+  *out = b;
+  if (e != hipSuccess) return bfail(b, M2P_EHIP, hipGetErrorString(e));
+  return 0;
+}
+
+int m2p_batch_add(m2p_batch* b, m2p_ctx* c) {
+  if (!b || !b->s || !c || !c->s) return M2P_ESTATE;
+  if (c->batch) return bfail(b, M2P_ESTATE, "scenario is already in a batch");
+  if (c->prec != b->prec) return bfail(b, M2P_EINVAL, "scenario precision differs from the batch's");
+  // the scenario's own queued work must be done before the batch stream touches it
+  BCHK(b, hipStreamSynchronize(c->s));
+  c->batch = b;
+  b->mem.push_back(c);
+  b->dirty = true;
+  return (int)b->mem.size() - 1;
+}
+
+int m2p_batch_remove(m2p_batch* b, m2p_ctx* c) {
+  if (!b || !c) return M2P_ESTATE;
+  auto it = std::find(b->mem.begin(), b->mem.end(), c);
+  if (it == b->mem.end()) return bfail(b, M2P_EINVAL, "scenario is not in this batch");
+  BCHK(b, hipStreamSynchronize(b->s));
+  b->mem.erase(it);
+  c->batch = nullptr;
+  b->dirty = true;
+  return 0;
+}
+
+int m2p_batch_size(const m2p_batch* b) { return b ? (int)b->mem.size() : M2P_ESTATE; }
+
+int m2p_batch_step(m2p_batch* b, int nsteps) {
+  if (!b || !b->s) return M2P_ESTATE;
+  if (nsteps < 0) return bfail(b, M2P_EINVAL, "negative step count");
+  return step_batch(b, nsteps, true);
+}
+
+int m2p_batch_sync(m2p_batch* b) {
+  if (!b || !b->s) return M2P_ESTATE;
+  BCHK(b, hipStreamSynchronize(b->s));
+  return 0;
+}
+
+int m2p_batch_set(m2p_batch* b, const char* key, int value) {
+  if (!b || !key) return M2P_ESTATE;
+  const std::string k(key);
+  if (k == "ty" && value > 0) {
+    b->ty = value;   // the (ty, waves) pair is checked when the batch is next stepped
+  } else if (k == "waves" && value > 0) {
+    b->w = value;
+  } else if (k == "remap" && (value == 0 || value == 1)) {
+    b->remap = value;
+  } else if (k == "order" && (value == 0 || value == 1)) {
+    b->order = value;
+  } else if (k == "graph_steps" && value >= 0) {
+    b->graph_steps = value;
+  } else {
+    return bfail(b, M2P_EINVAL, "unknown key or bad value: " + k);
+  }
+  b->dirty = true;
+  return 0;
+}
+
+int m2p_batch_get(const m2p_batch* b, const char* key) {
+  if (!b || !key) return M2P_ESTATE;
+  const std::string k(key);
+  if (k == "ty") return b->ty;
+  if (k == "waves") return b->w;
+  if (k == "remap") return b->remap;
+  if (k == "order") return b->order;
+  if (k == "graph_steps") return b->graph_steps;
+  if (k == "stage_blocks") return b->B.nStage;
+  return M2P_EINVAL;
+}
+
+int m2p_batch_tune(m2p_batch* b, int trial_steps, char* report, int report_len) {
+  if (!b || !b->s) return M2P_ESTATE;
+  if (trial_steps < 1) return bfail(b, M2P_EINVAL, "trial_steps must be >= 1");
+  if (b->mem.empty()) return bfail(b, M2P_ESTATE, "empty batch");
+  Snapshot sn;
+  int rc = save(b, sn, false);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  struct Cand {
+    int ty, w, remap, order;
+    float ms;
+  };
+  std::vector<Cand> cands;
+  const m2p::StageVariant* v;
+  const int nv = m2p::stage_variants(b->prec, &v);
+  for (int i = 0; i < nv; i++)
+    for (int remap = 1; remap >= 0; remap--)
+      for (int order = 1; order >= 0; order--) cands.push_back({v[i].ty, v[i].w, remap, order, 0.f});
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = bfail(b, M2P_EHIP, "hipEventCreate");
+  std::string rep;
+  for (auto& cd : cands) {
+    if (rc) break;
+    b->ty = cd.ty;
+    b->w = cd.w;
+    b->remap = cd.remap;
+    b->order = cd.order;
+    b->dirty = true;
+    if ((rc = step_batch(b, 1, false))) break;  // warm-up
+    if (hipEventRecord(e0, b->s) != hipSuccess) rc = bfail(b, M2P_EHIP, "hipEventRecord");
+    if (!rc) rc = step_batch(b, trial_steps, false);
+    if (!rc && hipEventRecord(e1, b->s) != hipSuccess) rc = bfail(b, M2P_EHIP, "hipEventRecord");
+    if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = bfail(b, M2P_EHIP, "hipEventSynchronize");
+    if (!rc) (void)hipEventElapsedTime(&cd.ms, e0, e1);
+    cd.ms /= trial_steps;
+    if (!rc) rc = save(b, sn, true);
+    char line[160];
+    std::snprintf(line, sizeof line, "ty=%d waves=%d remap=%d order=%d: %.4f ms/step\n", cd.ty, cd.w,
+                  cd.remap, cd.order, cd.ms);
+    rep += line;
+  }
+  if (!rc) {
+    const Cand* best = &cands[0];
+    for (auto& cd : cands)
+      if (cd.ms < best->ms) best = &cd;
+    b->ty = best->ty;
+    b->w = best->w;
+    b->remap = best->remap;
+    b->order = best->order;
+    b->dirty = true;
+    char line[160];
+    std::snprintf(line, sizeof line, "chosen: ty=%d waves=%d remap=%d order=%d (%.4f ms/step)\n",
+                  best->ty, best->w, best->remap, best->order, best->ms);
+    rep += line;
+  }
+  for (void* p : sn.buf) (void)hipFree(p);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  b->report = rep;
+  if (report && report_len > 0) {
+    std::snprintf(report, (size_t)report_len, "%s", rep.c_str());
+  }
+  return rc;
+}
+
+const char* m2p_batch_last_error(const m2p_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+void m2p_batch_destroy(m2p_batch* b) {
+  if (!b) return;
+  if (b->s) (void)hipStreamSynchronize(b->s);
+  for (m2p_ctx* c : b->mem) c->batch = nullptr;
+  drop_graph(b);
+  if (b->dMem) (void)hipFree(b->dMem);
+  if (b->dMap) (void)hipFree(b->dMap);
+  if (b->s) (void)hipStreamDestroy(b->s);
+  delete b;
+}
+
+}  // extern "C"

@@ -1,0 +1,137 @@
+// m2p_batch_kernels.hip -- the batched per-step kernels: one launch per phase of the time
+// step covers every member of a batch.  A block finds its member through the launch's
+// block map, then runs the same device body as the single-scenario kernels (m2p_tiles.h),
+// so a member's results do not depend on the batch it is in, its position or the tile
+// shape (each face and cell is computed once, by the same expressions).
+// Compiled once per scalar type: M2P_BATCH_T is double or float.
+#include <hip/hip_runtime.h>
+
+#include "m2p_batch.h"
+#include "m2p_tiles.h"
+
+#ifndef M2P_BATCH_T
+#error "M2P_BATCH_T must be double or float"
+#endif
+
+namespace m2p {
+
+template <typename T>
+__global__ void kb_inflow(BatchDev B) {
+  const Member& m = B.mem[blockIdx.x];
+  if (threadIdx.x != 0 || m.mode != 1 || halted_pre(m.st)) return;
+  inflow_body(m.g, (T*)m.U0, m.M, m.table, m.loc, m.dire, m.nloc, m.nsamp, m.phiS0,
+              m.st->inflowCount, m.LIM);
+}
+
+// post = 0: fill U0 before the CFL pass (previous-stage source U1); post = 1: fill U1
+// before the corrector (source U0).
+template <typename T>
+__global__ void __launch_bounds__(GHOST_LINES) kb_ghost(BatchDev B, int post) {
+  const Member& m = B.mem[B.mapGhost[blockIdx.x]];
+  if (post ? halted_post(m.st) : halted_pre(m.st)) return;
+  const int line = (int)(blockIdx.x - m.ghost0) * GHOST_LINES + threadIdx.x;
+  if (line >= m.g.nyd + m.g.nxd) return;
+  T* U = (T*)(post ? m.U1 : m.U0);
+  const T* prev = (const T*)(post ? m.U0 : m.U1);
+  ghost_line(m.g, m.mode, U, prev, line, (int)blockIdx.y);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) kb_cfl(BatchDev B) {
+  const Member& m = B.mem[B.mapBox[blockIdx.x]];
+  if (halted_pre(m.st)) return;
+  const int lb = (int)blockIdx.x - m.box0;
+  block_max_to(cfl_box((const T*)m.U0, (const T*)m.MT, m.g, m.ph, lb % m.nbx, lb / m.nbx), m.st);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) kb_ring(BatchDev B) {
+  const Member& m = B.mem[B.mapRing[blockIdx.x]];
+  if (halted_pre(m.st)) return;
+  const int t = ((int)blockIdx.x - m.ring0) * 64 + threadIdx.x;
+  double v = cfl_ring_face((const T*)m.U0, (const T*)m.MT, m.g, m.ph, t);
+  v = wave_max(v);
+  if (threadIdx.x == 0) atomicMax(&m.st->maxbits, (unsigned long long)__double_as_longlong(v));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) kb_dt(BatchDev B) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  if (i >= B.nmem) return;
+  const Member& m = B.mem[i];
+  dt_body(m.dp, m.st, m.tf, m.itime);
+}
+
+template <int IO, typename T, int TY, int W>
+__global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per_eu(W, W)))
+    kb_stage(BatchDev B) {
+  __shared__ T F[ST_NQ][TY + 1][64];
+  const int b = blockIdx.x;
+  int t = b;
+  if (B.remap) {
+    const int per = (B.nStage + N_XCD - 1) / N_XCD;
+    t = (b % N_XCD) * per + b / N_XCD;
+  }
+  if (t >= B.nStage) return;
+  const Member& m = B.mem[B.mapStage[t]];
+  DevState* st = m.st;
+  if (halted_post(st)) return;
+  int tx, ty;
+  if (!tile_of(m.g, TY, t - m.stage0, tx, ty)) return;
+  const T* Uin = (const T*)(IO == 0 ? m.U0 : m.U1);
+  T* Uout = (T*)(IO == 0 ? m.U1 : m.U0);
+  const double v2 = stage_tile<IO, T, TY>(F, m.g, m.ph, Uin, (const T*)m.U0, Uout, (T*)m.ONE,
+                                          (const T*)m.MT, m.LIM,
+                                          IO == 0 ? st->dt_pred : st->dt_corr, tx, ty);
+  if (IO == 1 && st->vstop2 > 0.0) block_max_bits(v2, &st->vmaxbits);
+}
+
+template <typename T, int TY, int W>
+static void stage_pair(const BatchDev& B, int io, hipStream_t s) {
+  const dim3 blk(64, TY + 1);
+  const unsigned nb = B.remap ? (unsigned)(((B.nStage + N_XCD - 1) / N_XCD) * N_XCD) : (unsigned)B.nStage;
+  if (io == 0)
+    kb_stage<0, T, TY, W><<<nb, blk, 0, s>>>(B);
+  else
+    kb_stage<1, T, TY, W><<<nb, blk, 0, s>>>(B);
+}
+
+// The compiled (tile height, waves per SIMD) variants; the list in stage_variants() must
+// match.
+template <typename T>
+static bool launch_stage_variant(const BatchDev& B, int ty, int w, int io, hipStream_t s) {
+  if constexpr (std::is_same<T, double>::value) {
+    if (ty == 7 && w == 4) stage_pair<double, 7, 4>(B, io, s);
+    else if (ty == 3 && w == 4) stage_pair<double, 3, 4>(B, io, s);
+    else if (ty == 7 && w == 3) stage_pair<double, 7, 3>(B, io, s);
+    else return false;
+  } else {
+    if (ty == 7 && w == 6) stage_pair<float, 7, 6>(B, io, s);
+    else if (ty == 7 && w == 4) stage_pair<float, 7, 4>(B, io, s);
+    else if (ty == 3 && w == 6) stage_pair<float, 3, 6>(B, io, s);
+    else if (ty == 15 && w == 4) stage_pair<float, 15, 4>(B, io, s);
+    else return false;
+  }
+  return true;
+}
+
+template <typename T>
+hipError_t launch_batch_steps(const BatchDev& B, int ty, int w, bool any_inflow, int nsteps,
+                              hipStream_t s) {
+  if (B.nmem == 0 || nsteps <= 0) return hipSuccess;
+  for (int n = 0; n < nsteps; n++) {
+    if (any_inflow) kb_inflow<T><<<B.nmem, 64, 0, s>>>(B);
+    kb_ghost<T><<<dim3(B.nGhost, MN), GHOST_LINES, 0, s>>>(B, 0);
+    kb_cfl<T><<<B.nBox, dim3(64, 4), 0, s>>>(B);
+    kb_ring<T><<<B.nRing, 64, 0, s>>>(B);
+    kb_dt<T><<<(B.nmem + 63) / 64, 64, 0, s>>>(B);
+    if (!launch_stage_variant<T>(B, ty, w, 0, s)) return hipErrorInvalidValue;
+    kb_ghost<T><<<dim3(B.nGhost, MN), GHOST_LINES, 0, s>>>(B, 1);
+    if (!launch_stage_variant<T>(B, ty, w, 1, s)) return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template hipError_t launch_batch_steps<M2P_BATCH_T>(const BatchDev&, int, int, bool, int, hipStream_t);
+
+}  // namespace m2p

@@ -14,44 +14,9 @@
 #include <vector>
 
 #include "../../include/moses2pdf.h"
+#include "m2p_ctx.h"
 #include "m2p_io.h"
 #include "m2p_kernels.h"
-
-using m2p::DevParams;
-using m2p::DevState;
-
-struct m2p_ctx {
-  int mode = 0;
-  int NX = 0, NY = 0, nxd = 0, nyd = 0;
-  long long P = 0;
-  double dx = 0, dy = 0, xll = 0, yll = 0;
-  m2p_params par{};
-  std::vector<double> topo, depth, phi;  // padded, scaled (x10)
-  std::vector<double> tf;                // output times (scaled), tf[0] = 0
-  int nsamp = 0, nloc = 0;
-  std::vector<double> table, loc, itime;
-  std::vector<int> dire;
-  hipStream_t s = nullptr;
-  int prec = M2P_FP64;     // scalar type of the state and of the per-step metric copy
-  size_t es = 8;           // its size in bytes
-  double *dTopo = nullptr, *dDepth = nullptr, *dPhi = nullptr, *dM = nullptr;
-  // state planes u^n (U0), u^{n+1/2} (U1), the limited fluid momenta (ONE) and the metric
-  // the step kernels read (MT: dM itself in FP64, a float copy in FP32), in type `prec`
-  void *dU0 = nullptr, *dU1 = nullptr, *dONE = nullptr, *dMT = nullptr;
-  unsigned char* dLIM = nullptr;  // FP32 only: limiter memory flags, one byte per cell
-  double *dTf = nullptr, *dMass = nullptr;
-  double *dTable = nullptr, *dLoc = nullptr, *dItime = nullptr;
-  int* dDire = nullptr;
-  DevState* dSt = nullptr;
-  m2p::Grid g{};
-  m2p::Phys ph{};
-  std::string err;
-  bool prof = false;
-  std::map<std::string, std::pair<double, int>> pstats;
-  std::vector<hipEvent_t> evpool;
-  size_t evnext = 0;
-  std::vector<std::pair<const char*, std::pair<hipEvent_t, hipEvent_t>>> pending;
-};
 
 namespace {
 
@@ -61,6 +26,12 @@ int fail(m2p_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
   return code;
 }
+
+// The stream a scenario's work is ordered on: its own, or its batch's while in a batch.
+hipStream_t strm(m2p_ctx* c) { return c->batch ? m2p_batch_stream(c->batch) : c->s; }
+
+#define IN_BATCH(c)                                                                        \
+  if ((c)->batch) return fail((c), M2P_ESTATE, "scenario is stepped by a batch")
 
 #define HIPCHK(ctx, call)                                                                  \
   do {                                                                                     \
@@ -221,14 +192,15 @@ int setup(m2p_ctx* c) {
 // State transfers in the reference's FP64 layout, whatever the context's precision.
 int download_state(m2p_ctx* c, void* dsrc, size_t planes, double* u) {
   const size_t n = planes * (size_t)c->P;
+  hipStream_t s = strm(c);
   if (c->prec == M2P_FP64) {
-    HIPCHK(c, hipMemcpyAsync(u, dsrc, n * sizeof(double), hipMemcpyDeviceToHost, c->s));
-    HIPCHK(c, hipStreamSynchronize(c->s));
+    HIPCHK(c, hipMemcpyAsync(u, dsrc, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
     return 0;
   }
   std::vector<float> f(n);
-  HIPCHK(c, hipMemcpyAsync(f.data(), dsrc, n * sizeof(float), hipMemcpyDeviceToHost, c->s));
-  HIPCHK(c, hipStreamSynchronize(c->s));
+  HIPCHK(c, hipMemcpyAsync(f.data(), dsrc, n * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
   for (size_t i = 0; i < n; i++) u[i] = (double)f[i];
   return 0;
 }
@@ -534,6 +506,7 @@ int m2p_open(const char* par_list, int mode, const char* overlay, m2p_ctx** out)
 
 int m2p_step(m2p_ctx* c, int nsteps) {
   if (!c || !c->s) return M2P_ESTATE;
+  IN_BATCH(c);
   if (nsteps < 0) return fail(c, M2P_EINVAL, "negative step count");
   for (int n = 0; n < nsteps; n++) {
     enqueue_step(c, 0);
@@ -545,6 +518,7 @@ int m2p_step(m2p_ctx* c, int nsteps) {
 
 int m2p_run_to(m2p_ctx* c, double t_end, int max_steps, int* steps_done) {
   if (!c || !c->s) return M2P_ESTATE;
+  IN_BATCH(c);
   if (!(t_end > 0) || max_steps < 0) return fail(c, M2P_EINVAL, "t_end must be > 0, max_steps >= 0");
   DevState st{};
   HIPCHK(c, hipMemcpyAsync(&st, c->dSt, sizeof st, hipMemcpyDeviceToHost, c->s));
@@ -585,6 +559,7 @@ int m2p_sync(m2p_ctx* c) {
 
 int m2p_run(m2p_ctx* c, const char* result_dir, double* wall_s) {
   if (!c || !c->s || !result_dir) return M2P_ESTATE;
+  IN_BATCH(c);
   const auto t0 = std::chrono::steady_clock::now();
   const std::string dir(result_dir);
   const size_t P = (size_t)c->P;
@@ -640,8 +615,8 @@ int m2p_run(m2p_ctx* c, const char* result_dir, double* wall_s) {
 int m2p_get_state(m2p_ctx* c, double* u, double* t, int* nt) {
   if (!c || !c->s) return M2P_ESTATE;
   DevState st{};
-  HIPCHK(c, hipMemcpyAsync(&st, c->dSt, sizeof st, hipMemcpyDeviceToHost, c->s));
-  HIPCHK(c, hipStreamSynchronize(c->s));
+  HIPCHK(c, hipMemcpyAsync(&st, c->dSt, sizeof st, hipMemcpyDeviceToHost, strm(c)));
+  HIPCHK(c, hipStreamSynchronize(strm(c)));
   if (u)
     if (int rc = download_state(c, c->dU0, 7, u)) return rc;
   if (t) *t = st.time;
@@ -651,6 +626,7 @@ int m2p_get_state(m2p_ctx* c, double* u, double* t, int* nt) {
 
 int m2p_set_state(m2p_ctx* c, const double* u) {
   if (!c || !c->s) return M2P_ESTATE;
+  IN_BATCH(c);
   if (!u) return fail(c, M2P_EINVAL, "null state");
   if (int rc = upload_state(c, c->dU0, u)) return rc;
   HIPCHK(c, hipMemcpyAsync(c->dU1, c->dU0, 7 * (size_t)c->P * c->es, hipMemcpyDeviceToDevice, c->s));
@@ -672,13 +648,14 @@ int m2p_mass(m2p_ctx* c, double* solid, double* fluid) {
   if (!c || !c->s) return M2P_ESTATE;
   const int nb = m2p::mass_blocks();
   std::vector<double> part(4 * (size_t)nb);
+  hipStream_t s = strm(c);
   if (c->prec == M2P_FP32)
-    m2p::launch_mass(c->g, (const float*)c->dU0, c->dM, c->dMass, c->s);
+    m2p::launch_mass(c->g, (const float*)c->dU0, c->dM, c->dMass, s);
   else
-    m2p::launch_mass(c->g, (const double*)c->dU0, c->dM, c->dMass, c->s);
+    m2p::launch_mass(c->g, (const double*)c->dU0, c->dM, c->dMass, s);
   HIPCHK(c, hipGetLastError());
-  HIPCHK(c, hipMemcpyAsync(part.data(), c->dMass, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->s));
-  HIPCHK(c, hipStreamSynchronize(c->s));
+  HIPCHK(c, hipMemcpyAsync(part.data(), c->dMass, part.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
   // double-double sum of the block partials, in block order
   auto add = [](double& hi, double& lo, double x) {
     double s = hi + x, bb = s - hi, e = (hi - (s - bb)) + (x - bb);
@@ -701,8 +678,8 @@ int m2p_mass(m2p_ctx* c, double* solid, double* fluid) {
 int m2p_scalars(m2p_ctx* c, double* dt, double* dmax, int* iter) {
   if (!c || !c->s) return M2P_ESTATE;
   DevState st{};
-  HIPCHK(c, hipMemcpyAsync(&st, c->dSt, sizeof st, hipMemcpyDeviceToHost, c->s));
-  HIPCHK(c, hipStreamSynchronize(c->s));
+  HIPCHK(c, hipMemcpyAsync(&st, c->dSt, sizeof st, hipMemcpyDeviceToHost, strm(c)));
+  HIPCHK(c, hipStreamSynchronize(strm(c)));
   if (dt) *dt = st.dt_corr;
   if (dmax) *dmax = st.dmax;
   if (iter) *iter = st.iter;
@@ -728,7 +705,7 @@ int m2p_get_field(m2p_ctx* c, const char* name, double* out) {
   }
   auto it = tab.find(n);
   if (it == tab.end()) return fail(c, M2P_EINVAL, "unknown field " + n);
-  HIPCHK(c, hipStreamSynchronize(c->s));
+  HIPCHK(c, hipStreamSynchronize(strm(c)));
   HIPCHK(c, hipMemcpy(out, c->dM + (size_t)it->second.first * P,
                       (size_t)it->second.second * P * sizeof(double), hipMemcpyDeviceToHost));
   return it->second.second;
@@ -756,10 +733,57 @@ int m2p_profile_get(m2p_ctx* c, const char* kernel, double* ms, int* launches) {
   return 0;
 }
 
+int m2p_set_stop(m2p_ctx* c, double t_end, double vstop, int max_steps) {
+  if (!c || !c->s) return M2P_ESTATE;
+  hipStream_t s = strm(c);
+  DevState st{};
+  HIPCHK(c, hipMemcpyAsync(&st, c->dSt, sizeof st, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  st.t_end = t_end > 0 ? t_end : -1.0;
+  st.vstop2 = vstop > 0 ? vstop * vstop : 0.0;
+  st.halt_at = max_steps < 0 ? INT_MAX : (max_steps > INT_MAX - st.nt ? INT_MAX : st.nt + max_steps);
+  st.skip = 0;
+  st.moved = 0;
+  st.vmaxbits = 0;
+  HIPCHK(c, hipMemcpyAsync(c->dSt, &st, sizeof st, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  return 0;
+}
+
+int m2p_status(m2p_ctx* c, double* t, int* nt, int* halted) {
+  if (!c || !c->s) return M2P_ESTATE;
+  hipStream_t s = strm(c);
+  DevState st{};
+  HIPCHK(c, hipMemcpyAsync(&st, c->dSt, sizeof st, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  if (t) *t = st.time;
+  if (nt) *nt = st.nt;
+  if (halted) *halted = (st.skip || st.nt >= st.halt_at) ? 1 : 0;
+  return 0;
+}
+
+int m2p_footprint(int mode, int NX, int NY, int precision, int nsamp, int nloc,
+                  long long* steady, long long* peak) {
+  if (NX < 2 || NY < 2 || (precision != M2P_FP64 && precision != M2P_FP32)) return M2P_EINVAL;
+  const long long P = (long long)(NX + 2 * MD) * (NY + 2 * MD);
+  long long b = (long long)m2p::MC_COUNT * P * 8 + 4LL * m2p::mass_blocks() * 8 + (long long)sizeof(DevState) + 4096;
+  if (precision == M2P_FP64)
+    b += 16 * P * 8;
+  else
+    b += 16 * P * 4 + (long long)m2p::MC_COUNT * P * 4 + P;
+  if (mode == 1) b += (3LL * nsamp * nloc + 3LL * nloc + nsamp) * 8 + 3LL * nloc * 4;
+  // set-up temporaries: topography, depth, per-cell phi, and the FP64 initial state (FP32)
+  const long long tmp = 3 * P * 8 + (precision == M2P_FP32 ? 7 * P * 8 : 0);
+  if (steady) *steady = b;
+  if (peak) *peak = b + tmp;
+  return 0;
+}
+
 const char* m2p_last_error(const m2p_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
 void m2p_close(m2p_ctx* c) {
   if (!c) return;
+  if (c->batch) (void)m2p_batch_remove(c->batch, c);
   release(c);
   delete c;
 }

@@ -12,10 +12,9 @@
 // bodies are device functions so the batched-ensemble kernels reuse them.
 #include <hip/hip_runtime.h>
 
-#include <type_traits>
-
 #include "m2p_kernels.h"
 #include "m2p_numerics.h"
+#include "m2p_tiles.h"
 
 namespace m2p {
 
@@ -151,47 +150,8 @@ __global__ void k_jacob(Grid g, double* __restrict__ M) {
 }
 
 // ======================================================================================
-// per step
+// per step: single-scenario kernels
 // ======================================================================================
-
-// Inflow1Kernel + Inflow2Kernel (I/K:818-887).  In the reference every thread of the
-// grid repeats the whole loop; one thread does it here, in the same order (FP64
-// arithmetic, stored in the state's type).
-template <typename T>
-__device__ __forceinline__ void inflow_body(const Grid& g, T* __restrict__ U, const double* __restrict__ M,
-                                            const double* __restrict__ table, const double* __restrict__ loc,
-                                            const int* __restrict__ dire, int nloc, int nsamp,
-                                            double phiS0, int ic, unsigned char* __restrict__ LIM) {
-  const long long P = g.P;
-  for (int i = 0; i < nloc; i++) {
-    const int direX = dire[i * 3 + 1], direY = dire[i * 3 + 2];
-    const int xloc = (int)loc[i * 3 + 1], yloc = (int)loc[i * 3 + 2];
-    const double Hin = table[0 * (long long)nsamp * nloc + (long long)ic * nloc + i];
-    for (int mm = 0; mm < direX + MD + 3; mm++)
-      for (int nn = 0; nn < direY + MD + 3; nn++) {
-        const long long c = idx(g, yloc + nn, xloc + mm);
-        U[0 * P + c] = (T)(Hin * (1 - phiS0) / phiS0);
-        U[3 * P + c] = (T)(Hin * (1.0 - phiS0) * M[MC_C * P + c]);
-      }
-  }
-  for (int i = 0; i < nloc; i++) {
-    const int direX = dire[i * 3 + 1], direY = dire[i * 3 + 2];
-    const int xloc = (int)loc[i * 3 + 1], yloc = (int)loc[i * 3 + 2];
-    const double Uin = table[1 * (long long)nsamp * nloc + (long long)ic * nloc + i];
-    const double Vin = table[2 * (long long)nsamp * nloc + (long long)ic * nloc + i];
-    for (int mm = 0; mm < direX + MD + 3; mm++)
-      for (int nn = 0; nn < direY + MD + 3; nn++) {
-        const long long c = idx(g, yloc + nn, xloc + mm);
-        const double u0 = (double)U[0 * P + c], u3 = (double)U[3 * P + c];
-        U[1 * P + c] = (T)(Uin * (u0 + H0));
-        U[2 * P + c] = (T)(Vin * (u0 + H0));
-        U[4 * P + c] = (T)(Uin * (u3 + H0));
-        U[5 * P + c] = (T)(Vin * (u3 + H0));
-        U[6 * P + c] = (T)(u0 / (u0 + u3 + H0));
-        if (LIM) LIM[c] = 0;
-      }
-  }
-}
 
 template <typename T>
 __global__ void k_inflow(Grid g, T* __restrict__ U, const double* __restrict__ M,
@@ -202,98 +162,6 @@ __global__ void k_inflow(Grid g, T* __restrict__ U, const double* __restrict__ M
   inflow_body(g, U, M, table, loc, dire, nloc, nsamp, phiS0, st->inflowCount, LIM);
 }
 
-// Boundary5..8Kernel (L/K:853-956; inflow I/K:936-997) as one launch.  Thread (k, line):
-// lines [0, nyd) fill the x-ghosts of a row, lines [nyd, nyd+nxd) the y-ghosts of a
-// column.  Momenta (k = 1,2,4,5) of every ghost cell written are 0 (Boundary7/8).
-//   landslide: x-ghosts on every row; on the ghost rows the sources (columns 3,4 and
-//     nx+1,nx+2) are the previous stage's ghost-row values, held in `prev` (Boundary6
-//     skips the corner columns, so the reference's corners lag one stage).  y-ghosts on
-//     columns [3, nx+2] only.
-//   inflow: x-ghosts on rows [3, ny+2]; y-ghosts on every column, the corner columns from
-//     the x-ghost values of rows 3,4 / ny+1,ny+2 (Boundary6 covers every column).
-template <typename T>
-__device__ __forceinline__ void extrap3(T a, T b, T& g2, T& g1, T& g0) {
-  g2 = T(0.5) * (a + b);   // u[2] = (u[3]+u[4])/2
-  g1 = T(0.5) * (g2 + a);  // u[1] = (u[2]+u[3])/2
-  g0 = T(0.5) * (g1 + g2); // u[0] = (u[1]+u[2])/2
-}
-
-template <typename T>
-__device__ __forceinline__ void ghost_line(const Grid& g, int mode, T* __restrict__ Tg,
-                                           const T* __restrict__ prev, int line, int k) {
-  const long long P = g.P;
-  T* A = Tg + k * P;
-  const bool mom = (k == 1 || k == 2 || k == 4 || k == 5);
-  const int nx = g.nx, ny = g.ny;
-  if (line < g.nyd) {
-    const int r = line;
-    const bool ghost_row = (r < MD || r >= ny + MD);
-    if (ghost_row && mode == 1) return;
-    const T* src = (ghost_row ? prev + k * P : A) + (long long)r * g.nxd;
-    T* row = A + (long long)r * g.nxd;
-    if (mom) {
-      row[0] = row[1] = row[2] = T(0);
-      row[nx + MD] = row[nx + MD + 1] = row[nx + MD + 2] = T(0);
-      return;
-    }
-    T g2, g1, g0, h3, h4, h5;
-    extrap3(src[MD], src[MD + 1], g2, g1, g0);
-    extrap3(src[nx + MD - 1], src[nx + MD - 2], h3, h4, h5);
-    row[2] = g2;
-    row[1] = g1;
-    row[0] = g0;
-    row[nx + MD] = h3;
-    row[nx + MD + 1] = h4;
-    row[nx + MD + 2] = h5;
-  } else {
-    const int c = line - g.nyd;
-    const bool ghost_col = (c < MD || c >= nx + MD);
-    if (ghost_col && mode == 0) return;
-    const long long n = g.nxd;
-    T* col = A + c;
-    if (mom) {
-      col[0] = col[n] = col[2 * n] = T(0);
-      col[(ny + MD) * n] = col[(ny + MD + 1) * n] = col[(ny + MD + 2) * n] = T(0);
-      return;
-    }
-    T a3, a4, b2, b1;
-    if (!ghost_col) {
-      a3 = col[MD * n];
-      a4 = col[(MD + 1) * n];
-      b2 = col[(ny + MD - 1) * n];
-      b1 = col[(ny + MD - 2) * n];
-    } else {
-      // x-ghost value of column c on rows 3, 4, ny+2, ny+1 (what Boundary5 just wrote)
-      T v[4];
-      const int rows[4] = {MD, MD + 1, ny + MD - 1, ny + MD - 2};
-      for (int q = 0; q < 4; q++) {
-        const T* rr = A + (long long)rows[q] * n;
-        T e2, e1, e0;
-        if (c < MD) {
-          extrap3(rr[MD], rr[MD + 1], e2, e1, e0);
-          v[q] = (c == 2) ? e2 : (c == 1) ? e1 : e0;
-        } else {
-          extrap3(rr[nx + MD - 1], rr[nx + MD - 2], e2, e1, e0);
-          v[q] = (c == nx + MD) ? e2 : (c == nx + MD + 1) ? e1 : e0;
-        }
-      }
-      a3 = v[0];
-      a4 = v[1];
-      b2 = v[2];
-      b1 = v[3];
-    }
-    T g2, g1, g0, h3, h4, h5;
-    extrap3(a3, a4, g2, g1, g0);
-    extrap3(b2, b1, h3, h4, h5);
-    col[2 * n] = g2;
-    col[1 * n] = g1;
-    col[0] = g0;
-    col[(ny + MD) * n] = h3;
-    col[(ny + MD + 1) * n] = h4;
-    col[(ny + MD + 2) * n] = h5;
-  }
-}
-
 template <typename T>
 __global__ void k_ghost(Grid g, int mode, T* __restrict__ Tg, const T* __restrict__ prev,
                         const DevState* __restrict__ st, int post) {
@@ -302,98 +170,11 @@ __global__ void k_ghost(Grid g, int mode, T* __restrict__ Tg, const T* __restric
   ghost_line(g, mode, Tg, prev, line, (int)blockIdx.y);
 }
 
-// Max of one wave (64 lanes) with fmax semantics (NaN ignored, as fmaxf in reduceKernel).
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-  return v;
-}
-
-// Flux1..6Kernel + CFL1/2Kernel + reduceKernel (L/K:1426-1848, 1938-2004) for the dt of
-// the predictor: the max over faces [1, N-3]^2 of em/dx, em/dy (solid and fluid).  Faces
-// outside that square have a zero face-averaged metric in the reference, so their em is
-// the 1e-10 floor, which never exceeds these.  Result: atomicMax on the IEEE bits of a
-// positive double.  k_cfl covers the faces of the interior box [3, ny+2] x [3, nx+2]
-// (InterfacesKernel states); k_cfl_ring the rest of [1, N-3]^2, where Interfaces2Kernel
-// overrides the states.  Speeds are formed in T; the quotient by dx and the max in FP64.
-template <typename T>
-__device__ __forceinline__ double face_cfl(const T* __restrict__ U, const T* __restrict__ M,
-                                           const Grid& g, const Phys& ph, int dir, int r, int c,
-                                           bool edge) {
-  T L[6], R[6];
-  if (edge)
-    final_states_edge(U, g, dir, r, c, L, R);
-  else
-    base_states(U, g, dir, r, c, L, R);
-  FaceMet<T> f;
-  face_metric(M, g, dir, r, c, f);
-  FaceSpeed<T> s;
-  face_speed(dir, L, R, f, ph, s);
-  const double d = dir == 0 ? g.dx : g.dy;
-  return fmax((double)s.emS / d, (double)s.emF / d);
-}
-
-__device__ __forceinline__ void block_max_to(double m, DevState* st) {
-  m = wave_max(m);
-  __shared__ double part[16];
-  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-  const int nw = (blockDim.x * blockDim.y + 63) / 64;
-  if ((tid & 63) == 0) part[tid >> 6] = m;
-  __syncthreads();
-  if (tid == 0) {
-    double b = part[0];
-    for (int w = 1; w < nw; w++) b = fmax(b, part[w]);
-    atomicMax(&st->maxbits, (unsigned long long)__double_as_longlong(b));
-  }
-}
-
-// one 64x4 block of interior faces (both directions) starting at box column bx*64, row by*4
-template <typename T>
-__device__ __forceinline__ double cfl_box(const T* __restrict__ U, const T* __restrict__ M,
-                                          const Grid& g, const Phys& ph, int bx, int by) {
-  const int c = bx * 64 + threadIdx.x + MD;
-  const int r = by * 4 + threadIdx.y + MD;
-  double m = 0.0;
-  if (r <= g.ny + MD - 1 && c <= g.nx + MD - 1) {
-    m = face_cfl(U, M, g, ph, 0, r, c, false);
-    m = fmax(m, face_cfl(U, M, g, ph, 1, r, c, false));
-  }
-  return m;
-}
-
 template <typename T>
 __global__ void __launch_bounds__(256) k_cfl(Grid g, Phys ph, const T* __restrict__ U,
                                              const T* __restrict__ M, DevState* __restrict__ st) {
   if (halted_pre(st)) return;
   block_max_to(cfl_box(U, M, g, ph, blockIdx.x, blockIdx.y), st);
-}
-
-// The ring [1, N-3]^2 minus the interior box, one thread per (face index, direction).
-// Face indices are enumerated as 4 strips: rows 1-2 and ny+3..ny+4 (columns 1..N-3), then
-// columns 1-2 and nx+3..nx+4 on rows 3..ny+2.
-__host__ __device__ __forceinline__ int ring_threads(const Grid& g) {
-  return 2 * (4 * (g.nxd - 3) + 4 * g.ny);
-}
-
-template <typename T>
-__device__ __forceinline__ double cfl_ring_face(const T* __restrict__ U, const T* __restrict__ M,
-                                                const Grid& g, const Phys& ph, int t) {
-  const int f = t >> 1, dir = t & 1;
-  const int w = g.nxd - 3;        // columns 1..N-3
-  const int nrow = 4 * w;         // 4 ghost face rows
-  const int ncol = 4 * g.ny;      // 4 ghost face columns on interior rows
-  if (f >= nrow + ncol) return 0.0;
-  int r, c;
-  if (f < nrow) {
-    const int k = f / w;
-    r = (k < 2) ? 1 + k : g.ny + MD + (k - 2);
-    c = 1 + f % w;
-  } else {
-    const int q = f - nrow, k = q / g.ny;
-    r = MD + q % g.ny;
-    c = (k < 2) ? 1 + k : g.nx + MD + (k - 2);
-  }
-  return face_cfl(U, M, g, ph, dir, r, c, true);
 }
 
 template <typename T>
@@ -405,365 +186,31 @@ __global__ void __launch_bounds__(64) k_cfl_ring(Grid g, Phys ph, const T* __res
   if (threadIdx.x == 0) atomicMax(&st->maxbits, (unsigned long long)__double_as_longlong(m));
 }
 
-// CFL3Kernel + the host's dt control + CFL4Kernel (L/K:2006-2036, L/R:978-991; inflow
-// sample advance I/R:1078-1081), on the device.  The max is rounded to float as the
-// reference's fmaxf reduction does (L/K:1988, 1996).  Also the halting rules, which have
-// no reference counterpart: a step budget (halt_at), an end time (t_end: the step that
-// reaches it is shortened to end on it) and a stop speed (vstop2: once the squared max
-// flow speed of a corrector has reached it and later falls below it, the next step is the
-// last).
-__device__ __forceinline__ void dt_body(const DevParams& dp, DevState* __restrict__ st,
-                                        const double* __restrict__ tf,
-                                        const double* __restrict__ itime) {
-  if (st->skip) return;
-  if (st->nt >= st->halt_at) {
-    st->skip = 1;
-    return;
-  }
-  if (st->vstop2 > 0.0 && st->nt > 0) {
-    const double v2 = __longlong_as_double((long long)st->vmaxbits);
-    if (v2 >= st->vstop2)
-      st->moved = 1;
-    else if (st->moved && st->halt_at > st->nt + 1)
-      st->halt_at = st->nt + 1;
-  }
-  st->vmaxbits = 0ull;
-  const double mx = __longlong_as_double((long long)st->maxbits);
-  const double dmax = (double)(float)mx;
-  double dtval = dp.cfl / dmax;
-  dtval = fmin(dp.dt_max, dtval);
-  dtval = fmax(dp.dt_min, dtval);
-  const double T = st->time;
-  st->step_start = T;
-  st->dmax = dmax;
-  if (st->iter < dp.ntf && (T + dtval) >= tf[st->iter]) {
-    dtval = tf[st->iter] - T;
-    st->schreiben = 1;
-    st->iter = st->iter + 1;
-  }
-  if (dp.nsamp > 0 && T >= itime[st->inflowCount] && st->inflowCount < (dp.nsamp - 1))
-    st->inflowCount = st->inflowCount + 1;
-  if (st->t_end > 0.0 && (T + dtval) >= st->t_end) {
-    dtval = st->t_end - T;
-    if (st->halt_at > st->nt + 1) st->halt_at = st->nt + 1;
-  }
-  st->dtval = dtval;
-  st->dt_pred = dtval / (2.0 - 0);
-  st->dt_corr = dtval / (2.0 - 1);
-  st->time = st->dt_corr + T;
-  st->maxbits = 0ull;
-  st->nt = st->nt + 1;
-}
-
 __global__ void k_dt(DevParams dp, DevState* __restrict__ st, const double* __restrict__ tf,
                      const double* __restrict__ itime) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   dt_body(dp, st, tf, itime);
 }
 
-// Slope-carrying cell values for the viscous term: u3, u4, u5 and the limited slopes of
-// u4, u5 along direction `tdir` (0: x, 1: y) at cell i.
-template <typename T>
-__device__ __forceinline__ PDSide<T> pd_side(const T* __restrict__ U, const Grid& g, long long i,
-                                             int tdir) {
-  const long long P = g.P, st = tdir == 0 ? 1 : g.nxd;
-  PDSide<T> p;
-  p.u3 = U[3 * P + i];
-  p.u4 = U[4 * P + i];
-  p.u5 = U[5 * P + i];
-  p.t4 = tvd_slope(U[4 * P + i - st], p.u4, U[4 * P + i + st]);
-  p.t5 = tvd_slope(U[5 * P + i - st], p.u5, U[5 * P + i + st]);
-  return p;
-}
+// Tile height and the waves-per-SIMD cap of the single-scenario stage kernel: FP64 is held
+// to 128 VGPRs (4 waves); the FP32 body needs ~80 VGPRs and runs at 6.
+template <typename T> struct PlainStage;
+template <> struct PlainStage<double> { static constexpr int TY = 7, W = 4; };
+template <> struct PlainStage<float> { static constexpr int TY = 7, W = 6; };
 
-// One stage (predictor IO = 0, corrector IO = 1): Flux8..21Kernel (L/K:2038-2726) and the
-// crossing limiter, Flux22..24 / Flux25..28 (L/K:2730-2937).
-//   IO = 0: Uin = U0 = u^n (also uzero).  Writes u^{n+1/2} into Uout = U1 and the limited
-//           fluid momenta uone[4], uone[5] into ONE.
-//   IO = 1: Uin = U1, uzero = U0, old velocities from (U1[0..3], ONE).  Writes u^{n+1}
-//           into U0 in place (each thread reads and writes only its own cell of U0).
-// A block of 8 waves owns a tile of TX = 63 columns x TY = 7 rows of cells.  Each face is
-// computed once: the 64 x-faces of a tile row by one wave, then (through LDS) the 64
-// y-faces of a face row by one wave; a cell reads its two faces of each direction from
-// LDS.  The update keeps the reference's left-to-right sum, so the x terms are formed
-// first and held in registers while the y faces are computed.
-#ifndef M2P_ST_TY
-#define M2P_ST_TY 7
-#endif
-constexpr int ST_TX = 63, ST_TY = M2P_ST_TY, ST_NQ = 12;
-
-// Phase A: InterfacesKernel states of one face into the tile (L: q 0-5, R: q 6-11).
-template <typename T>
-__device__ __forceinline__ void stage_states(T (*Tl)[ST_TY + 1][64], int w, int lane,
-                                             const T* __restrict__ U, const Grid& g, int dir, int r,
-                                             int c) {
-  T L[6], R[6];
-  base_states(U, g, dir, r, c, L, R);
-#pragma unroll
-  for (int k = 0; k < 6; k++) {
-    Tl[k][w][lane] = L[k];
-    Tl[6 + k][w][lane] = R[k];
-  }
-}
-
-// Phase B: the face's final states.  The only Interfaces2Kernel overrides a stage needs are
-// the x-face at column 2 (from the faces at columns 3, 4: lanes +1, +2) and the y-face at
-// row 2 (from rows 3, 4: tile rows +1, +2).
-template <typename T>
-__device__ __forceinline__ void load_states(T (*Tl)[ST_TY + 1][64], int w, int lane, bool ovr,
-                                            int dir, T L[6], T R[6]) {
-#pragma unroll
-  for (int k = 0; k < 6; k++) {
-    if (!ovr) {
-      L[k] = Tl[k][w][lane];
-      R[k] = Tl[6 + k][w][lane];
-    } else if (dir == 0) {
-      L[k] = T(0.5) * (Tl[k][w][lane + 1] + Tl[k][w][lane + 2]);
-      R[k] = T(0.5) * (Tl[6 + k][w][lane + 1] + Tl[6 + k][w][lane + 2]);
-    } else {
-      L[k] = T(0.5) * (Tl[k][w + 1][lane] + Tl[k][w + 2][lane]);
-      R[k] = T(0.5) * (Tl[6 + k][w + 1][lane] + Tl[6 + k][w + 2][lane]);
-    }
-  }
-}
-
-// Phase C: fluxes of one face from its final states, into the tile.
-template <typename T>
-__device__ __forceinline__ void flux_store(T (*Tl)[ST_TY + 1][64], int w, int lane,
-                                           const T* __restrict__ U, const T* __restrict__ M,
-                                           const Grid& g, const Phys& ph, int dir, int r, int c,
-                                           T L[6], T R[6]) {
-  FaceMet<T> f;
-  face_metric(M, g, dir, r, c, f);
-  FaceSpeed<T> sp;
-  face_speed(dir, L, R, f, ph, sp);
-  const long long i = idx(g, r, c);
-  const long long j = i + (dir == 0 ? 1 : g.nxd);
-  const PDSide<T> pl = pd_side(U, g, i, 1 - dir);
-  const PDSide<T> pr = pd_side(U, g, j, 1 - dir);
-  FaceFlux<T> o;
-  face_flux(dir, L, R, f, sp, pl, pr, g, ph, o);
-#pragma unroll
-  for (int k = 0; k < 6; k++) Tl[k][w][lane] = o.H[k];
-  Tl[6][w][lane] = o.P1;
-  Tl[7][w][lane] = o.P2;
-  Tl[8][w][lane] = o.P4;
-  Tl[9][w][lane] = o.P5;
-  Tl[10][w][lane] = o.D4;
-  Tl[11][w][lane] = o.D5;
-}
-
-// Returns, for IO = 1, the larger squared flow speed (solid, fluid) of the cell's new state
-// if its flow depth exceeds WET_SPEED (0 otherwise, and for IO = 0).
-constexpr double WET_SPEED = 1.0e-2;  // scaled depth (1 mm)
-
-template <int IO, typename T>
-__device__ __forceinline__ double stage_tile(T (*F)[ST_TY + 1][64], const Grid& g, const Phys& ph,
-                                           const T* __restrict__ Uin, const T* Uzero, T* Uout,
-                                           T* __restrict__ ONE, const T* __restrict__ M,
-                                           unsigned char* __restrict__ LIM, double dt_d, int tx,
-                                           int ty) {
-  const int lane = threadIdx.x;
-  const int w = threadIdx.y;
-  const int c0 = tx * ST_TX + MD;
-  const int r0 = ty * ST_TY + MD;
-  const int xmax = g.nx + MD - 1, ymax = g.ny + MD - 1;
-  const long long P = g.P;
-
-  // x faces (r0+w, c0-1+lane), faces (r, c)-(r, c+1)
-  {
-    const int r = r0 + w, c = c0 - 1 + lane;
-    const bool has = (w < ST_TY && r <= ymax && c <= xmax);
-    if (has) stage_states(F, w, lane, Uin, g, 0, r, c);
-    __syncthreads();
-    T L[6], R[6];
-    if (has) load_states(F, w, lane, c == 2, 0, L, R);
-    __syncthreads();
-    if (has) flux_store(F, w, lane, Uin, M, g, ph, 0, r, c, L, R);
-  }
-  __syncthreads();
-
-  const int r = r0 + w, c = c0 + lane;
-  const bool active = (w < ST_TY) && (lane < ST_TX) && r <= ymax && c <= xmax;
-  const long long i = idx(g, r, c);
-  const T dt = T(dt_d);
-  const T dtdx = qdiv(dt, T(g.dx)), dtdy = qdiv(dt, T(g.dy));
-  const T epsilon0 = T(1.0);
-  T phi = T(0), fP = T(0), fD = T(0);
-  T tH[6], tP[4], tD[2];
-  if (active) {
-    phi = Uin[6 * P + i];
-    fP = epsilon0 * phi;
-    fD = qdiv(epsilon0 * (T(1.0) - phi), T(ph.N_R));
-#pragma unroll
-    for (int k = 0; k < 6; k++) tH[k] = (dtdx) * (F[k][w][lane + 1] - F[k][w][lane]);
-#pragma unroll
-    for (int q = 0; q < 4; q++) tP[q] = (dtdx) * (F[6 + q][w][lane + 1] - F[6 + q][w][lane]) * (fP);
-#pragma unroll
-    for (int q = 0; q < 2; q++) tD[q] = (dtdx) * (F[10 + q][w][lane + 1] - F[10 + q][w][lane]) * (fD);
-  }
-  __syncthreads();
-
-  // y faces (r0-1+w, c0+lane), faces (r, c)-(r+1, c)
-  {
-    const int rf = r0 - 1 + w, cf = c0 + lane;
-    const bool has = (lane < ST_TX && rf <= ymax && cf <= xmax);
-    if (has) stage_states(F, w, lane, Uin, g, 1, rf, cf);
-    __syncthreads();
-    T L[6], R[6];
-    if (has) load_states(F, w, lane, rf == 2, 1, L, R);
-    __syncthreads();
-    if (has) flux_store(F, w, lane, Uin, M, g, ph, 1, rf, cf, L, R);
-  }
-  __syncthreads();
-  if (!active) return 0.0;
-
-  T u[7];
-#pragma unroll
-  for (int k = 0; k < 7; k++) u[k] = Uin[k * P + i];
-  CellMet<T> cm;
-  cell_metric(M, P, i, cm);
-  T s[6];
-  cell_source(u, cm, ph, s);
-  T uz[6];
-#pragma unroll
-  for (int k = 0; k < 6; k++) uz[k] = Uzero[k * P + i];
-
-  // Flux18Kernel L/K:2641-2652 (Pp and PD are stored zeros for the components they skip)
-  T vnew[6];
-#pragma unroll
-  for (int k = 0; k < 6; k++) {
-    const int q = k == 1 ? 0 : k == 2 ? 1 : k == 4 ? 2 : 3;  // P1, P2, P4, P5
-    const bool hasP = (k == 1 || k == 2 || k == 4 || k == 5);
-    const bool hasD = (k == 4 || k == 5);
-    const T zero = T(0.0);
-    const T yH = (dtdy) * (F[k][w + 1][lane] - F[k][w][lane]);
-    const T xP = hasP ? tP[q] : (dtdx) * (zero - zero) * (fP);
-    const T yP = hasP ? (dtdy) * (F[6 + q][w + 1][lane] - F[6 + q][w][lane]) * (fP)
-                      : (dtdy) * (zero - zero) * (fP);
-    const T xD = hasD ? tD[k - 4] : (dtdx) * (zero - zero) * (fD);
-    const T yD = hasD ? (dtdy) * (F[10 + (k - 4)][w + 1][lane] - F[10 + (k - 4)][w][lane]) * (fD)
-                      : (dtdy) * (zero - zero) * (fD);
-    vnew[k] = uz[k] * cm.det - tH[k] - yH + xP + yP + xD + yD + (dt)*s[k];
-  }
-  const T h0 = T(H0);
-  T un[7];
-#pragma unroll
-  for (int k = 0; k < 6; k++) un[k] = qdiv(vnew[k], cm.det);  // Flux19
-  if (un[0] <= T(H00)) {                                     // Flux20
-    un[0] = T(0);
-    un[1] = T(0);
-    un[2] = T(0);
-  }
-  if (un[3] <= T(H00)) {
-    un[3] = T(0);
-    un[4] = T(0);
-    un[5] = T(0);
-  }
-  un[6] = qdiv(un[0], un[0] + un[3] + h0);  // Flux21
-
-  // velocity-crossing limiter (Flux23/24, Flux26/27)
-  T o0, o1, o2, o3, o4, o5;
-  if (IO == 0) {
-    o0 = uz[0]; o1 = uz[1]; o2 = uz[2]; o3 = uz[3]; o4 = uz[4]; o5 = uz[5];
-  } else {
-    o0 = u[0]; o1 = u[1]; o2 = u[2]; o3 = u[3]; o4 = ONE[i]; o5 = ONE[P + i];
-  }
-  const T usxnew = qdiv(un[1], un[0] + h0);
-  const T ufxnew = qdiv(un[4], un[3] + h0);
-  const T usxold = qdiv(o1, o0 + h0);
-  const T ufxold = qdiv(o4, o3 + h0);
-  const T usynew = qdiv(un[2], un[0] + h0);
-  const T ufynew = qdiv(un[5], un[3] + h0);
-  const T usyold = qdiv(o2, o0 + h0);
-  const T ufyold = qdiv(o5, o3 + h0);
-  T lim4, lim5;
-  if constexpr (std::is_same<T, double>::value) {
-    lim4 = ((usxnew - ufxnew) * (usxold - ufxold) >= T(0.0)) ? un[4] : usxnew * un[3];
-    lim5 = ((usynew - ufynew) * (usyold - ufyold) >= T(0.0)) ? un[5] : usynew * un[3];
-  } else {
-    // Limiter memory (FP32 only).  When a stage limits, the stored fluid momentum is
-    // us*hf, and in FP64 the next stage's old relative velocity is us*H0/(hf+H0): about
-    // 1e-8 of us, with the sign of us.  Float storage rounds that to noise, so the flag
-    // of the stored state says to take the sign of us instead.  Bits 0-1: flags of the
-    // stage-1 output (U0, read by stage 0); bits 2-3: flags of ONE (read by stage 1).
-    const unsigned f = IO == 0 ? (LIM[i] & 3u) : ((LIM[i] >> 2) & 3u);
-    const T dxo = (f & 1u) ? usxold : (usxold - ufxold);
-    const T dyo = (f & 2u) ? usyold : (usyold - ufyold);
-    const bool kx = (usxnew - ufxnew) * dxo >= T(0.0);
-    const bool ky = (usynew - ufynew) * dyo >= T(0.0);
-    lim4 = kx ? un[4] : usxnew * un[3];
-    lim5 = ky ? un[5] : usynew * un[3];
-    const unsigned nb = (kx ? 0u : 1u) | (ky ? 0u : 2u);
-    LIM[i] = (unsigned char)(IO == 0 ? ((f & 3u) | (nb << 2)) : nb);
-  }
-  if (IO == 0) {
-#pragma unroll
-    for (int k = 0; k < 7; k++) Uout[k * P + i] = un[k];
-    ONE[i] = lim4;
-    ONE[P + i] = lim5;
-    return 0.0;
-  } else {
-    un[4] = lim4;
-    un[5] = lim5;
-#pragma unroll
-    for (int k = 0; k < 7; k++) Uout[k * P + i] = un[k];
-    if ((double)un[0] + (double)un[3] <= WET_SPEED) return 0.0;
-    const double ufx = (double)qdiv(lim4, un[3] + h0), ufy = (double)qdiv(lim5, un[3] + h0);
-    const double sx = (double)usxnew, sy = (double)usynew;
-    return fmax(sx * sx + sy * sy, ufx * ufx + ufy * ufy);
-  }
-}
-
-// Max over the block of a per-thread value into a DevState field (atomicMax on the IEEE
-// bits of a non-negative double).  Every thread of the block must call it.
-__device__ __forceinline__ void block_max_bits(double m, unsigned long long* dst) {
-  m = wave_max(m);
-  __shared__ double part[16];
-  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-  const int nw = (blockDim.x * blockDim.y + 63) / 64;
-  if ((tid & 63) == 0) part[tid >> 6] = m;
-  __syncthreads();
-  if (tid == 0) {
-    double b = part[0];
-    for (int w = 1; w < nw; w++) b = fmax(b, part[w]);
-    if (b > 0.0) atomicMax(dst, (unsigned long long)__double_as_longlong(b));
-  }
-}
-
-// Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).  Block b runs on
-// XCD b % 8; it is given tile number (b % 8) * per + b / 8 in row-major tile order, so
-// each XCD sweeps a contiguous band of tile rows and vertically adjacent tiles, which
-// share halo rows, meet in the same L2.
-constexpr int N_XCD = 8;
-__host__ __device__ __forceinline__ int stage_tiles(const Grid& g) {
-  return ((g.nx + ST_TX - 1) / ST_TX) * ((g.ny + ST_TY - 1) / ST_TY);
-}
-__device__ __forceinline__ bool tile_of(const Grid& g, int t, int& tx, int& ty) {
-  const int ntx = (g.nx + ST_TX - 1) / ST_TX;
-  if (t >= stage_tiles(g)) return false;
-  tx = t % ntx;
-  ty = t / ntx;
-  return true;
-}
-
-#ifndef M2P_STAGE_ATTR
-#define M2P_STAGE_ATTR __attribute__((amdgpu_waves_per_eu(4, 4)))
-#endif
-
-template <int IO, typename T>
-__global__ void __launch_bounds__(64 * (ST_TY + 1)) M2P_STAGE_ATTR
+template <int IO, typename T, int TY, int W>
+__global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per_eu(W, W)))
     k_stage(Grid g, Phys ph, const T* __restrict__ Uin, const T* Uzero, T* Uout,
             T* __restrict__ ONE, const T* __restrict__ M, unsigned char* __restrict__ LIM,
             DevState* __restrict__ st) {
-  __shared__ T F[ST_NQ][ST_TY + 1][64];
+  __shared__ T F[ST_NQ][TY + 1][64];
   if (halted_post(st)) return;
   const int b = blockIdx.x;
-  const int per = (stage_tiles(g) + N_XCD - 1) / N_XCD;
+  const int per = (stage_tiles(g, TY) + N_XCD - 1) / N_XCD;
   int tx, ty;
-  if (!tile_of(g, (b % N_XCD) * per + b / N_XCD, tx, ty)) return;
-  const double v2 = stage_tile<IO, T>(F, g, ph, Uin, Uzero, Uout, ONE, M, LIM,
-                                      IO == 0 ? st->dt_pred : st->dt_corr, tx, ty);
+  if (!tile_of(g, TY, (b % N_XCD) * per + b / N_XCD, tx, ty)) return;
+  const double v2 = stage_tile<IO, T, TY>(F, g, ph, Uin, Uzero, Uout, ONE, M, LIM,
+                                          IO == 0 ? st->dt_pred : st->dt_corr, tx, ty);
   if (IO == 1 && st->vstop2 > 0.0) block_max_bits(v2, &st->vmaxbits);
 }
 
@@ -880,12 +327,13 @@ void launch_dt(const DevParams& dp, DevState* st, const double* tf, const double
 template <typename T>
 void launch_stage(int io, const Grid& g, const Phys& ph, const T* Uin, const T* Uzero, T* Uout,
                   T* ONE, const T* M, unsigned char* LIM, DevState* st, hipStream_t s) {
-  const dim3 b(64, ST_TY + 1);
-  const dim3 gr(((stage_tiles(g) + N_XCD - 1) / N_XCD) * N_XCD);
+  constexpr int TY = PlainStage<T>::TY, W = PlainStage<T>::W;
+  const dim3 b(64, TY + 1);
+  const dim3 gr(((stage_tiles(g, TY) + N_XCD - 1) / N_XCD) * N_XCD);
   if (io == 0)
-    k_stage<0, T><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st);
+    k_stage<0, T, TY, W><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st);
   else
-    k_stage<1, T><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st);
+    k_stage<1, T, TY, W><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st);
 }
 
 int mass_blocks() { return 512; }
@@ -913,7 +361,8 @@ M2P_INSTANTIATE(float)
 // otherwise surfaces only at the first launch).
 hipError_t probe_kernels() {
   hipFuncAttributes a;
-  return hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_stage<0, double>));
+  return hipFuncGetAttributes(&a, reinterpret_cast<const void*>(
+                                   &k_stage<0, double, PlainStage<double>::TY, PlainStage<double>::W>));
 }
 
 }  // namespace m2p

@@ -26,7 +26,11 @@ PRECISIONS = {"fp64": M2P_FP64, "fp32": M2P_FP32}
 
 EXPORTS = ["m2p_default_params", "m2p_open", "m2p_open_grid", "m2p_run", "m2p_step", "m2p_run_to", "m2p_sync",
            "m2p_get_state", "m2p_set_state", "m2p_dims", "m2p_mass", "m2p_scalars",
-           "m2p_get_field", "m2p_profile", "m2p_profile_get", "m2p_last_error", "m2p_close"]
+           "m2p_get_field", "m2p_profile", "m2p_profile_get", "m2p_last_error", "m2p_close",
+           "m2p_set_stop", "m2p_status", "m2p_footprint",
+           "m2p_batch_create", "m2p_batch_add", "m2p_batch_remove", "m2p_batch_size",
+           "m2p_batch_step", "m2p_batch_sync", "m2p_batch_tune", "m2p_batch_set", "m2p_batch_get",
+           "m2p_batch_last_error", "m2p_batch_destroy"]
 
 
 class M2PError(RuntimeError):
@@ -74,6 +78,24 @@ def lib():
     L.m2p_last_error.restype = ctypes.c_char_p
     L.m2p_close.argtypes = [vp]
     L.m2p_close.restype = None
+    llp = ctypes.POINTER(ctypes.c_longlong)
+    L.m2p_set_stop.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    L.m2p_status.argtypes = [vp, dp, ip, ip]
+    L.m2p_footprint.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_int, llp, llp]
+    L.m2p_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+    L.m2p_batch_add.argtypes = [vp, vp]
+    L.m2p_batch_remove.argtypes = [vp, vp]
+    L.m2p_batch_size.argtypes = [vp]
+    L.m2p_batch_step.argtypes = [vp, ctypes.c_int]
+    L.m2p_batch_sync.argtypes = [vp]
+    L.m2p_batch_tune.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    L.m2p_batch_set.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
+    L.m2p_batch_get.argtypes = [vp, ctypes.c_char_p]
+    L.m2p_batch_last_error.argtypes = [vp]
+    L.m2p_batch_last_error.restype = ctypes.c_char_p
+    L.m2p_batch_destroy.argtypes = [vp]
+    L.m2p_batch_destroy.restype = None
     _LIB = L
     return L
 
@@ -171,6 +193,18 @@ class Simulation:
     def sync(self) -> None:
         self._check(lib().m2p_sync(self._h), "m2p_sync")
 
+    def set_stop(self, t_end: float = -1.0, vstop: float = -1.0, max_steps: int = -1) -> None:
+        """Halting rules counted from now (see m2p_set_stop)."""
+        self._check(lib().m2p_set_stop(self._h, float(t_end), float(vstop), int(max_steps)),
+                    "m2p_set_stop")
+
+    def status(self):
+        """(scaled time, steps done, halted)."""
+        t, nt, h = ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
+        self._check(lib().m2p_status(self._h, ctypes.byref(t), ctypes.byref(nt), ctypes.byref(h)),
+                    "m2p_status")
+        return t.value, nt.value, bool(h.value)
+
     def state(self) -> np.ndarray:
         u = np.empty(7 * self.P, dtype=np.float64)
         self._check(lib().m2p_get_state(self._h, _dp(u), None, None), "m2p_get_state")
@@ -222,6 +256,89 @@ class Simulation:
         if getattr(self, "_h", None):
             lib().m2p_close(self._h)
             self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def footprint(mode: int, nx: int, ny: int, precision: str = "fp64", nsamp: int = 0,
+              nloc: int = 0):
+    """(steady, peak) device bytes of a scenario of that shape."""
+    a, b = ctypes.c_longlong(), ctypes.c_longlong()
+    rc = lib().m2p_footprint(int(mode), int(nx), int(ny), PRECISIONS[precision], int(nsamp),
+                             int(nloc), ctypes.byref(a), ctypes.byref(b))
+    if rc:
+        raise M2PError(f"m2p_footprint failed ({rc})")
+    return a.value, b.value
+
+
+class Batch:
+    """Many scenarios of one precision stepped together (m2p_batch_*)."""
+
+    def __init__(self, precision: str = "fp64"):
+        self._h = ctypes.c_void_p()
+        self.precision = precision
+        self.members = []
+        rc = lib().m2p_batch_create(PRECISIONS[precision], ctypes.byref(self._h))
+        if rc:
+            msg = self.last_error()
+            self.close()
+            raise M2PError(f"m2p_batch_create failed ({rc}): {msg}")
+
+    def last_error(self) -> str:
+        if not self._h:
+            return ""
+        m = lib().m2p_batch_last_error(self._h)
+        return m.decode() if m else ""
+
+    def _check(self, rc: int, what: str):
+        if rc < 0:
+            raise M2PError(f"{what} failed ({rc}): {self.last_error()}")
+        return rc
+
+    def add(self, sim: "Simulation") -> int:
+        k = self._check(lib().m2p_batch_add(self._h, sim._h), "m2p_batch_add")
+        self.members.append(sim)
+        return k
+
+    def remove(self, sim: "Simulation") -> None:
+        self._check(lib().m2p_batch_remove(self._h, sim._h), "m2p_batch_remove")
+        self.members.remove(sim)
+
+    def __len__(self) -> int:
+        return self._check(lib().m2p_batch_size(self._h), "m2p_batch_size")
+
+    def step(self, n: int = 1) -> None:
+        self._check(lib().m2p_batch_step(self._h, int(n)), "m2p_batch_step")
+
+    def sync(self) -> None:
+        self._check(lib().m2p_batch_sync(self._h), "m2p_batch_sync")
+
+    def tune(self, trial_steps: int = 3) -> str:
+        buf = ctypes.create_string_buffer(8192)
+        self._check(lib().m2p_batch_tune(self._h, int(trial_steps), buf, len(buf)), "m2p_batch_tune")
+        return buf.value.decode()
+
+    def set(self, key: str, value: int) -> None:
+        self._check(lib().m2p_batch_set(self._h, key.encode(), int(value)), "m2p_batch_set")
+
+    def get(self, key: str) -> int:
+        return self._check(lib().m2p_batch_get(self._h, key.encode()), "m2p_batch_get")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().m2p_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+        self.members = []
 
     def __del__(self):
         try:

@@ -1,0 +1,178 @@
+"""Batched ensembles: every member of a batch ends bit-identical to the same scenario
+stepped alone (whose FP64 path is bit-identical to the oracle), for both precisions,
+every compiled stage-kernel variant and block schedule, graph and non-graph stepping,
+members of different sizes and modes, members halting on their own rules, and members
+leaving mid-run.  The autotuner restores every member's state."""
+import sys
+
+import numpy as np
+import pytest
+
+from moses_2pdf_amd import Batch, Simulation
+from moses_2pdf_amd import scenarios as S
+from util import df004
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def population():
+    ms = S.ensemble_members(256)
+    mem = [S.ensemble_scenario(m) for m in [m for m in ms if m.n == 128][:3]]
+    mem.append(S.landslide_plane(n=72))
+    mem.append(S.landslide_fractal(n=40, seed=9))          # per-cell phi
+    sc = S.landslide_plane(n=50)
+    sc.cfl, sc.rho_ratio = 0.4, 1000.0 / 2700.0
+    mem.append(sc)
+    mem.append(df004())                                      # inflow mode
+    return mem
+
+
+def alone(scs, steps, precision):
+    out = []
+    for sc in scs:
+        with Simulation(sc, precision=precision) as s:
+            s.step(steps)
+            out.append((s.state(), s.time_steps()))
+    return out
+
+
+def batched(scs, steps, precision, chunks=None, setup=None):
+    sims = [Simulation(sc, precision=precision) for sc in scs]
+    b = Batch(precision)
+    for s in sims:
+        b.add(s)
+    if setup:
+        setup(b)
+    for n in (chunks or [steps]):
+        b.step(n)
+    b.sync()
+    res = [(s.state(), s.time_steps()) for s in sims]
+    b.close()
+    for s in sims:
+        s.close()
+    return res
+
+
+def same(a, b):
+    return all(np.array_equal(x[0], y[0]) and x[1] == y[1] for x, y in zip(a, b))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_equals_alone(precision):
+    scs = population()
+    ref = alone(scs, 37, precision)
+    got = batched(scs, 37, precision, chunks=[1, 20, 16])   # graph of 16 steps + direct
+    for k, (x, y) in enumerate(zip(got, ref)):
+        assert x[1] == y[1], (k, x[1], y[1])
+        assert np.array_equal(x[0], y[0]), k
+    say(precision, "batch of", len(scs), "equal to alone after 37 steps")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_every_variant_and_schedule_gives_the_same_result(precision):
+    scs = population()[:5]
+    ref = alone(scs, 12, precision)
+    b0 = Batch(precision)
+    from moses_2pdf_amd import engine
+    import ctypes
+    variants = {"fp64": [(7, 4), (3, 4), (7, 3)], "fp32": [(7, 6), (7, 4), (3, 6), (15, 4)]}[precision]
+    for ty, w in variants:
+        for remap in (0, 1):
+            for order in (0, 1):
+                def setup(b, ty=ty, w=w, remap=remap, order=order):
+                    b.set("ty", ty)
+                    b.set("waves", w)
+                    b.set("remap", remap)
+                    b.set("order", order)
+                    b.set("graph_steps", 5)
+                got = batched(scs, 12, precision, setup=setup)
+                assert same(got, ref), (ty, w, remap, order)
+    b0.close()
+
+
+def test_tune_restores_state_and_keeps_results():
+    scs = population()
+    ref = alone(scs, 10, "fp32")
+    sims = [Simulation(sc, precision="fp32") for sc in scs]
+    b = Batch("fp32")
+    for s in sims:
+        b.add(s)
+    b.step(4)
+    b.sync()
+    before = [s.state() for s in sims]
+    rep = b.tune(trial_steps=2)
+    say(rep)
+    assert "chosen:" in rep
+    after = [s.state() for s in sims]
+    assert all(np.array_equal(x, y) for x, y in zip(before, after))
+    assert all(s.time_steps()[1] == 4 for s in sims)
+    b.step(6)
+    b.sync()
+    assert all(np.array_equal(s.state(), r[0]) for s, r in zip(sims, ref))
+    b.close()
+
+
+def test_members_halt_on_their_own_rules():
+    scs = population()[:4]
+    sims = [Simulation(sc) for sc in scs]
+    ref_t = None
+    with Simulation(scs[1]) as s:
+        s.step(6)
+        t6 = s.time_steps()[0]
+        s.step(1)
+        ref_t = 0.5 * (t6 + s.time_steps()[0])   # inside step 7
+    b = Batch("fp64")
+    for s in sims:
+        b.add(s)
+    sims[0].set_stop(max_steps=5)
+    sims[1].set_stop(t_end=ref_t)
+    b.step(20)
+    b.sync()
+    st = [s.status() for s in sims]
+    assert st[0][1] == 5 and st[0][2]
+    assert st[1][1] == 7 and st[1][0] == ref_t and st[1][2]
+    assert st[2][1] == 20 and not st[2][2]
+    # the halted members equal their alone runs at the halting point
+    with Simulation(scs[0]) as s:
+        s.step(5)
+        assert np.array_equal(s.state(), sims[0].state())
+    with Simulation(scs[1]) as s:
+        s.run_to(ref_t)
+        assert np.array_equal(s.state(), sims[1].state())
+    b.close()
+
+
+def test_member_leaves_mid_run_and_steps_alone():
+    scs = population()[:4]
+    ref = alone(scs, 15, "fp64")
+    sims = [Simulation(sc) for sc in scs]
+    b = Batch("fp64")
+    for s in sims:
+        b.add(s)
+    b.step(5)
+    b.remove(sims[2])
+    assert len(b) == 3
+    with pytest.raises(Exception):
+        b.add(sims[0])              # already a member
+    sims[2].step(10)                # alone again
+    b.step(10)
+    b.sync()
+    for s, r in zip(sims, ref):
+        assert np.array_equal(s.state(), r[0])
+    b.close()
+
+
+def test_stop_speed_rule_halts_a_settling_flow():
+    sc = S.landslide_plane(n=48)
+    with Simulation(sc) as s:
+        s.set_stop(vstop=1e9)          # never reached: never halts
+        s.step(30)
+        assert not s.status()[2]
+        s.set_stop(vstop=1e-30)        # reached at once, then never below it
+        s.step(30)
+        assert not s.status()[2]
