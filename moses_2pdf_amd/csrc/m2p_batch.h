@@ -29,6 +29,7 @@ struct Member {
   const double* loc;
   const int* dire;
   int ghost0, box0, ring0, stage0;   // first block of this member in each launch
+  int box_n, ring_n, stage_n;        // and how many
 };
 
 // Launch description of a batch: the member table and, per launch, the member of each
@@ -39,8 +40,11 @@ struct BatchDev {
   const int* mapBox;
   const int* mapRing;
   const int* mapStage;
+  double* part;   // per-block partial maxima: CFL box [0, nBox), ring [nBox, +nRing),
+                  // corrector tiles [nBox + nRing, +nStage)
   int nmem, nGhost, nBox, nRing, nStage;
   int remap;   // stage blocks dealt to the 8 XCDs in contiguous bands
+  int vmax;    // some member has a stop speed: reduce corrector speeds after each step
 };
 
 // Tile shapes and occupancy caps with a compiled batched stage kernel, per scalar type.

@@ -49,6 +49,8 @@ struct m2p_batch {
   size_t capMem = 0;
   int* dMap = nullptr;
   size_t capMap = 0;
+  double* dPart = nullptr;
+  size_t capPart = 0;
   m2p::BatchDev B{};
   bool any_inflow = false;
   hipGraph_t graph = nullptr;
@@ -58,6 +60,7 @@ struct m2p_batch {
 };
 
 hipStream_t m2p_batch_stream(m2p_batch* b) { return b->s; }
+void m2p_batch_mark_dirty(m2p_batch* b) { b->dirty = true; }
 
 namespace {
 
@@ -104,6 +107,7 @@ int rebuild(m2p_batch* b) {
   std::vector<m2p::Member> M(n);
   std::vector<int> mg, mb, mr, ms;
   b->any_inflow = false;
+  bool any_vstop = false;
   for (int k = 0; k < n; k++) {
     m2p_ctx* c = b->mem[ord[k]];
     m2p::Member& m = M[k];
@@ -133,6 +137,10 @@ int rebuild(m2p_batch* b) {
     m.loc = c->dLoc;
     m.dire = c->dDire;
     if (c->mode == 1) b->any_inflow = true;
+    if (c->vstop_on) any_vstop = true;
+    m.box_n = m.nbx * m2p::box_blocks_y(c->g);
+    m.ring_n = m2p::ring_blocks(c->g);
+    m.stage_n = m2p::stage_tiles(c->g, b->ty);
     m.ghost0 = (int)mg.size();
     mg.insert(mg.end(), (c->nyd + c->nxd + m2p::GHOST_LINES - 1) / m2p::GHOST_LINES, k);
     m.box0 = (int)mb.size();
@@ -156,6 +164,13 @@ int rebuild(m2p_batch* b) {
     b->capMap = std::max(nmap, 2 * b->capMap);
     BCHK(b, hipMalloc((void**)&b->dMap, b->capMap * sizeof(int)));
   }
+  const size_t npart = mb.size() + mr.size() + ms.size();
+  if (npart > b->capPart) {
+    if (b->dPart) (void)hipFree(b->dPart);
+    b->dPart = nullptr;
+    b->capPart = std::max(npart, 2 * b->capPart);
+    BCHK(b, hipMalloc((void**)&b->dPart, b->capPart * sizeof(double)));
+  }
   std::vector<int> all;
   all.reserve(nmap);
   for (auto* v : {&mg, &mb, &mr, &ms}) all.insert(all.end(), v->begin(), v->end());
@@ -164,6 +179,7 @@ int rebuild(m2p_batch* b) {
   BCHK(b, hipStreamSynchronize(b->s));
   m2p::BatchDev& B = b->B;
   B.mem = b->dMem;
+  B.part = b->dPart;
   B.mapGhost = b->dMap;
   B.mapBox = b->dMap + mg.size();
   B.mapRing = b->dMap + mg.size() + mb.size();
@@ -174,6 +190,7 @@ int rebuild(m2p_batch* b) {
   B.nRing = (int)mr.size();
   B.nStage = (int)ms.size();
   B.remap = b->remap;
+  B.vmax = any_vstop ? 1 : 0;
   b->dirty = false;
   return 0;
 }
@@ -410,6 +427,7 @@ void m2p_batch_destroy(m2p_batch* b) {
   drop_graph(b);
   if (b->dMem) (void)hipFree(b->dMem);
   if (b->dMap) (void)hipFree(b->dMap);
+  if (b->dPart) (void)hipFree(b->dPart);
   if (b->s) (void)hipStreamDestroy(b->s);
   delete b;
 }

@@ -41,7 +41,8 @@ __global__ void __launch_bounds__(256) kb_cfl(BatchDev B) {
   const Member& m = B.mem[B.mapBox[blockIdx.x]];
   if (halted_pre(m.st)) return;
   const int lb = (int)blockIdx.x - m.box0;
-  block_max_to(cfl_box((const T*)m.U0, (const T*)m.MT, m.g, m.ph, lb % m.nbx, lb / m.nbx), m.st);
+  block_max_store(cfl_box((const T*)m.U0, (const T*)m.MT, m.g, m.ph, lb % m.nbx, lb / m.nbx),
+                  B.part + blockIdx.x);
 }
 
 template <typename T>
@@ -51,15 +52,32 @@ __global__ void __launch_bounds__(64) kb_ring(BatchDev B) {
   const int t = ((int)blockIdx.x - m.ring0) * 64 + threadIdx.x;
   double v = cfl_ring_face((const T*)m.U0, (const T*)m.MT, m.g, m.ph, t);
   v = wave_max(v);
-  if (threadIdx.x == 0) atomicMax(&m.st->maxbits, (unsigned long long)__double_as_longlong(v));
+  if (threadIdx.x == 0) B.part[B.nBox + blockIdx.x] = v;
 }
 
+// One block per member: its CFL and corrector partials, then the dt control.
 template <typename T>
-__global__ void __launch_bounds__(64) kb_dt(BatchDev B) {
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  if (i >= B.nmem) return;
-  const Member& m = B.mem[i];
-  dt_body(m.dp, m.st, m.tf, m.itime);
+__global__ void __launch_bounds__(256) kb_dt(BatchDev B) {
+  const Member& m = B.mem[blockIdx.x];
+  __shared__ int halted;
+  if (threadIdx.x == 0) halted = dt_halted(m.st) ? 1 : 0;
+  __syncthreads();
+  if (halted) return;
+  const double mb = block_max_of(B.part + m.box0, m.box_n);
+  const double mr = block_max_of(B.part + B.nBox + m.ring0, m.ring_n);
+  if (threadIdx.x == 0) dt_body(m.dp, m.st, m.tf, m.itime, fmax(mb, mr));
+}
+
+// One block per member after the corrector: its squared max flow speed into DevState.
+template <typename T>
+__global__ void __launch_bounds__(256) kb_vmax(BatchDev B) {
+  const Member& m = B.mem[blockIdx.x];
+  if (halted_post(m.st) || !(m.st->vstop2 > 0.0)) return;
+  const double v2 = block_max_of(B.part + B.nBox + B.nRing + m.stage0, m.stage_n);
+  if (threadIdx.x == 0) {
+    m.st->v2 = v2;
+    m.st->v2_nt = m.st->nt;
+  }
 }
 
 template <int IO, typename T, int TY, int W>
@@ -83,7 +101,7 @@ __global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per
   const double v2 = stage_tile<IO, T, TY>(F, m.g, m.ph, Uin, (const T*)m.U0, Uout, (T*)m.ONE,
                                           (const T*)m.MT, m.LIM,
                                           IO == 0 ? st->dt_pred : st->dt_corr, tx, ty);
-  if (IO == 1 && st->vstop2 > 0.0) block_max_bits(v2, &st->vmaxbits);
+  if (IO == 1 && st->vstop2 > 0.0) block_max_store(v2, B.part + B.nBox + B.nRing + t);
 }
 
 template <typename T, int TY, int W>
@@ -124,10 +142,11 @@ hipError_t launch_batch_steps(const BatchDev& B, int ty, int w, bool any_inflow,
     kb_ghost<T><<<dim3(B.nGhost, MN), GHOST_LINES, 0, s>>>(B, 0);
     kb_cfl<T><<<B.nBox, dim3(64, 4), 0, s>>>(B);
     kb_ring<T><<<B.nRing, 64, 0, s>>>(B);
-    kb_dt<T><<<(B.nmem + 63) / 64, 64, 0, s>>>(B);
+    kb_dt<T><<<B.nmem, 256, 0, s>>>(B);
     if (!launch_stage_variant<T>(B, ty, w, 0, s)) return hipErrorInvalidValue;
     kb_ghost<T><<<dim3(B.nGhost, MN), GHOST_LINES, 0, s>>>(B, 1);
     if (!launch_stage_variant<T>(B, ty, w, 1, s)) return hipErrorInvalidValue;
+    if (B.vmax) kb_vmax<T><<<B.nmem, 256, 0, s>>>(B);
   }
   return hipGetLastError();
 }

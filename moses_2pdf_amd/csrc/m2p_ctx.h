@@ -14,6 +14,8 @@
 
 // the stream of the batch that steps a scenario (m2p_batch.hip)
 hipStream_t m2p_batch_stream(m2p_batch* b);
+// the batch must re-read its members' settings before its next step
+void m2p_batch_mark_dirty(m2p_batch* b);
 
 using m2p::DevParams;
 using m2p::DevState;
@@ -38,6 +40,7 @@ struct m2p_ctx {
   void *dU0 = nullptr, *dU1 = nullptr, *dONE = nullptr, *dMT = nullptr;
   unsigned char* dLIM = nullptr;  // FP32 only: limiter memory flags, one byte per cell
   double *dTf = nullptr, *dMass = nullptr;
+  double* dPart = nullptr;  // per-block partial maxima of a step (m2p::partial_count)
   double *dTable = nullptr, *dLoc = nullptr, *dItime = nullptr;
   int* dDire = nullptr;
   DevState* dSt = nullptr;
@@ -50,4 +53,5 @@ struct m2p_ctx {
   size_t evnext = 0;
   std::vector<std::pair<const char*, std::pair<hipEvent_t, hipEvent_t>>> pending;
   m2p_batch* batch = nullptr;  // the batch that steps this scenario, if any
+  bool vstop_on = false;       // a stop speed is set (the step reduces corrector speeds)
 };

@@ -97,12 +97,14 @@ void release(m2p_ctx* c) {
   c->pending.clear();
   if (c->dMT == c->dM) c->dMT = nullptr;
   void* ps[] = {c->dTopo, c->dDepth, c->dPhi, c->dM,     c->dU0,   c->dU1,    c->dONE, c->dMT,
-                c->dTf,   c->dMass,  c->dTable, c->dLoc, c->dItime, c->dDire, c->dSt,  c->dLIM};
+                c->dTf,   c->dMass,  c->dTable, c->dLoc, c->dItime, c->dDire, c->dSt,  c->dLIM,
+                c->dPart};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   c->dTopo = c->dDepth = c->dPhi = c->dM = c->dTf = c->dMass = c->dTable = c->dLoc = c->dItime = nullptr;
   c->dU0 = c->dU1 = c->dONE = c->dMT = nullptr;
   c->dLIM = nullptr;
+  c->dPart = nullptr;
   c->dDire = nullptr;
   c->dSt = nullptr;
   if (c->s) (void)hipStreamDestroy(c->s);
@@ -164,11 +166,13 @@ int setup(m2p_ctx* c) {
   c->ph.N_R = c->par.N_R;
   c->ph.varTheta = c->par.varTheta;
   c->ph.tande = std::tan(c->par.delta0 * 3.14159 / 180.);  // makeTopo11Kernel, L/K:298
+  if ((rc = dalloc(c, &c->dPart, m2p::partial_count(c->g, c->prec)))) return rc;
 
   DevState st{};
   st.iter = 1;
   st.t_end = -1.0;
   st.halt_at = INT_MAX;
+  st.v2_nt = -1;
   HIPCHK(c, hipMemcpyAsync(c->dSt, &st, sizeof st, hipMemcpyHostToDevice, c->s));
   m2p::launch_metric_setup(c->g, c->mode, c->dTopo, c->dDepth, c->dPhi, c->par.phiS0, c->dM,
                            dU64, c->s);
@@ -297,15 +301,16 @@ void enqueue_step_t(m2p_ctx* c, int ntf) {
   }
   {
     Timer t(c, "cfl");
-    m2p::launch_cfl(c->g, c->ph, (const T*)U0, MT, c->dSt, c->s);
+    m2p::launch_cfl(c->g, c->ph, (const T*)U0, MT, c->dSt, c->dPart, c->s);
   }
   {
     Timer t(c, "dt");
-    m2p::launch_dt(dp, c->dSt, c->dTf, c->dItime, c->s);
+    m2p::launch_dt(dp, c->dSt, c->dTf, c->dItime, c->g, c->prec, c->dPart, c->s);
   }
   {
     Timer t(c, "stage0");
-    m2p::launch_stage(0, c->g, c->ph, (const T*)U0, (const T*)U0, U1, ONE, MT, c->dLIM, c->dSt, c->s);
+    m2p::launch_stage(0, c->g, c->ph, (const T*)U0, (const T*)U0, U1, ONE, MT, c->dLIM, c->dSt,
+                      c->dPart, c->s);
   }
   {
     Timer t(c, "ghost");
@@ -313,8 +318,10 @@ void enqueue_step_t(m2p_ctx* c, int ntf) {
   }
   {
     Timer t(c, "stage1");
-    m2p::launch_stage(1, c->g, c->ph, (const T*)U1, (const T*)U0, U0, ONE, MT, c->dLIM, c->dSt, c->s);
+    m2p::launch_stage(1, c->g, c->ph, (const T*)U1, (const T*)U0, U0, ONE, MT, c->dLIM, c->dSt,
+                      c->dPart, c->s);
   }
+  if (c->vstop_on) m2p::launch_vmax(c->dSt, c->g, c->prec, c->dPart, c->s);
 }
 
 void enqueue_step(m2p_ctx* c, int ntf) {
@@ -744,9 +751,11 @@ int m2p_set_stop(m2p_ctx* c, double t_end, double vstop, int max_steps) {
   st.halt_at = max_steps < 0 ? INT_MAX : (max_steps > INT_MAX - st.nt ? INT_MAX : st.nt + max_steps);
   st.skip = 0;
   st.moved = 0;
-  st.vmaxbits = 0;
+  st.v2_nt = -1;
   HIPCHK(c, hipMemcpyAsync(c->dSt, &st, sizeof st, hipMemcpyHostToDevice, s));
   HIPCHK(c, hipStreamSynchronize(s));
+  c->vstop_on = vstop > 0;
+  if (c->batch) m2p_batch_mark_dirty(c->batch);
   return 0;
 }
 
@@ -767,6 +776,7 @@ int m2p_footprint(int mode, int NX, int NY, int precision, int nsamp, int nloc,
   if (NX < 2 || NY < 2 || (precision != M2P_FP64 && precision != M2P_FP32)) return M2P_EINVAL;
   const long long P = (long long)(NX + 2 * MD) * (NY + 2 * MD);
   long long b = (long long)m2p::MC_COUNT * P * 8 + 4LL * m2p::mass_blocks() * 8 + (long long)sizeof(DevState) + 4096;
+  b += (P / 256 + P / 441 + 4LL * (NX + NY) + 64) * 8;   // per-block partial maxima
   if (precision == M2P_FP64)
     b += 16 * P * 8;
   else

@@ -20,8 +20,6 @@ struct DevState {
                       //      scenario halts
   double vstop2;      // > 0: halt once the squared max flow speed falls below it (after
                       //      it has been reached once)
-  unsigned long long maxbits;   // running max wave speed (IEEE bits of a positive double)
-  unsigned long long vmaxbits;  // squared max flow speed of the last corrector (IEEE bits)
   int iter;           // next output index (L/R:390)
   int schreiben;      // output due after this step
   int nstop;
@@ -29,7 +27,9 @@ struct DevState {
   int nt;             // steps done
   int halt_at;        // step count after which every kernel of this scenario returns at once
   int skip;           // halted (set by the dt kernel for the kernels after it in a step)
+  double v2;          // squared max flow speed of the corrector of step v2_nt
   int moved;          // the flow speed has reached the stop threshold at least once
+  int v2_nt;          // step whose corrector v2 comes from (-1: none)
 };
 
 // A scenario is halted for the kernels that run before the dt kernel of a step
@@ -56,13 +56,20 @@ void launch_inflow(const Grid& g, T* U, const double* M, const double* table, co
 template <typename T>
 void launch_ghost(const Grid& g, int mode, T* Tg, const T* prev, const DevState* st, int post,
                   hipStream_t s);
+// Per-block partial maxima of one step (CFL box blocks, CFL ring blocks, corrector tiles),
+// reduced by the dt kernel: partial_count() doubles per scenario.
+size_t partial_count(const Grid& g, int prec);
 template <typename T>
-void launch_cfl(const Grid& g, const Phys& ph, const T* U, const T* M, DevState* st, hipStream_t s);
+void launch_cfl(const Grid& g, const Phys& ph, const T* U, const T* M, DevState* st, double* part,
+                hipStream_t s);
 void launch_dt(const DevParams& dp, DevState* st, const double* tf, const double* itime,
-               hipStream_t s);
+               const Grid& g, int prec, const double* part, hipStream_t s);
+// After a corrector: the squared max flow speed of its tiles into DevState (stop speed).
+void launch_vmax(DevState* st, const Grid& g, int prec, const double* part, hipStream_t s);
 template <typename T>
 void launch_stage(int io, const Grid& g, const Phys& ph, const T* Uin, const T* Uzero, T* Uout,
-                  T* ONE, const T* M, unsigned char* LIM, DevState* st, hipStream_t s);
+                  T* ONE, const T* M, unsigned char* LIM, DevState* st, double* part,
+                  hipStream_t s);
 int mass_blocks();
 template <typename T>
 void launch_mass(const Grid& g, const T* U, const double* M, double* out, hipStream_t s);

@@ -12,6 +12,8 @@
 // bodies are device functions so the batched-ensemble kernels reuse them.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "m2p_kernels.h"
 #include "m2p_numerics.h"
 #include "m2p_tiles.h"
@@ -170,26 +172,53 @@ __global__ void k_ghost(Grid g, int mode, T* __restrict__ Tg, const T* __restric
   ghost_line(g, mode, Tg, prev, line, (int)blockIdx.y);
 }
 
+// Blocks stride over the 64x4 face boxes (at most CFL_BLOCKS blocks), so the dt kernel
+// reduces few partials.
+constexpr int CFL_BLOCKS = 2048;
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_cfl(Grid g, Phys ph, const T* __restrict__ U,
-                                             const T* __restrict__ M, DevState* __restrict__ st) {
+                                             const T* __restrict__ M, DevState* __restrict__ st,
+                                             double* __restrict__ part, int nbx, int nbox) {
   if (halted_pre(st)) return;
-  block_max_to(cfl_box(U, M, g, ph, blockIdx.x, blockIdx.y), st);
+  double m = 0.0;
+  for (int t = blockIdx.x; t < nbox; t += gridDim.x) m = fmax(m, cfl_box(U, M, g, ph, t % nbx, t / nbx));
+  block_max_store(m, part + blockIdx.x);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(64) k_cfl_ring(Grid g, Phys ph, const T* __restrict__ U,
-                                                 const T* __restrict__ M, DevState* __restrict__ st) {
+                                                 const T* __restrict__ M, DevState* __restrict__ st,
+                                                 double* __restrict__ part) {
   if (halted_pre(st)) return;
   double m = cfl_ring_face(U, M, g, ph, (int)(blockIdx.x * blockDim.x + threadIdx.x));
   m = wave_max(m);
-  if (threadIdx.x == 0) atomicMax(&st->maxbits, (unsigned long long)__double_as_longlong(m));
+  if (threadIdx.x == 0) part[blockIdx.x] = m;
 }
 
-__global__ void k_dt(DevParams dp, DevState* __restrict__ st, const double* __restrict__ tf,
-                     const double* __restrict__ itime) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  dt_body(dp, st, tf, itime);
+// One block: the max over the CFL partials [0, ncfl), then the dt control in thread 0.
+__global__ void __launch_bounds__(256) k_dt(DevParams dp, DevState* __restrict__ st,
+                                            const double* __restrict__ tf,
+                                            const double* __restrict__ itime,
+                                            const double* __restrict__ part, int ncfl) {
+  __shared__ int halted;
+  if (threadIdx.x == 0) halted = dt_halted(st) ? 1 : 0;
+  __syncthreads();
+  if (halted) return;
+  const double mx = block_max_of(part, ncfl);
+  if (threadIdx.x == 0) dt_body(dp, st, tf, itime, mx);
+}
+
+// One block, after the corrector of a step with a stop speed: the squared max flow speed
+// over its tile partials, stamped with the step.
+__global__ void __launch_bounds__(256) k_vmax(DevState* __restrict__ st,
+                                              const double* __restrict__ part, int n) {
+  if (halted_post(st) || !(st->vstop2 > 0.0)) return;
+  const double v2 = block_max_of(part, n);
+  if (threadIdx.x == 0) {
+    st->v2 = v2;
+    st->v2_nt = st->nt;
+  }
 }
 
 // Tile height and the waves-per-SIMD cap of the single-scenario stage kernel: FP64 is held
@@ -202,16 +231,17 @@ template <int IO, typename T, int TY, int W>
 __global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per_eu(W, W)))
     k_stage(Grid g, Phys ph, const T* __restrict__ Uin, const T* Uzero, T* Uout,
             T* __restrict__ ONE, const T* __restrict__ M, unsigned char* __restrict__ LIM,
-            DevState* __restrict__ st) {
+            DevState* __restrict__ st, double* __restrict__ part) {
   __shared__ T F[ST_NQ][TY + 1][64];
   if (halted_post(st)) return;
   const int b = blockIdx.x;
   const int per = (stage_tiles(g, TY) + N_XCD - 1) / N_XCD;
   int tx, ty;
-  if (!tile_of(g, TY, (b % N_XCD) * per + b / N_XCD, tx, ty)) return;
+  const int t = (b % N_XCD) * per + b / N_XCD;
+  if (!tile_of(g, TY, t, tx, ty)) return;
   const double v2 = stage_tile<IO, T, TY>(F, g, ph, Uin, Uzero, Uout, ONE, M, LIM,
                                           IO == 0 ? st->dt_pred : st->dt_corr, tx, ty);
-  if (IO == 1 && st->vstop2 > 0.0) block_max_bits(v2, &st->vmaxbits);
+  if (IO == 1 && st->vstop2 > 0.0) block_max_store(v2, part + t);
 }
 
 // Sum of u0*Det and u3*Det over interior cells with double-double accumulation; one
@@ -312,28 +342,48 @@ void launch_ghost(const Grid& g, int mode, T* Tg, const T* prev, const DevState*
   k_ghost<T><<<dim3((lines + 127) / 128, MN), 128, 0, s>>>(g, mode, Tg, prev, st, post);
 }
 
+static int cfl_boxes(const Grid& g) { return ((g.nx + 63) / 64) * ((g.ny + 3) / 4); }
+static int cfl_box_blocks(const Grid& g) { return std::min(cfl_boxes(g), CFL_BLOCKS); }
+static int cfl_ring_blocks(const Grid& g) { return (ring_threads(g) + 63) / 64; }
+
+size_t partial_count(const Grid& g, int prec) {
+  const int ty = prec == 1 ? PlainStage<float>::TY : PlainStage<double>::TY;
+  return (size_t)cfl_box_blocks(g) + cfl_ring_blocks(g) + stage_tiles(g, ty);
+}
+
 template <typename T>
-void launch_cfl(const Grid& g, const Phys& ph, const T* U, const T* M, DevState* st, hipStream_t s) {
+void launch_cfl(const Grid& g, const Phys& ph, const T* U, const T* M, DevState* st, double* part,
+                hipStream_t s) {
   const dim3 b(64, 4);
-  k_cfl<T><<<grid2(g.nx, g.ny, b), b, 0, s>>>(g, ph, U, M, st);
-  k_cfl_ring<T><<<(ring_threads(g) + 63) / 64, 64, 0, s>>>(g, ph, U, M, st);
+  k_cfl<T><<<cfl_box_blocks(g), b, 0, s>>>(g, ph, U, M, st, part, (g.nx + 63) / 64, cfl_boxes(g));
+  k_cfl_ring<T><<<cfl_ring_blocks(g), 64, 0, s>>>(g, ph, U, M, st, part + cfl_box_blocks(g));
 }
 
 void launch_dt(const DevParams& dp, DevState* st, const double* tf, const double* itime,
-               hipStream_t s) {
-  k_dt<<<1, 64, 0, s>>>(dp, st, tf, itime);
+               const Grid& g, int prec, const double* part, hipStream_t s) {
+  const int ncfl = cfl_box_blocks(g) + cfl_ring_blocks(g);
+  const int ty = prec == 1 ? PlainStage<float>::TY : PlainStage<double>::TY;
+  (void)ty;
+  k_dt<<<1, 256, 0, s>>>(dp, st, tf, itime, part, ncfl);
+}
+
+void launch_vmax(DevState* st, const Grid& g, int prec, const double* part, hipStream_t s) {
+  const int ty = prec == 1 ? PlainStage<float>::TY : PlainStage<double>::TY;
+  k_vmax<<<1, 256, 0, s>>>(st, part + cfl_box_blocks(g) + cfl_ring_blocks(g), stage_tiles(g, ty));
 }
 
 template <typename T>
 void launch_stage(int io, const Grid& g, const Phys& ph, const T* Uin, const T* Uzero, T* Uout,
-                  T* ONE, const T* M, unsigned char* LIM, DevState* st, hipStream_t s) {
+                  T* ONE, const T* M, unsigned char* LIM, DevState* st, double* part,
+                  hipStream_t s) {
   constexpr int TY = PlainStage<T>::TY, W = PlainStage<T>::W;
   const dim3 b(64, TY + 1);
   const dim3 gr(((stage_tiles(g, TY) + N_XCD - 1) / N_XCD) * N_XCD);
+  double* sp = part + cfl_box_blocks(g) + cfl_ring_blocks(g);
   if (io == 0)
-    k_stage<0, T, TY, W><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st);
+    k_stage<0, T, TY, W><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp);
   else
-    k_stage<1, T, TY, W><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st);
+    k_stage<1, T, TY, W><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp);
 }
 
 int mass_blocks() { return 512; }
@@ -349,10 +399,10 @@ void launch_mass(const Grid& g, const T* U, const double* M, double* out, hipStr
                                  unsigned char*, hipStream_t);                                 \
   template void launch_ghost<T>(const Grid&, int, T*, const T*, const DevState*, int,         \
                                 hipStream_t);                                                 \
-  template void launch_cfl<T>(const Grid&, const Phys&, const T*, const T*, DevState*,         \
+  template void launch_cfl<T>(const Grid&, const Phys&, const T*, const T*, DevState*, double*, \
                               hipStream_t);                                                    \
   template void launch_stage<T>(int, const Grid&, const Phys&, const T*, const T*, T*, T*,     \
-                                const T*, unsigned char*, DevState*, hipStream_t);             \
+                                const T*, unsigned char*, DevState*, double*, hipStream_t);    \
   template void launch_mass<T>(const Grid&, const T*, const double*, double*, hipStream_t);
 M2P_INSTANTIATE(double)
 M2P_INSTANTIATE(float)

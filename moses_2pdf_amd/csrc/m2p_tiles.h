@@ -178,7 +178,10 @@ __device__ __forceinline__ double face_cfl(const T* __restrict__ U, const T* __r
   return fmax((double)s.emS / d, (double)s.emF / d);
 }
 
-__device__ __forceinline__ void block_max_to(double m, DevState* st) {
+// Max over the block of a per-thread value (fmax: NaN ignored unless all are NaN), written
+// by thread 0 to *out.  Every thread of the block must call it.  Each block writes its own
+// slot and the dt kernel reduces the slots, so no two blocks contend for one address.
+__device__ __forceinline__ void block_max_store(double m, double* out) {
   m = wave_max(m);
   __shared__ double part[16];
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
@@ -188,8 +191,33 @@ __device__ __forceinline__ void block_max_to(double m, DevState* st) {
   if (tid == 0) {
     double b = part[0];
     for (int w = 1; w < nw; w++) b = fmax(b, part[w]);
-    atomicMax(&st->maxbits, (unsigned long long)__double_as_longlong(b));
+    *out = b;
   }
+}
+
+// fmax of p[0..n) by all threads of a 1-D block; every thread gets the result.  The max
+// of a set does not depend on the order it is formed in.
+__device__ __forceinline__ double block_max_of(const double* __restrict__ p, int n) {
+  double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;   // four independent chains of loads
+  const int s = blockDim.x;
+  int i = threadIdx.x;
+  for (; i + 3 * s < n; i += 4 * s) {
+    m0 = fmax(m0, p[i]);
+    m1 = fmax(m1, p[i + s]);
+    m2 = fmax(m2, p[i + 2 * s]);
+    m3 = fmax(m3, p[i + 3 * s]);
+  }
+  for (; i < n; i += s) m0 = fmax(m0, p[i]);
+  double m = fmax(fmax(m0, m1), fmax(m2, m3));
+  m = wave_max(m);
+  __shared__ double part[16];
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  double b = part[0];
+  for (int w = 1; w < (int)((blockDim.x + 63) / 64); w++) b = fmax(b, part[w]);
+  __syncthreads();
+  return b;
 }
 
 // one 64x4 block of interior faces (both directions) starting at box column bx*64, row by*4
@@ -235,29 +263,31 @@ __device__ __forceinline__ double cfl_ring_face(const T* __restrict__ U, const T
 }
 
 // CFL3Kernel + the host's dt control + CFL4Kernel (L/K:2006-2036, L/R:978-991; inflow
-// sample advance I/R:1078-1081), on the device.  The max is rounded to float as the
-// reference's fmaxf reduction does (L/K:1988, 1996).  Also the halting rules, which have
-// no reference counterpart: a step budget (halt_at), an end time (t_end: the step that
-// reaches it is shortened to end on it) and a stop speed (vstop2: once the squared max
-// flow speed of a corrector has reached it and later falls below it, the next step is the
-// last).
-__device__ __forceinline__ void dt_body(const DevParams& dp, DevState* __restrict__ st,
-                                        const double* __restrict__ tf,
-                                        const double* __restrict__ itime) {
-  if (st->skip) return;
+// sample advance I/R:1078-1081), on the device.  mx is the max wave speed over the CFL
+// faces; it is rounded to float as the reference's fmaxf reduction does (L/K:1988, 1996).
+// Also the halting rules, which have no reference counterpart: a step budget (halt_at),
+// an end time (t_end: the step that reaches it is shortened to end on it) and a stop speed
+// (vstop2: once the squared max flow speed v2 of a corrector has reached it and a later one
+// falls below it, the next step is the last).
+__device__ __forceinline__ bool dt_halted(DevState* __restrict__ st) {
+  if (st->skip) return true;
   if (st->nt >= st->halt_at) {
     st->skip = 1;
-    return;
+    return true;
   }
-  if (st->vstop2 > 0.0 && st->nt > 0) {
-    const double v2 = __longlong_as_double((long long)st->vmaxbits);
+  return false;
+}
+
+__device__ __forceinline__ void dt_body(const DevParams& dp, DevState* __restrict__ st,
+                                        const double* __restrict__ tf,
+                                        const double* __restrict__ itime, double mx) {
+  if (st->vstop2 > 0.0 && st->v2_nt == st->nt) {
+    const double v2 = st->v2;
     if (v2 >= st->vstop2)
       st->moved = 1;
     else if (st->moved && st->halt_at > st->nt + 1)
       st->halt_at = st->nt + 1;
   }
-  st->vmaxbits = 0ull;
-  const double mx = __longlong_as_double((long long)st->maxbits);
   const double dmax = (double)(float)mx;
   double dtval = dp.cfl / dmax;
   dtval = fmin(dp.dt_max, dtval);
@@ -280,7 +310,6 @@ __device__ __forceinline__ void dt_body(const DevParams& dp, DevState* __restric
   st->dt_pred = dtval / (2.0 - 0);
   st->dt_corr = dtval / (2.0 - 1);
   st->time = st->dt_corr + T;
-  st->maxbits = 0ull;
   st->nt = st->nt + 1;
 }
 
@@ -533,22 +562,6 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
     const double ufx = (double)qdiv(lim4, un[3] + h0), ufy = (double)qdiv(lim5, un[3] + h0);
     const double sx = (double)usxnew, sy = (double)usynew;
     return fmax(sx * sx + sy * sy, ufx * ufx + ufy * ufy);
-  }
-}
-
-// Max over the block of a per-thread value into a DevState field (atomicMax on the IEEE
-// bits of a non-negative double).  Every thread of the block must call it.
-__device__ __forceinline__ void block_max_bits(double m, unsigned long long* dst) {
-  m = wave_max(m);
-  __shared__ double part[16];
-  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-  const int nw = (blockDim.x * blockDim.y + 63) / 64;
-  if ((tid & 63) == 0) part[tid >> 6] = m;
-  __syncthreads();
-  if (tid == 0) {
-    double b = part[0];
-    for (int w = 1; w < nw; w++) b = fmax(b, part[w]);
-    if (b > 0.0) atomicMax(dst, (unsigned long long)__double_as_longlong(b));
   }
 }
 
