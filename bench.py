@@ -49,17 +49,28 @@ def traffic_from_profiles(kernel: str, n: int):
 
 
 def cpu_baseline(sc, steps: int):
-    """The FP64 oracle (serial C++ restatement of the reference kernels) on the same
-    scenario, timed around `steps` steps only."""
-    from oracle.oracle import Oracle
-    o = Oracle(sc)
+    """The FP64 oracle (C++ restatement of the reference kernels, OpenMP over the host's
+    threads; its results do not depend on the thread count) on the same scenario, timed
+    around `steps` steps only, with all threads and with one."""
+    from oracle import oracle as O
+    nthr = O.threads(int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
+    o = O.Oracle(sc)
+    o.step(1)   # first touch of the work arrays
     t0 = time.perf_counter()
     o.step(steps)
     dt = time.perf_counter() - t0
-    return {"value": sc.nx * sc.ny * steps / dt, "unit": "cell-updates/s", "cores": 1,
-            "kind": "port",
-            "sample": f"{steps} full step(s) of the same {sc.nx}x{sc.ny} landslide workload, "
-                      f"serial oracle (-O2 -ffp-contract=off), step loop only, {dt:.1f} s"}
+    O.threads(1)
+    o1 = O.Oracle(sc)
+    o1.step(1)
+    t1 = time.perf_counter()
+    o1.step(1)
+    d1 = time.perf_counter() - t1
+    O.threads(nthr)
+    return {"value": sc.nx * sc.ny * steps / dt, "unit": "cell-updates/s", "cores": nthr,
+            "kind": "port", "single_core_value": sc.nx * sc.ny / d1,
+            "sample": f"{steps} full step(s) of the same {sc.nx}x{sc.ny} landslide workload after one "
+                      f"untimed step, FP64 oracle (-O2 -ffp-contract=off, OpenMP) on {nthr} threads, "
+                      f"step loop only, {dt:.1f} s; one step on 1 thread took {d1:.1f} s"}
 
 
 def ensemble(args) -> int:
@@ -173,7 +184,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--precision", choices=sorted(ELEM), default="fp64",
                     help="fp64: the reference's arithmetic (the headline); fp32: FP32-bulk")
     ap.add_argument("--copy-calib", action="store_true",

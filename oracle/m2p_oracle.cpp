@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <omp.h>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -432,6 +433,7 @@ struct Orc {
 
   // UzeroKernel L/K:832-851
   void uzeroCopy() {
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = 0; k < MN; k++)
       for (int row = 3; row < nyd - MD; row++)
         for (int col = 3; col < nxd - MD; col++) uzero[K(k) + I(row, col)] = u[K(k) + I(row, col)];
@@ -493,6 +495,7 @@ struct Orc {
   }
   // TVD1..5Kernel L/K:1000-1119: minmod(minmod(a,b), central) slopes.
   void tvd() {
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = 0; k < MN; k++)
       for (int row = 0; row < nyd - 1; row++)
         for (int col = 0; col < nxd - 1; col++) {
@@ -500,6 +503,7 @@ struct Orc {
           dux[i] = u[K(k) + I(row, col + 1)] - u[i];  // TVD1
           duy[i] = u[K(k) + I(row + 1, col)] - u[i];
         }
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = 0; k < MN; k++)
       for (int row = 1; row < nyd - 1; row++)
         for (int col = 1; col < nxd - 1; col++) {
@@ -509,6 +513,7 @@ struct Orc {
           sgnAy[i] = (duy[sN] > 0.0) ? 1.0 : -1.0;
           sgnBy[i] = (duy[i] > 0.0) ? 1.0 : -1.0;
         }
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = 0; k < MN; k++)
       for (int row = 1; row < nyd - 1; row++)
         for (int col = 1; col < nxd - 1; col++) {
@@ -518,6 +523,7 @@ struct Orc {
           t1y[i] = 0.5 * (sgnAy[i] + sgnBy[i]) * std::fmin(duy[sN] * sgnAy[i], duy[i] * sgnBy[i]);
           t2y[i] = 0.5 * (duy[sN] + duy[i]);
         }
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = 0; k < MN; k++)
       for (int row = 1; row < nyd - 1; row++)
         for (int col = 1; col < nxd - 1; col++) {
@@ -527,6 +533,7 @@ struct Orc {
           sgnAy[i] = (t1y[i] > 0.0) ? 1.0 : -1.0;
           sgnBy[i] = (t2y[i] > 0.0) ? 1.0 : -1.0;
         }
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = 0; k < MN; k++)
       for (int row = 1; row < nyd - 1; row++)
         for (int col = 1; col < nxd - 1; col++) {
@@ -537,6 +544,7 @@ struct Orc {
   }
   // InterfacesKernel L/K:1226-1252, range [0, N-2]^2
   void interfaces() {
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = 0; k < MN; k++)
       for (int row = 0; row < nyd - 1; row++)
         for (int col = 0; col < nxd - 1; col++) {
@@ -583,6 +591,7 @@ struct Orc {
   void keepPositivity() {
     Vec* F[4] = {&uE, &uW, &uN, &uS};
     for (Vec* f : F)
+#pragma omp parallel for schedule(static)
       for (size_t i = 0; i < P; i++) {
         Vec& a = *f;
         if (a[K(0) + i] <= H00) {
@@ -598,6 +607,7 @@ struct Orc {
       }
     Vec* G[4] = {&uE, &uW, &uS, &uN};
     for (Vec* f : G)
+#pragma omp parallel for schedule(static)
       for (size_t i = 0; i < P; i++) {
         Vec& a = *f;
         a[K(6) + i] = a[K(0) + i] / (a[K(0) + i] + a[K(3) + i] + H0);
@@ -607,6 +617,7 @@ struct Orc {
   void faceFluxes() {
     const double Drtio1 = 1.0 - rr;
     const double epsilon0 = 1.0;
+#pragma omp parallel for schedule(static)
     for (int row = 0; row < nyd - 1; row++)
       for (int col = 0; col < nxd - 1; col++) {
         const size_t i = I(row, col);
@@ -736,8 +747,17 @@ struct Orc {
   // reduceKernel L/K:1977-2004 (fmaxf: the max is rounded to float) and CFL3Kernel
   // L/K:2006-2020 / I/K:2050-2063
   void cflReduce() {
+    // fmaxf over all entries; per-thread maxima combined with fmaxf give the same value in
+    // any order (NaN is ignored unless every entry is NaN)
     float m = 0.0f;
-    for (size_t i = 0; i < P; i++) m = std::fmax(m, (float)Val[i]);
+#pragma omp parallel
+    {
+      float t = 0.0f;
+#pragma omp for schedule(static) nowait
+      for (size_t i = 0; i < P; i++) t = std::fmax(t, (float)Val[i]);
+#pragma omp critical
+      m = std::fmax(m, t);
+    }
     dmax = (double)m;
     dtval = cfl / dmax;
     dtval = std::fmin(dthi, dtval);
@@ -747,6 +767,7 @@ struct Orc {
   void numFluxes() {
     const double epsilon0 = 1.0;
     const double Dratio = rr;
+#pragma omp parallel for schedule(static)
     for (int row = 1; row < nyd - 1; row++)
       for (int col = 1; col < nxd - 1; col++) {
         const size_t i = I(row, col);
@@ -790,6 +811,7 @@ struct Orc {
         dudyN[i] = (u4[n] / (u3[n] + H0) - u4[i] / (u3[i] + H0)) / dy;
         dvdyN[i] = (u5[n] / (u3[n] + H0) - u5[i] / (u3[i] + H0)) / dy;
       }
+#pragma omp parallel for schedule(static)
     for (int row = 1; row < nyd - 1; row++)
       for (int col = 1; col < nxd - 1; col++) {
         const size_t i = I(row, col);
@@ -813,6 +835,7 @@ struct Orc {
           dvetdety[i] = b21 * dudyN[i] + b22 * dvdyN[i] - b23 * (b31 * dudyN[i] + b32 * dvdyN[i]) / b33;
         }
       }
+#pragma omp parallel for schedule(static)
     for (int row = 1; row < nyd - 1; row++)
       for (int col = 1; col < nxd - 1; col++) {
         const size_t i = I(row, col);
@@ -839,6 +862,7 @@ struct Orc {
     const double Dratio = rr;
     const double epsilon0 = 1.0;
     const int lo = (mode == 0) ? 1 : 0;
+#pragma omp parallel for schedule(static)
     for (int row = lo; row < nyd - 1; row++)
       for (int col = lo; col < nxd - 1; col++) {
         const size_t i = I(row, col);
@@ -904,6 +928,7 @@ struct Orc {
   // Flux18..21Kernel L/K:2623-2726, range [3, N-4]^2
   void update() {
     const double epsilon0 = 1.0;
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = 0; k < MN - 1; k++)
       for (int row = 3; row < nyd - MD; row++)
         for (int col = 3; col < nxd - MD; col++) {
@@ -917,12 +942,14 @@ struct Orc {
                       (dt / dy) * (PDy[kk + i] - PDy[kk + sN]) * (epsilon0 * (1.0 - phi) / N_R) +
                       (dt)*s[kk + i];
         }
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = 0; k < MN - 1; k++)  // Flux19
       for (int row = 3; row < nyd - MD; row++)
         for (int col = 3; col < nxd - MD; col++) {
           const size_t i = I(row, col);
           u[K(k) + i] = v[K(k) + i] / Detmin[i];
         }
+#pragma omp parallel for schedule(static)
     for (int row = 3; row < nyd - MD; row++)  // Flux20
       for (int col = 3; col < nxd - MD; col++) {
         const size_t i = I(row, col);
@@ -937,6 +964,7 @@ struct Orc {
           u[K(5) + i] = 0.0;
         }
       }
+#pragma omp parallel for schedule(static)
     for (int row = 3; row < nyd - MD; row++)  // Flux21
       for (int col = 3; col < nxd - MD; col++) {
         const size_t i = I(row, col);
@@ -948,9 +976,11 @@ struct Orc {
   void crossing(int io) {
     Vec& dst = (io == 0) ? uone : utwo;
     const Vec& old = (io == 0) ? uzero : uone;
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = 0; k < MN; k++)
       for (int row = 3; row < nyd - MD; row++)
         for (int col = 3; col < nxd - MD; col++) dst[K(k) + I(row, col)] = u[K(k) + I(row, col)];
+#pragma omp parallel for schedule(static)
     for (int row = 3; row < nyd - MD; row++)
       for (int col = 3; col < nxd - MD; col++) {
         const size_t i = I(row, col);
@@ -963,6 +993,7 @@ struct Orc {
         usyold[i] = old[K(2) + i] / (old[K(0) + i] + H0);
         ufyold[i] = old[K(5) + i] / (old[K(3) + i] + H0);
       }
+#pragma omp parallel for schedule(static)
     for (int row = 3; row < nyd - MD; row++)
       for (int col = 3; col < nxd - MD; col++) {
         const size_t i = I(row, col);
@@ -977,6 +1008,7 @@ struct Orc {
         dst[K(6) + i] = u[K(6) + i];
       }
     if (io == 1)
+#pragma omp parallel for collapse(2) schedule(static)
       for (int k = 0; k < MN; k++)
         for (int row = 3; row < nyd - MD; row++)
           for (int col = 3; col < nxd - MD; col++) u[K(k) + I(row, col)] = utwo[K(k) + I(row, col)];
@@ -1216,6 +1248,13 @@ int orc_get(void* h, const char* name, double* out) {
 void orc_set_u(void* h, const double* in) {
   Orc* o = (Orc*)h;
   std::memcpy(o->u.data(), in, o->u.size() * sizeof(double));
+}
+
+// Threads of the OpenMP loops (0: leave the runtime's default).  Results do not depend on
+// it: every parallel loop writes only its own cells and the only reduction is a max.
+int orc_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
 }
 
 double orc_scalar(void* h, const char* name) {

@@ -50,8 +50,15 @@ def lib():
         L.orc_inject.argtypes = [ctypes.c_void_p]
         L.orc_nxd.argtypes = [ctypes.c_void_p]
         L.orc_nyd.argtypes = [ctypes.c_void_p]
+        L.orc_threads.argtypes = [ctypes.c_int]
+        L.orc_threads.restype = ctypes.c_int
         _LIB = L
     return _LIB
+
+
+def threads(n: int = 0) -> int:
+    """Sets (n > 0) and returns the oracle's OpenMP thread count."""
+    return lib().orc_threads(int(n))
 
 
 def _dp(a):

@@ -155,6 +155,41 @@ def test_c1_workload_parity_256():
     run_pair(sc, 100, [100], "c1-256")
 
 
+def _threads():
+    from oracle import oracle as O
+    import os
+    return O.threads(int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
+
+
+def test_c1_workload_parity_full_2000_steps():
+    """The 256^2 landslide configuration for its full 2000 steps, bit for bit (the oracle
+    runs its loops on all host threads; results do not depend on the thread count)."""
+    say("oracle threads", _threads())
+    sc = S.landslide_plane(n=256, phi=0.6)
+    sc.cfl = 0.4
+    sc.rho_ratio = 1000.0 / 2700.0
+    ndiff = run_pair(sc, 2000, [500, 500, 1000], "c1-256-full")
+    assert ndiff == 0
+
+
+def test_c3_inflow_valley_full_size():
+    """The 1024 x 4096 inflow configuration (V-shaped valley, triangular hydrograph on the
+    upstream edge): 30 steps, bit for bit."""
+    _threads()
+    sc = S.inflow_valley()
+    assert (sc.nx, sc.ny) == (1024, 4096)
+    ndiff = run_pair(sc, 30, [1, 29], "c3-inflow")
+    assert ndiff == 0
+
+
+def test_c2_landslide_2048_full_size():
+    """The 2048^2 throughput configuration: 10 steps, bit for bit."""
+    _threads()
+    sc = S.landslide_fractal(n=2048, seed=2)
+    ndiff = run_pair(sc, 10, [2, 8], "c2-2048")
+    assert ndiff == 0
+
+
 def test_gpu_matches_committed_golden_states():
     """The HIP path against the oracle states frozen in tests/golden/oracle_golden.npz."""
     import os
