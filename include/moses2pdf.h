@@ -90,6 +90,12 @@ int m2p_open_grid(int mode, int NX, int NY, double cellsize, const double* topo,
  * *wall_s (nullable) receives the wall time of the call. */
 int m2p_run(m2p_ctx* ctx, const char* result_dir, double* wall_s);
 
+/* m2p_run for at most max_steps steps from the context's current state, so a long run can
+ * be split into segments (with m2p_checkpoint / m2p_restore in between).  *finished
+ * (nullable) is 1 once the last output time has been written. */
+int m2p_run_steps(m2p_ctx* ctx, const char* result_dir, int max_steps, int* finished,
+                  double* wall_s);
+
 /* Advances `nsteps` full time steps (predictor + corrector) with no output clipping and no
  * host synchronisation inside the loop.  No reference counterpart as a call: it is the
  * body of the kernelStep loop (adNOC_2Pb_runKernels.cu:646-1330) without the file output. */
@@ -137,6 +143,22 @@ const char* m2p_last_error(const m2p_ctx* ctx);
 
 /* Closes the context (removing it from its batch first) and frees its device memory. */
 void m2p_close(m2p_ctx* ctx);
+
+/* ---- Checkpoint / resume (no reference counterpart: its text outputs are lossy) ---- */
+
+/* Writes the scenario's resumable state to `path` (binary, written to path.tmp and then
+ * renamed): a header (magic "M2PCKPT1", version 1, mode, NX, NY, precision, output-time
+ * count, P), the device loop state (time, dt, output and inflow counters, step count,
+ * halting rules), the Time.dat history of m2p_run, the 7 state planes of u^n and of the
+ * half-step buffer (its ghost rows feed the next step's landslide corners), the FP32
+ * limiter flags, and an FNV-1a checksum.  Stepping a context restored from it gives
+ * bit-identical results to never having stopped. */
+int m2p_checkpoint(m2p_ctx* ctx, const char* path);
+
+/* Loads a checkpoint into a context opened on the same scenario (mode, grid, precision
+ * and output times must match; M2P_EINVAL otherwise, with the reason in
+ * m2p_last_error).  m2p_run then continues the output sequence where it stopped. */
+int m2p_restore(m2p_ctx* ctx, const char* path);
 
 /* ---- Halting rules and status (no reference counterpart) -------------------------- */
 

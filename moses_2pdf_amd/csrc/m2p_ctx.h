@@ -28,6 +28,7 @@ struct m2p_ctx {
   m2p_params par{};
   std::vector<double> topo, depth, phi;  // padded, scaled (x10)
   std::vector<double> tf;                // output times (scaled), tf[0] = 0
+  std::vector<double> outtime;           // Time.dat entries written so far (m2p_run)
   int nsamp = 0, nloc = 0;
   std::vector<double> table, loc, itime;
   std::vector<int> dire;

@@ -8,6 +8,7 @@
 #include <cstddef>
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
 #include <cstring>
 #include <map>
 #include <string>
@@ -564,7 +565,14 @@ int m2p_sync(m2p_ctx* c) {
   return 0;
 }
 
-int m2p_run(m2p_ctx* c, const char* result_dir, double* wall_s) {
+}  // extern "C"
+
+namespace {
+
+// The time loop of RunKernels::run (L/R:306-356, 646-1330) with its output, from the
+// context's current state, for at most max_steps steps.  *finished: the last output time
+// was reached.
+int run_loop(m2p_ctx* c, const char* result_dir, int max_steps, double* wall_s, int* finished) {
   if (!c || !c->s || !result_dir) return M2P_ESTATE;
   IN_BATCH(c);
   const auto t0 = std::chrono::steady_clock::now();
@@ -587,12 +595,16 @@ int m2p_run(m2p_ctx* c, const char* result_dir, double* wall_s) {
     if (!e.empty()) return fail(c, M2P_EIO, e);
   }
   const int ntf = (int)c->tf.size();
-  std::vector<double> outtime(ntf + 1, 0.0);
+  std::vector<double>& outtime = c->outtime;   // kept in the context: a checkpoint carries it
+  outtime.resize(ntf + 1, 0.0);
   std::vector<double> u(7 * P);
   DevState st{};
+  HIPCHK(c, hipMemcpyAsync(&st, c->dSt, sizeof st, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(c, hipStreamSynchronize(c->s));
   int nt;
-  bool stop = (ntf <= 1);
-  for (nt = 1; !stop && nt < 100000; nt++) {
+  bool stop = (ntf <= 1) || st.iter >= ntf;   // a resumed run may already be complete
+  const int nt_end = st.nt + max_steps;
+  for (nt = st.nt + 1; !stop && nt < 100000 && nt <= nt_end; nt++) {
     enqueue_step(c, ntf);
     HIPCHK(c, hipMemcpyAsync(&st, c->dSt, sizeof st, hipMemcpyDeviceToHost, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
@@ -612,12 +624,30 @@ int m2p_run(m2p_ctx* c, const char* result_dir, double* wall_s) {
     std::fflush(stdout);
   }
   HIPCHK(c, hipStreamSynchronize(c->s));
-  std::printf("\nTotal time : %g sec  ", st.time / 10);
-  std::printf("\nTotal number of steps: %d\n", nt);
+  if (finished) *finished = stop ? 1 : 0;
+  if (stop || nt >= 100000) {
+    std::printf("\nTotal time : %g sec  ", st.time / 10);
+    std::printf("\nTotal number of steps: %d\n", nt);
+  }
   if (wall_s)
     *wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return 0;
 }
+
+}  // namespace
+
+extern "C" {
+
+int m2p_run(m2p_ctx* c, const char* result_dir, double* wall_s) {
+  return run_loop(c, result_dir, 100000, wall_s, nullptr);
+}
+
+int m2p_run_steps(m2p_ctx* c, const char* result_dir, int max_steps, int* finished, double* wall_s) {
+  if (c && max_steps < 0) return fail(c, M2P_EINVAL, "negative step count");
+  return run_loop(c, result_dir, max_steps, wall_s, finished);
+}
+
+
 
 int m2p_get_state(m2p_ctx* c, double* u, double* t, int* nt) {
   if (!c || !c->s) return M2P_ESTATE;
@@ -786,6 +816,116 @@ int m2p_footprint(int mode, int NX, int NY, int precision, int nsamp, int nloc,
   const long long tmp = 3 * P * 8 + (precision == M2P_FP32 ? 7 * P * 8 : 0);
   if (steady) *steady = b;
   if (peak) *peak = b + tmp;
+  return 0;
+}
+
+// ---- checkpoint / resume ------------------------------------------------------------
+
+namespace {
+
+constexpr char kMagic[8] = {'M', '2', 'P', 'C', 'K', 'P', 'T', '1'};
+
+uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
+  const unsigned char* b = (const unsigned char*)p;
+  for (size_t i = 0; i < n; i++) {
+    h ^= b[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+struct CkptHeader {
+  char magic[8];
+  int32_t version, mode, NX, NY, prec, ntf;
+  int64_t P;
+  uint32_t devstate_bytes, lim_bytes_per_cell;
+};
+
+}  // namespace
+
+int m2p_checkpoint(m2p_ctx* c, const char* path) {
+  if (!c || !c->s) return M2P_ESTATE;
+  if (!path) return fail(c, M2P_EINVAL, "null path");
+  hipStream_t s = strm(c);
+  const size_t P = (size_t)c->P, nst = 7 * P * c->es;
+  std::vector<char> u0(nst), u1(nst), lim(c->dLIM ? P : 0);
+  DevState st{};
+  HIPCHK(c, hipMemcpyAsync(&st, c->dSt, sizeof st, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(u0.data(), c->dU0, nst, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(u1.data(), c->dU1, nst, hipMemcpyDeviceToHost, s));
+  if (c->dLIM) HIPCHK(c, hipMemcpyAsync(lim.data(), c->dLIM, P, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  CkptHeader h{};
+  std::memcpy(h.magic, kMagic, 8);
+  h.version = 1;
+  h.mode = c->mode;
+  h.NX = c->NX;
+  h.NY = c->NY;
+  h.prec = c->prec;
+  h.ntf = (int32_t)c->tf.size();
+  h.P = c->P;
+  h.devstate_bytes = sizeof(DevState);
+  h.lim_bytes_per_cell = c->dLIM ? 1 : 0;
+  std::vector<double> ot(c->outtime);
+  ot.resize(c->tf.size() + 1, 0.0);
+  const std::string tmp = std::string(path) + ".tmp";
+  FILE* f = std::fopen(tmp.c_str(), "wb");
+  if (!f) return fail(c, M2P_EIO, "can't write " + tmp);
+  uint64_t hsum = 1469598103934665603ull;
+  auto put = [&](const void* p, size_t n) {
+    hsum = fnv1a(hsum, p, n);
+    return std::fwrite(p, 1, n, f) == n;
+  };
+  bool ok = put(&h, sizeof h) && put(&st, sizeof st) && put(ot.data(), ot.size() * sizeof(double)) &&
+            put(u0.data(), nst) && put(u1.data(), nst) && (lim.empty() || put(lim.data(), P));
+  ok = ok && std::fwrite(&hsum, 1, sizeof hsum, f) == sizeof hsum;
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok || std::rename(tmp.c_str(), path) != 0) {
+    std::remove(tmp.c_str());
+    return fail(c, M2P_EIO, std::string("can't write ") + path);
+  }
+  return 0;
+}
+
+int m2p_restore(m2p_ctx* c, const char* path) {
+  if (!c || !c->s) return M2P_ESTATE;
+  IN_BATCH(c);
+  if (!path) return fail(c, M2P_EINVAL, "null path");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(c, M2P_EIO, std::string("can't open ") + path);
+  uint64_t hsum = 1469598103934665603ull;
+  auto get = [&](void* p, size_t n) {
+    if (std::fread(p, 1, n, f) != n) return false;
+    hsum = fnv1a(hsum, p, n);
+    return true;
+  };
+  CkptHeader h{};
+  DevState st{};
+  const size_t P = (size_t)c->P, nst = 7 * P * c->es;
+  std::string why;
+  if (!get(&h, sizeof h) || std::memcmp(h.magic, kMagic, 8) != 0 || h.version != 1)
+    why = "not a checkpoint of this format";
+  else if (h.mode != c->mode || h.NX != c->NX || h.NY != c->NY || h.prec != c->prec ||
+           h.P != c->P || h.ntf != (int32_t)c->tf.size() || h.devstate_bytes != sizeof(DevState) ||
+           h.lim_bytes_per_cell != (c->dLIM ? 1u : 0u))
+    why = "checkpoint does not match this scenario (mode, grid, precision or output times)";
+  std::vector<double> ot(c->tf.size() + 1);
+  std::vector<char> u0(nst), u1(nst), lim(c->dLIM ? P : 0);
+  if (why.empty() && !(get(&st, sizeof st) && get(ot.data(), ot.size() * sizeof(double)) &&
+                       get(u0.data(), nst) && get(u1.data(), nst) && (lim.empty() || get(lim.data(), P))))
+    why = "checkpoint is truncated";
+  uint64_t stored = 0;
+  if (why.empty() && (std::fread(&stored, 1, sizeof stored, f) != sizeof stored || stored != hsum))
+    why = "checkpoint checksum mismatch";
+  std::fclose(f);
+  if (!why.empty()) return fail(c, M2P_EINVAL, why + ": " + path);
+  HIPCHK(c, hipMemcpyAsync(c->dSt, &st, sizeof st, hipMemcpyHostToDevice, c->s));
+  HIPCHK(c, hipMemcpyAsync(c->dU0, u0.data(), nst, hipMemcpyHostToDevice, c->s));
+  HIPCHK(c, hipMemcpyAsync(c->dU1, u1.data(), nst, hipMemcpyHostToDevice, c->s));
+  if (c->dLIM) HIPCHK(c, hipMemcpyAsync(c->dLIM, lim.data(), P, hipMemcpyHostToDevice, c->s));
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  c->outtime = ot;
+  c->vstop_on = st.vstop2 > 0;
   return 0;
 }
 

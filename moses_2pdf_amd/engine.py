@@ -27,7 +27,8 @@ PRECISIONS = {"fp64": M2P_FP64, "fp32": M2P_FP32}
 EXPORTS = ["m2p_default_params", "m2p_open", "m2p_open_grid", "m2p_run", "m2p_step", "m2p_run_to", "m2p_sync",
            "m2p_get_state", "m2p_set_state", "m2p_dims", "m2p_mass", "m2p_scalars",
            "m2p_get_field", "m2p_profile", "m2p_profile_get", "m2p_last_error", "m2p_close",
-           "m2p_set_stop", "m2p_status", "m2p_footprint",
+           "m2p_set_stop", "m2p_status", "m2p_footprint", "m2p_checkpoint", "m2p_restore",
+           "m2p_run_steps",
            "m2p_batch_create", "m2p_batch_add", "m2p_batch_remove", "m2p_batch_size",
            "m2p_batch_step", "m2p_batch_sync", "m2p_batch_tune", "m2p_batch_set", "m2p_batch_get",
            "m2p_batch_last_error", "m2p_batch_destroy"]
@@ -80,6 +81,9 @@ def lib():
     L.m2p_close.restype = None
     llp = ctypes.POINTER(ctypes.c_longlong)
     L.m2p_set_stop.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    L.m2p_checkpoint.argtypes = [vp, ctypes.c_char_p]
+    L.m2p_restore.argtypes = [vp, ctypes.c_char_p]
+    L.m2p_run_steps.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, ip, dp]
     L.m2p_status.argtypes = [vp, dp, ip, ip]
     L.m2p_footprint.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 ctypes.c_int, llp, llp]
@@ -251,6 +255,19 @@ class Simulation:
         w = ctypes.c_double()
         self._check(lib().m2p_run(self._h, result_dir.encode(), ctypes.byref(w)), "m2p_run")
         return w.value
+
+    def run_steps(self, result_dir: str, max_steps: int) -> bool:
+        """m2p_run for at most max_steps steps; True once the last output is written."""
+        fin, w = ctypes.c_int(), ctypes.c_double()
+        self._check(lib().m2p_run_steps(self._h, result_dir.encode(), int(max_steps), ctypes.byref(fin),
+                                        ctypes.byref(w)), "m2p_run_steps")
+        return bool(fin.value)
+
+    def checkpoint(self, path: str) -> None:
+        self._check(lib().m2p_checkpoint(self._h, path.encode()), "m2p_checkpoint")
+
+    def restore(self, path: str) -> None:
+        self._check(lib().m2p_restore(self._h, path.encode()), "m2p_restore")
 
     def close(self) -> None:
         if getattr(self, "_h", None):

@@ -63,6 +63,7 @@ int main(int argc, char** argv) {
     m2p::OutputGrid og{nx, ny, nxd, nyd};
     std::vector<double> t = {0.0, 30.25, 60.5};
     std::string e = m2p::write_result(argv[2], 7, og, u.data());
+    if (e.empty()) e = m2p::write_result(argv[2], 1234, og, u.data());   // past 999
     if (e.empty()) e = m2p::write_time(argv[2], t, 3);
     if (e.empty()) e = m2p::write_info(argv[2], 185, 111, 200.0, 200.0, 212862.0625, 2561457.5, 61);
     std::vector<double> depth(P);

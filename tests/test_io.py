@@ -131,6 +131,8 @@ def test_result_writers_byte_exact(io_check, tmp_path):
     u[:P][idx % 5 == 0] = 0.0
     u[3 * P:4 * P][idx % 7 == 0] = 1e-6
     assert (tmp_path / "007.dat").read_text() == expected_result(u, nx, ny)
+    # %03d numbering (L/R:1273) keeps counting past 999
+    assert (tmp_path / "1234.dat").read_text() == expected_result(u, nx, ny)
     assert (tmp_path / "Time.dat").read_text() == "%20.4f%20.4f%20.4f" % (0.0, 30.25, 60.5)
     assert (tmp_path / "Info.dat").read_text() == (
         'VARIABLES = "x-point", "y-point", "dx", "dy", "xllcorner", "yllcorner", "TotalStep"\n '
