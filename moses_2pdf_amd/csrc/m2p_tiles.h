@@ -420,15 +420,21 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
   const int xmax = g.nx + MD - 1, ymax = g.ny + MD - 1;
   const long long P = g.P;
 
-  // x faces (r0+w, c0-1+lane), faces (r, c)-(r, c+1)
+  // x faces (r0+w, c0-1+lane), faces (r, c)-(r, c+1).  Only tiles in the first tile column
+  // hold the overridden face at column 2; the others form their face states in registers
+  // and skip the LDS round trip (the branch is uniform over the block).
   {
     const int r = r0 + w, c = c0 - 1 + lane;
     const bool has = (w < TY && r <= ymax && c <= xmax);
-    if (has) stage_states<T, TY>(F, w, lane, Uin, g, 0, r, c);
-    __syncthreads();
     T L[6], R[6];
-    if (has) load_states<T, TY>(F, w, lane, c == 2, 0, L, R);
-    __syncthreads();
+    if (tx == 0) {
+      if (has) stage_states<T, TY>(F, w, lane, Uin, g, 0, r, c);
+      __syncthreads();
+      if (has) load_states<T, TY>(F, w, lane, c == 2, 0, L, R);
+      __syncthreads();
+    } else if (has) {
+      base_states(Uin, g, 0, r, c, L, R);
+    }
     if (has) flux_store<T, TY>(F, w, lane, Uin, M, g, ph, 0, r, c, L, R);
   }
   __syncthreads();
@@ -454,15 +460,20 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
   }
   __syncthreads();
 
-  // y faces (r0-1+w, c0+lane), faces (r, c)-(r+1, c)
+  // y faces (r0-1+w, c0+lane), faces (r, c)-(r+1, c); the overridden face row 2 lies in
+  // the first tile row only
   {
     const int rf = r0 - 1 + w, cf = c0 + lane;
     const bool has = (lane < ST_TX && rf <= ymax && cf <= xmax);
-    if (has) stage_states<T, TY>(F, w, lane, Uin, g, 1, rf, cf);
-    __syncthreads();
     T L[6], R[6];
-    if (has) load_states<T, TY>(F, w, lane, rf == 2, 1, L, R);
-    __syncthreads();
+    if (ty == 0) {
+      if (has) stage_states<T, TY>(F, w, lane, Uin, g, 1, rf, cf);
+      __syncthreads();
+      if (has) load_states<T, TY>(F, w, lane, rf == 2, 1, L, R);
+      __syncthreads();
+    } else if (has) {
+      base_states(Uin, g, 1, rf, cf, L, R);
+    }
     if (has) flux_store<T, TY>(F, w, lane, Uin, M, g, ph, 1, rf, cf, L, R);
   }
   __syncthreads();
