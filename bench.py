@@ -33,10 +33,11 @@ ELEM = {"fp64": 8, "fp32": 4}
 ALG_BYTES = {k: v * 8 for k, v in PLANES.items()}
 
 
-def traffic_from_profiles(kernel: str, n: int):
+def traffic_from_profiles(kernel: str, n: int, precision: str = "fp64"):
     """HBM bytes per launch of `kernel` at grid n, from the committed rocprofv3 PMC summary
-    (profiles/hbm_traffic_<n>.txt lines 'kernel: bytes_per_launch'), or None."""
-    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{n}.txt")
+    (profiles/hbm_traffic[_fp32]_<n>.txt lines 'kernel: bytes_per_launch'), or None."""
+    tag = "" if precision == "fp64" else f"{precision}_"
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{tag}{n}.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -251,7 +252,7 @@ def landslide(args) -> int:
     avg_ms = kern[dom][0]
     alg = alg_bytes[dom] * cells
     achieved = alg / (avg_ms * 1e-3) / 1e9
-    traffic = traffic_from_profiles(dom, args.n) if args.precision == "fp64" else None
+    traffic = traffic_from_profiles(dom, args.n, args.precision)
     step_alg = sum(alg_bytes.values()) * cells
     out = {
         "metric": "cell-updates/s",
@@ -272,7 +273,11 @@ def landslide(args) -> int:
                    "grid": [sc.nx, sc.ny], "cfl": sc.cfl, "precision": args.precision},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "alg_bytes_per_cell": alg_bytes[dom]},
+                     "alg_bytes_per_cell": alg_bytes[dom],
+                     "measured_GBs": (traffic / (avg_ms * 1e-3) / 1e9) if traffic else None,
+                     "note": "the step is bound by VALU issue and latency, not HBM: every FP64 "
+                             "divide and sqrt stays IEEE (bit parity); VALU issue share and wait "
+                             "share per kernel in profiles/valu_counters_2048.txt"},
         "step_hbm_alg_GBs": step_alg * args.steps / elapsed / 1e9,
         "kernels_ms": {k: v[0] for k, v in kern.items()},
         "state_finite": ok,
