@@ -20,7 +20,7 @@
 namespace m2p {
 
 static const StageVariant kVar64[] = {{7, 4}, {3, 4}, {7, 3}};
-static const StageVariant kVar32[] = {{7, 6}, {7, 4}, {3, 6}, {15, 4}};
+static const StageVariant kVar32[] = {{7, 6}, {7, 4}, {3, 6}, {15, 4}, {3, 8}, {7, 8}};
 
 int stage_variants(int prec, const StageVariant** out) {
   if (prec == M2P_FP32) {

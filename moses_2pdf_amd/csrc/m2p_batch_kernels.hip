@@ -128,6 +128,8 @@ static bool launch_stage_variant(const BatchDev& B, int ty, int w, int io, hipSt
     else if (ty == 7 && w == 4) stage_pair<float, 7, 4>(B, io, s);
     else if (ty == 3 && w == 6) stage_pair<float, 3, 6>(B, io, s);
     else if (ty == 15 && w == 4) stage_pair<float, 15, 4>(B, io, s);
+    else if (ty == 3 && w == 8) stage_pair<float, 3, 8>(B, io, s);
+    else if (ty == 7 && w == 8) stage_pair<float, 7, 8>(B, io, s);
     else return false;
   }
   return true;

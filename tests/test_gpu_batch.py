@@ -80,7 +80,7 @@ def test_every_variant_and_schedule_gives_the_same_result(precision):
     b0 = Batch(precision)
     from moses_2pdf_amd import engine
     import ctypes
-    variants = {"fp64": [(7, 4), (3, 4), (7, 3)], "fp32": [(7, 6), (7, 4), (3, 6), (15, 4)]}[precision]
+    variants = {"fp64": [(7, 4), (3, 4), (7, 3)], "fp32": [(7, 6), (7, 4), (3, 6), (15, 4), (3, 8), (7, 8)]}[precision]
     for ty, w in variants:
         for remap in (0, 1):
             for order in (0, 1):
