@@ -92,13 +92,17 @@ def ensemble(args) -> int:
     # whose FP64 run went non-finite are left out: the reference scheme itself fails there)
     fin64 = [bool(np.isfinite(r[0]).all()) for r in ref]
     keep = [k for k in range(len(scs)) if fin64[k]]
-    acc = E.run_batch([scs[k] for k in keep], 20 * steps, "fp32", tune_steps=0,
+    # (a member whose FP32 dt collapses to its floor cannot reach the FP64 time within 4x the
+    # steps: it is counted as stalled, not compared)
+    acc = E.run_batch([scs[k] for k in keep], 4 * steps, "fp32", tune_steps=0,
                       t_end=[ref[k][1] for k in keep])
     fin32 = [bool(np.isfinite(o[0]).all()) for o in acc["out"]]
-    both = [j for j in range(len(keep)) if fin32[j]]
+    reached = [acc["out"][j][1] == ref[keep[j]][1] for j in range(len(keep))]
+    both = [j for j in range(len(keep)) if fin32[j] and reached[j]]
     err, l1 = E.errors([acc["out"][j][0] for j in both], [ref[keep[j]][0] for j in both])
     err["members_nonfinite_fp64"] = len(scs) - len(keep)
-    err["members_nonfinite_fp32"] = len(keep) - len(both)
+    err["members_nonfinite_fp32"] = len(keep) - sum(fin32)
+    err["members_stalled_fp32"] = sum(1 for j in range(len(keep)) if fin32[j] and not reached[j])
     ok = [keep[j] for j in both]
     fidx, fl1 = E.floor_l1([scs[k] for k in ok], steps, [ref[k] for k in ok], count=args.floor_members)
     err["floor_l1_median_subset"] = float(np.median(fl1))
