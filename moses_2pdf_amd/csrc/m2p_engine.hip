@@ -120,7 +120,9 @@ int setup(m2p_ctx* c) {
   int rc;
   double* dU64 = nullptr;  // FP64 initial state (the metric setup writes it)
   if ((rc = dalloc(c, &c->dTopo, P)) || (rc = dalloc(c, &c->dDepth, P)) ||
-      (rc = dalloc(c, &c->dM, (size_t)m2p::MC_COUNT * P)) || (rc = dalloc(c, &dU64, 7 * P)) ||
+      // FP32 keeps only the FP64 cell planes; its face planes go straight into the float copy
+      (rc = dalloc(c, &c->dM, (size_t)(c->prec == M2P_FP64 ? m2p::MC_COUNT : m2p::MC_CELL_END) * P)) ||
+      (rc = dalloc(c, &dU64, 7 * P)) ||
       (rc = dalloc(c, &c->dTf, c->tf.size())) || (rc = dalloc(c, &c->dMass, 4 * (size_t)m2p::mass_blocks())) ||
       (rc = dalloc(c, &c->dSt, 1))) {
     if (dU64) (void)hipFree(dU64);
@@ -179,7 +181,10 @@ int setup(m2p_ctx* c) {
                            dU64, c->s);
   if (c->prec == M2P_FP32) {
     m2p::launch_convert(dU64, (float*)c->dU0, (long long)(7 * P), c->s);
-    m2p::launch_convert(c->dM, (float*)c->dMT, (long long)(m2p::MC_COUNT * P), c->s);
+    m2p::launch_convert(c->dM, (float*)c->dMT, (long long)(m2p::MC_CELL_END * P), c->s);
+    m2p::launch_face_metric(c->g, c->dM, (float*)c->dMT + m2p::MC_FX * P, c->s);
+  } else {
+    m2p::launch_face_metric(c->g, c->dM, c->dM + m2p::MC_FX * P, c->s);
   }
   HIPCHK(c, hipGetLastError());
   // the first predictor's corner ghosts come from the initial state's own ghost rows
@@ -733,7 +738,17 @@ int m2p_get_field(m2p_ctx* c, const char* name, double* out) {
       {"dxdxi11", {m2p::MC_X11, 1}}, {"dxdxi12", {m2p::MC_X12, 1}}, {"dxdxi21", {m2p::MC_X21, 1}},
       {"dxdxi22", {m2p::MC_X22, 1}}, {"J13dxi", {m2p::MC_J13X, 1}}, {"J23dxi", {m2p::MC_J23X, 1}},
       {"J33dxi", {m2p::MC_J33X, 1}}, {"J13det", {m2p::MC_J13E, 1}}, {"J23det", {m2p::MC_J23E, 1}},
-      {"J33det", {m2p::MC_J33E, 1}}};
+      {"J33det", {m2p::MC_J33E, 1}},
+      {"invJ11_avgEW", {m2p::MC_FX + m2p::FQ_I11, 1}}, {"invJ12_avgEW", {m2p::MC_FX + m2p::FQ_I12, 1}},
+      {"invJ13_avgEW", {m2p::MC_FX + m2p::FQ_I13, 1}}, {"invJ21_avgEW", {m2p::MC_FX + m2p::FQ_I21, 1}},
+      {"invJ22_avgEW", {m2p::MC_FX + m2p::FQ_I22, 1}}, {"invJ23_avgEW", {m2p::MC_FX + m2p::FQ_I23, 1}},
+      {"Detmin_avgEW", {m2p::MC_FX + m2p::FQ_DET, 1}}, {"dxdxi11_avgEW", {m2p::MC_FX + m2p::FQ_XA, 1}},
+      {"dxdxi21_avgEW", {m2p::MC_FX + m2p::FQ_XB, 1}},
+      {"invJ11_avgSN", {m2p::MC_FY + m2p::FQ_I11, 1}}, {"invJ12_avgSN", {m2p::MC_FY + m2p::FQ_I12, 1}},
+      {"invJ13_avgSN", {m2p::MC_FY + m2p::FQ_I13, 1}}, {"invJ21_avgSN", {m2p::MC_FY + m2p::FQ_I21, 1}},
+      {"invJ22_avgSN", {m2p::MC_FY + m2p::FQ_I22, 1}}, {"invJ23_avgSN", {m2p::MC_FY + m2p::FQ_I23, 1}},
+      {"Detmin_avgSN", {m2p::MC_FY + m2p::FQ_DET, 1}}, {"dxdxi12_avgSN", {m2p::MC_FY + m2p::FQ_XA, 1}},
+      {"dxdxi22_avgSN", {m2p::MC_FY + m2p::FQ_XB, 1}}};
   const size_t P = (size_t)c->P;
   std::string n(name);
   if (n == "uone45") {
@@ -742,6 +757,8 @@ int m2p_get_field(m2p_ctx* c, const char* name, double* out) {
   }
   auto it = tab.find(n);
   if (it == tab.end()) return fail(c, M2P_EINVAL, "unknown field " + n);
+  if (c->prec == M2P_FP32 && it->second.first >= m2p::MC_CELL_END)
+    return fail(c, M2P_EINVAL, "the FP32 variant keeps face averages only as floats: " + n);
   HIPCHK(c, hipStreamSynchronize(strm(c)));
   HIPCHK(c, hipMemcpy(out, c->dM + (size_t)it->second.first * P,
                       (size_t)it->second.second * P * sizeof(double), hipMemcpyDeviceToHost));
@@ -805,12 +822,13 @@ int m2p_footprint(int mode, int NX, int NY, int precision, int nsamp, int nloc,
                   long long* steady, long long* peak) {
   if (NX < 2 || NY < 2 || (precision != M2P_FP64 && precision != M2P_FP32)) return M2P_EINVAL;
   const long long P = (long long)(NX + 2 * MD) * (NY + 2 * MD);
-  long long b = (long long)m2p::MC_COUNT * P * 8 + 4LL * m2p::mass_blocks() * 8 + (long long)sizeof(DevState) + 4096;
+  const long long full = (long long)m2p::MC_COUNT * P * 8, cells = (long long)m2p::MC_CELL_END * P * 8;
+  long long b = 4LL * m2p::mass_blocks() * 8 + (long long)sizeof(DevState) + 4096;
   b += (P / 256 + P / 441 + 4LL * (NX + NY) + 64) * 8;   // per-block partial maxima
   if (precision == M2P_FP64)
-    b += 16 * P * 8;
+    b += full + 16 * P * 8;
   else
-    b += 16 * P * 4 + (long long)m2p::MC_COUNT * P * 4 + P;
+    b += cells + 16 * P * 4 + (long long)m2p::MC_COUNT * P * 4 + P;
   if (mode == 1) b += (3LL * nsamp * nloc + 3LL * nloc + nsamp) * 8 + 3LL * nloc * 4;
   // set-up temporaries: topography, depth, per-cell phi, and the FP64 initial state (FP32)
   const long long tmp = 3 * P * 8 + (precision == M2P_FP32 ? 7 * P * 8 : 0);

@@ -44,6 +44,10 @@ struct DevParams {
   int nsamp;   // hydrograph samples (0: landslide)
 };
 
+// The 18 face-average planes (MC_FX..MC_COUNT of the metric layout) into F, from the cell
+// planes of M; F is M + MC_FX * P in FP64, the float copy's face planes in FP32.
+template <typename TO>
+void launch_face_metric(const Grid& g, const double* M, TO* F, hipStream_t s);
 void launch_metric_setup(const Grid& g, int mode, const double* topo, const double* depth,
                          const double* phicell, double phiS0, double* M, double* U,
                          hipStream_t s);

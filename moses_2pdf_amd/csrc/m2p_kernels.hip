@@ -133,6 +133,25 @@ __global__ void k_ghost_copy_y(Grid g, double* __restrict__ M, int p0, int p1) {
 }
 
 // JacobKernel (L/K:660-701) on [3, N-4]^2, after the svec/cvalue ghost copies.
+// MeanKernel (L/K:1121-1224): the face averages the flux code reads, once, on [1, N-3]^2,
+// formed in FP64 from the cell planes of M and stored (rounded to TO) into the 18 face
+// planes at F.
+template <typename TO>
+__global__ void k_face_metric(Grid g, const double* __restrict__ M, TO* __restrict__ F) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int r = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  if (r > g.nyd - 3 || c > g.nxd - 3) return;
+  const long long P = g.P, i = idx(g, r, c), e = i + 1, n = i + g.nxd;
+  const int cellq[FQ_COUNT] = {MC_I11, MC_I12, MC_I13, MC_I21, MC_I22, MC_I23, MC_DET, 0, 0};
+#pragma unroll
+  for (int q = 0; q < FQ_COUNT; q++) {
+    const double* ax = M + (q == FQ_XA ? MC_X11 : q == FQ_XB ? MC_X21 : cellq[q]) * P;
+    const double* ay = M + (q == FQ_XA ? MC_X12 : q == FQ_XB ? MC_X22 : cellq[q]) * P;
+    F[q * P + i] = (TO)(0.5 * (ax[i] + ax[e]));
+    F[(FQ_COUNT + q) * P + i] = (TO)(0.5 * (ay[i] + ay[n]));
+  }
+}
+
 __global__ void k_jacob(Grid g, double* __restrict__ M) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x + 3;
   const int r = blockIdx.y * blockDim.y + threadIdx.y + 3;
@@ -318,9 +337,17 @@ void launch_metric_setup(const Grid& g, int mode, const double* topo, const doub
   k_ghost_copy_x<<<dim3((g.nyd + 63) / 64, MC_J13X), 64, 0, s>>>(g, M, 0, MC_J13X);
   k_ghost_copy_y<<<dim3((g.nxd + 63) / 64, MC_J13X), 64, 0, s>>>(g, M, 0, MC_J13X);
   k_jacob<<<grid2(g.nxd - 6, g.nyd - 6, b), b, 0, s>>>(g, M);
-  k_ghost_copy_x<<<dim3((g.nyd + 63) / 64, MC_COUNT - MC_J13X), 64, 0, s>>>(g, M, MC_J13X, MC_COUNT);
-  k_ghost_copy_y<<<dim3((g.nxd + 63) / 64, MC_COUNT - MC_J13X), 64, 0, s>>>(g, M, MC_J13X, MC_COUNT);
+  k_ghost_copy_x<<<dim3((g.nyd + 63) / 64, MC_CELL_END - MC_J13X), 64, 0, s>>>(g, M, MC_J13X, MC_CELL_END);
+  k_ghost_copy_y<<<dim3((g.nxd + 63) / 64, MC_CELL_END - MC_J13X), 64, 0, s>>>(g, M, MC_J13X, MC_CELL_END);
 }
+
+template <typename TO>
+void launch_face_metric(const Grid& g, const double* M, TO* F, hipStream_t s) {
+  const dim3 b(64, 4);
+  k_face_metric<TO><<<grid2(g.nxd - 3, g.nyd - 3, b), b, 0, s>>>(g, M, F);
+}
+template void launch_face_metric<double>(const Grid&, const double*, double*, hipStream_t);
+template void launch_face_metric<float>(const Grid&, const double*, float*, hipStream_t);
 
 template <typename T>
 void launch_convert(const double* a, T* b, long long n, hipStream_t s) {

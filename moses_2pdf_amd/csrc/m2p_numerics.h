@@ -21,15 +21,22 @@ constexpr double H0 = 1.0e-6;     // L/KH:11
 constexpr double H00 = 0.0;       // L/KH:12
 constexpr double U0 = 1.0e-6;     // L/KH:13
 
-// Metric planes kept on the device (cell centred, padded nxd*nyd each).  invJ31, invJ32
-// and invJ33 are exactly -svec0, -svec1 and cvalue (L/K:244-248), so they are not stored.
+// Metric planes kept on the device (padded nxd*nyd each).  Cell-centred planes first:
+// invJ31, invJ32 and invJ33 are exactly -svec0, -svec1 and cvalue (L/K:244-248), so they
+// are not stored.  Then the face averages of MeanKernel (L/K:1121-1224) for x-faces
+// (r,c)-(r,c+1) and y-faces (r,c)-(r+1,c), stored at (r,c) for faces in [1, N-3]^2, in
+// the order of FaceQ (the x-face's XA/XB are dxdxi11/dxdxi21, the y-face's dxdxi12/22).
 enum MetricPlane {
   MC_C = 0, MC_S0, MC_S1, MC_DET,
   MC_I11, MC_I12, MC_I13, MC_I21, MC_I22, MC_I23,
   MC_X11, MC_X12, MC_X21, MC_X22,
   MC_J13X, MC_J23X, MC_J33X, MC_J13E, MC_J23E, MC_J33E,
-  MC_COUNT
+  MC_CELL_END,
+  MC_FX = MC_CELL_END,           // 9 x-face planes
+  MC_FY = MC_FX + 9,             // 9 y-face planes
+  MC_COUNT = MC_FY + 9
 };
+enum FaceQ { FQ_I11 = 0, FQ_I12, FQ_I13, FQ_I21, FQ_I22, FQ_I23, FQ_DET, FQ_XA, FQ_XB, FQ_COUNT };
 
 struct Grid {
   int nx, ny, nxd, nyd;
@@ -150,8 +157,8 @@ __device__ __forceinline__ T keep_positive(T a[6]) {
   return qdiv(a[0], a[0] + a[3] + T(H0));
 }
 
-// Face-averaged metric of MeanKernel (L/K:1121-1224) plus the cell values Flux2Kernel
-// reads on either side (L/K:1506-1536).
+// Face-averaged metric of MeanKernel (L/K:1121-1224, read from the stored face planes) plus
+// the cell values Flux2Kernel reads on either side (L/K:1506-1536).
 template <typename T>
 struct FaceMet {
   T j11, j12, j13, j21, j22, j23, j31, j32, j33;  // invJ*_avg
@@ -167,8 +174,10 @@ __device__ __forceinline__ T avg(const T* a, long long i, long long j) {
 }
 
 // dir 0: face (r,c)-(r,c+1); dir 1: face (r,c)-(r+1,c).  Valid for faces in [1,N-3]^2,
-// where MeanKernel writes the averages.
-template <typename T>
+// where MeanKernel writes the averages.  STORED reads the face planes; otherwise the
+// averages are formed here from the cell planes in T (the FP32 stage does this: fewer
+// bytes per cell, and its float averages differ from the rounded FP64 ones by an ulp).
+template <typename T, bool STORED = true>
 __device__ __forceinline__ void face_metric(const T* __restrict__ M, const Grid& g, int dir, int r,
                                             int c, FaceMet<T>& f) {
   const long long i = idx(g, r, c);
@@ -177,12 +186,28 @@ __device__ __forceinline__ void face_metric(const T* __restrict__ M, const Grid&
   const T* C = M + MC_C * P;
   const T* S0 = M + MC_S0 * P;
   const T* S1 = M + MC_S1 * P;
-  f.j11 = avg(M + MC_I11 * P, i, j);
-  f.j12 = avg(M + MC_I12 * P, i, j);
-  f.j13 = avg(M + MC_I13 * P, i, j);
-  f.j21 = avg(M + MC_I21 * P, i, j);
-  f.j22 = avg(M + MC_I22 * P, i, j);
-  f.j23 = avg(M + MC_I23 * P, i, j);
+  const T* Fq = M + (dir == 0 ? MC_FX : MC_FY) * P;   // the stored face averages
+  if constexpr (STORED) {
+    f.j11 = Fq[FQ_I11 * P + i];
+    f.j12 = Fq[FQ_I12 * P + i];
+    f.j13 = Fq[FQ_I13 * P + i];
+    f.j21 = Fq[FQ_I21 * P + i];
+    f.j22 = Fq[FQ_I22 * P + i];
+    f.j23 = Fq[FQ_I23 * P + i];
+    f.det = Fq[FQ_DET * P + i];
+    f.xa = Fq[FQ_XA * P + i];
+    f.xb = Fq[FQ_XB * P + i];
+  } else {
+    f.j11 = avg(M + MC_I11 * P, i, j);
+    f.j12 = avg(M + MC_I12 * P, i, j);
+    f.j13 = avg(M + MC_I13 * P, i, j);
+    f.j21 = avg(M + MC_I21 * P, i, j);
+    f.j22 = avg(M + MC_I22 * P, i, j);
+    f.j23 = avg(M + MC_I23 * P, i, j);
+    f.det = avg(M + MC_DET * P, i, j);
+    f.xa = avg(M + (dir == 0 ? MC_X11 : MC_X12) * P, i, j);
+    f.xb = avg(M + (dir == 0 ? MC_X21 : MC_X22) * P, i, j);
+  }
   f.sL0 = S0[i];
   f.sL1 = S1[i];
   f.cL = C[i];
@@ -193,14 +218,6 @@ __device__ __forceinline__ void face_metric(const T* __restrict__ M, const Grid&
   f.j32 = T(0.5) * ((-f.sL1) + (-f.sR1));
   f.j33 = T(0.5) * (f.cL + f.cR);
   f.cav = f.j33;
-  f.det = avg(M + MC_DET * P, i, j);
-  if (dir == 0) {
-    f.xa = avg(M + MC_X11 * P, i, j);
-    f.xb = avg(M + MC_X21 * P, i, j);
-  } else {
-    f.xa = avg(M + MC_X12 * P, i, j);
-    f.xb = avg(M + MC_X22 * P, i, j);
-  }
 }
 
 // Wave speeds of one face (Flux1..6Kernel, L/K:1426-1848): em for the solid (emS) and the

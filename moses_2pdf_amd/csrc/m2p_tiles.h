@@ -384,7 +384,7 @@ __device__ __forceinline__ void flux_store(T (*Tl)[TY + 1][64], int w, int lane,
                                            const Grid& g, const Phys& ph, int dir, int r, int c,
                                            T L[6], T R[6]) {
   FaceMet<T> f;
-  face_metric(M, g, dir, r, c, f);
+  face_metric<T, std::is_same<T, double>::value>(M, g, dir, r, c, f);
   FaceSpeed<T> sp;
   face_speed(dir, L, R, f, ph, sp);
   const long long i = idx(g, r, c);

@@ -1234,7 +1234,25 @@ int orc_get(void* h, const char* name, double* out) {
                  {"dbdy", &o->dbdy},
                  {"v", &o->v},
                  {"apEW", &o->apEW},
-                 {"apSN", &o->apSN}};
+                 {"apSN", &o->apSN},
+                 {"invJ11_avgEW", &o->invJ11_avgEW},
+                 {"invJ12_avgEW", &o->invJ12_avgEW},
+                 {"invJ13_avgEW", &o->invJ13_avgEW},
+                 {"invJ21_avgEW", &o->invJ21_avgEW},
+                 {"invJ22_avgEW", &o->invJ22_avgEW},
+                 {"invJ23_avgEW", &o->invJ23_avgEW},
+                 {"Detmin_avgEW", &o->Detmin_avgEW},
+                 {"dxdxi11_avgEW", &o->dxdxi11_avgEW},
+                 {"dxdxi21_avgEW", &o->dxdxi21_avgEW},
+                 {"invJ11_avgSN", &o->invJ11_avgSN},
+                 {"invJ12_avgSN", &o->invJ12_avgSN},
+                 {"invJ13_avgSN", &o->invJ13_avgSN},
+                 {"invJ21_avgSN", &o->invJ21_avgSN},
+                 {"invJ22_avgSN", &o->invJ22_avgSN},
+                 {"invJ23_avgSN", &o->invJ23_avgSN},
+                 {"Detmin_avgSN", &o->Detmin_avgSN},
+                 {"dxdxi12_avgSN", &o->dxdxi12_avgSN},
+                 {"dxdxi22_avgSN", &o->dxdxi22_avgSN}};
   for (auto& t : tab)
     if (std::strcmp(t.n, name) == 0) {
       std::memcpy(out, t.v->data(), t.v->size() * sizeof(double));

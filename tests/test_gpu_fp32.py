@@ -3,8 +3,10 @@
 The FP32 run takes its own dt sequence and is stepped to the FP64 run's final time with
 m2p_run_to (its last step shortened to land on it).  Stated tolerances:
   - short horizon (30 steps; 10 for the inflow case, whose hydrograph sample switches on
-    a time test): L1 final-depth error <= 1e-4, mass totals within 1e-6 relative;
-  - 1000 steps: L1 <= 2 * floor + 1e-2 and runout-extent IoU >= 0.98, where `floor` is
+    a time test): both runs are stepped to a time half-way through the FP64 run's last
+    step, so both take the same number of steps, the last one shortened, whatever the
+    float rounding of dt; L1 final-depth error <= 1e-4, mass totals within 1e-6 relative;
+  - 1000 steps (to a time half-way through the FP64 run's step 1000): L1 <= 2 * floor + 1e-2 and runout-extent IoU >= 0.98, where `floor` is
     the L1 distance between the FP64 run and an FP64 run whose initial state carries a
     1e-7 relative perturbation (the size of a float rounding).  The scheme's switches
     (velocity-crossing limiter, Coulomb friction near rest, dry clipping) amplify such
@@ -39,10 +41,20 @@ CASES = {"member128": lambda: member(128), "member256": lambda: member(256),
          "hsiaolin": hsiaolin, "df004": df004}
 
 
-def fp64_run(sc, steps):
+def mid_step_time(sc, steps):
+    """A time half-way through the FP64 run's step number `steps`."""
     with Simulation(sc) as a:
-        a.step(steps)
-        return a.state(), a.mass(), a.time_steps()[0]
+        a.step(steps - 1)
+        t0 = a.time_steps()[0]
+        a.step(1)
+        return 0.5 * (t0 + a.time_steps()[0])
+
+
+def fp64_to(sc, t_end, max_steps):
+    with Simulation(sc) as a:
+        n = a.run_to(t_end, max_steps)
+        assert a.time_steps()[0] == t_end
+        return a.state(), a.mass(), n
 
 
 def fp32_to(sc, t_end, max_steps):
@@ -67,11 +79,13 @@ def test_fp32_initial_state_is_rounded_fp64():
 def test_fp32_short_horizon(case):
     sc = CASES[case]()
     steps = 10 if case == "df004" else 30
-    u64, m64, t = fp64_run(sc, steps)
-    u32, m32, n = fp32_to(sc, t, 20 * steps)
+    t = mid_step_time(sc, steps)
+    u64, m64, n64 = fp64_to(sc, t, 2 * steps)
+    u32, m32, n = fp32_to(sc, t, 2 * steps)
     e = depth_error(u32, u64)
-    say(f"{case}: {steps} steps (FP32 {n}) L1 {e['l1']:.2e} Linf {e['linf']:.2e} "
+    say(f"{case}: {n64} steps (FP32 {n}) L1 {e['l1']:.2e} Linf {e['linf']:.2e} "
         f"mass {mass_rel(m32, m64):.2e}")
+    assert n64 == steps and n == steps
     assert np.isfinite(u32).all()
     assert e["l1"] <= 1e-4, e
     assert mass_rel(m32, m64) <= 1e-6
@@ -81,7 +95,8 @@ def test_fp32_short_horizon(case):
 def test_fp32_within_conditioning_floor(case):
     sc = CASES[case]()
     steps = 1000
-    u64, m64, t = fp64_run(sc, steps)
+    t = mid_step_time(sc, steps)
+    u64, m64, _ = fp64_to(sc, t, 2 * steps)
     with Simulation(sc) as p:   # the perturbed FP64 run, to the same time
         u = p.state()
         p.set_state(u * (1 + 1e-7 * np.random.default_rng(1).standard_normal(u.shape)))

@@ -22,6 +22,10 @@ TOL = 1e-10
 METRIC = ["cvalue", "Detmin", "invJ11", "invJ12", "invJ13", "invJ21", "invJ22", "invJ23",
           "dxdxi11", "dxdxi12", "dxdxi21", "dxdxi22", "J13dxi", "J23dxi", "J33dxi", "J13det",
           "J23det", "J33det"]
+# the stored face averages (MeanKernel)
+METRIC += [f"{q}_avg{d}" for d in ("EW", "SN")
+           for q in ("invJ11", "invJ12", "invJ13", "invJ21", "invJ22", "invJ23", "Detmin")]
+METRIC += ["dxdxi11_avgEW", "dxdxi21_avgEW", "dxdxi12_avgSN", "dxdxi22_avgSN"]
 
 
 def say(*a):

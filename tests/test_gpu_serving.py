@@ -49,4 +49,30 @@ def test_footprint_grows_with_grid_and_precision():
     assert p64 >= a64 and p32 >= a32
     assert b64 > 3.5 * a64
     P = 262 * 262
-    assert a64 >= (20 + 16) * 8 * P   # metric + state planes
+    assert a64 >= (38 + 16) * 8 * P   # metric (cell and face planes) + state planes
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_footprint_matches_device_memory(prec):
+    import ctypes
+    from moses_2pdf_amd import Simulation
+    from moses_2pdf_amd import scenarios as S
+    hip = ctypes.CDLL("libamdhip64.so.7")   # the HIP runtime the library itself uses
+
+    def free_bytes():
+        f, t = ctypes.c_size_t(), ctypes.c_size_t()
+        assert hip.hipMemGetInfo(ctypes.byref(f), ctypes.byref(t)) == 0
+        return f.value
+
+    with Simulation(S.landslide_plane(n=16)):
+        pass                              # the runtime is initialised before the first reading
+    free0 = free_bytes()
+    with Simulation(S.landslide_plane(n=1024), precision=prec):
+        free1 = free_bytes()
+    steady, peak = footprint(0, 1024, 1024, prec)
+    held = free0 - free1
+    print(prec, "held", held, "steady", steady, "peak", peak, flush=True)
+    assert peak >= steady
+    # the HIP runtime may keep the freed set-up temporaries for the process to reuse, so
+    # the device can show anything between the steady and the peak figure as held
+    assert 0.95 * steady <= held <= 1.05 * peak + (8 << 20)
