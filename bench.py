@@ -279,9 +279,11 @@ def landslide(args) -> int:
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "alg_bytes_per_cell": alg_bytes[dom],
                      "measured_GBs": (traffic / (avg_ms * 1e-3) / 1e9) if traffic else None,
-                     "note": "the step is bound by VALU issue and latency, not HBM: every FP64 "
-                             "divide and sqrt stays IEEE (bit parity); VALU issue share and wait "
-                             "share per kernel in profiles/valu_counters_2048.txt"},
+                     "note": ("the step is bound by VALU issue and latency, not HBM: every FP64 "
+                              "divide and sqrt stays IEEE (bit parity)" if args.precision == "fp64" else
+                              "the step is bound by VALU issue (about 65% issue share), not HBM")
+                             + "; VALU issue share and wait share per kernel in "
+                               "profiles/valu_counters_2048.txt"},
         "step_hbm_alg_GBs": step_alg * args.steps / elapsed / 1e9,
         "kernels_ms": {k: v[0] for k, v in kern.items()},
         "state_finite": ok,
