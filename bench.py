@@ -84,10 +84,11 @@ def ensemble(args) -> int:
     cells = sum(sc.nx * sc.ny for sc in scs)
     steps = args.ens_steps
     # plain FP64 path: each scenario alone, one after another
-    t_plain, ref = E.run_plain(scs, steps, "fp64")
+    dry = not args.dense
+    t_plain, ref = E.run_plain(scs, steps, "fp64", dry_skip=dry)
     # batched FP64 and FP32 (autotuned, graph-captured)
-    b64 = E.run_batch(scs, steps, "fp64", keep=False)
-    b32 = E.run_batch(scs, steps, "fp32", keep=False)
+    b64 = E.run_batch(scs, steps, "fp64", keep=False, dry_skip=dry)
+    b32 = E.run_batch(scs, steps, "fp32", keep=False, dry_skip=dry)
     # FP32 error at equal time: each member stepped to its FP64 run's final time (members
     # whose FP64 run went non-finite are left out: the reference scheme itself fails there)
     fin64 = [bool(np.isfinite(r[0]).all()) for r in ref]
@@ -127,7 +128,7 @@ def ensemble(args) -> int:
                                "(self-affine relief), sides 128/256/512/1024 with weights 4:3:2:1, "
                                "LHS basal friction 15-30 deg, volume 1e5-5e6 m^3, phi 0.4-0.7, "
                                f"{steps} steps each, per-scenario dt",
-                   "members": n, "cells": cells, "steps": steps},
+                   "members": n, "cells": cells, "steps": steps, "dry_skip": dry},
         "speedup_vs_plain_fp64": t_plain / b32["wall_s"],
         "plain_fp64": {"scenarios_per_hour": per_h(t_plain), "wall_s": t_plain},
         "batched_fp64": {"scenarios_per_hour": per_h(b64["wall_s"]), "wall_s": b64["wall_s"],
@@ -151,8 +152,9 @@ def serve(args) -> int:
     V_log = lambda *a: print("#", *a, file=sys.stderr, flush=True)
     V_log(f"{len(reqs)} requests built in {time.perf_counter() - t0:.1f} s "
           f"({sum(r.mode for r in reqs)} inflow)")
+    dry = not args.dense
     srv = V.Server(precision="fp32", hbm_budget=args.hbm_budget * 1e9, tick_steps=args.tick,
-                   max_steps=args.max_steps)
+                   max_steps=args.max_steps, dry_skip=dry)
     t1 = time.perf_counter()
     res = srv.run(reqs)
     wall = time.perf_counter() - t1
@@ -160,7 +162,8 @@ def serve(args) -> int:
     cb.update({"peak_active": srv.peak_active, "peak_device_GB": srv.peak_held / 1e9})
     V_log("continuous batching:", cb)
     t1 = time.perf_counter()
-    res0 = V.serve_alone(reqs, precision="fp64", max_steps=args.max_steps, tick_steps=args.tick)
+    res0 = V.serve_alone(reqs, precision="fp64", max_steps=args.max_steps, tick_steps=args.tick,
+                         dry_skip=dry)
     wall0 = time.perf_counter() - t1
     plain = V.summary(res0, wall0, args.max_steps)
     V_log("plain fp64 one at a time:", plain)
@@ -173,7 +176,8 @@ def serve(args) -> int:
                                "lognormal (median 384, sigma 0.5) clipped to 128-2048, landslide/inflow "
                                f"70/30, stop at max flow speed < 0.01 m/s after motion or {args.max_steps} "
                                f"steps, admission by HBM footprint ({args.hbm_budget} GB budget)",
-                   "requests": len(reqs), "cells": cells, "tick_steps": args.tick},
+                   "requests": len(reqs), "cells": cells, "tick_steps": args.tick,
+                   "dry_skip": dry},
         "latency_p50_s": cb["latency_p50_s"], "latency_p99_s": cb["latency_p99_s"],
         "continuous_batching_fp32": cb, "plain_fp64_one_at_a_time": plain,
         "speedup_runs_per_hour": cb["runs_per_hour"] / plain["runs_per_hour"],
@@ -192,6 +196,10 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--precision", choices=sorted(ELEM), default="fp64",
                     help="fp64: the reference's arithmetic (the headline); fp32: FP32-bulk")
+    ap.add_argument("--dense", action="store_true",
+                    help="ensemble and serve: compute every tile (dry-tile skipping off) on both sides")
+    ap.add_argument("--no-dry-skip", action="store_true",
+                    help="leave out the second run of the landslide workload with dry-tile skipping")
     ap.add_argument("--copy-calib", action="store_true",
                     help="after the timed steps, copy the 7 state planes once device to device "
                          "(a plain copy kernel of known size, for HBM counter calibration)")
@@ -218,12 +226,37 @@ def main() -> int:
     return landslide(args)
 
 
+def dry_skip_run(sc, args, u_dense, dense_value) -> dict:
+    """The same workload with dry-tile skipping (m2p_params.dry_skip = 1); its final state
+    must equal the dense run's bit for bit."""
+    import numpy as np
+    from moses_2pdf_amd import Simulation
+    with Simulation(sc, precision=args.precision, dry_skip=True) as sim:
+        sim.step(args.warmup)
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.step(args.steps)
+        sim.sync()
+        el = time.perf_counter() - t0
+        u = sim.state()
+    v = sc.nx * sc.ny * args.steps / el
+    print(f"# dry-skip run: {args.steps} steps in {el:.3f} s", file=sys.stderr, flush=True)
+    return {"value": v, "unit": "cell-updates/s", "ms_per_step": el / args.steps * 1e3,
+            "speedup": v / dense_value, "identical_state": bool(np.array_equal(u, u_dense)),
+            "wet_cells": int(((u[0] > 0) | (u[3] > 0)).sum()),
+            "note": "same workload and steps with m2p_params.dry_skip = 1: tiles whose inputs "
+                    "are all zero are left out (exact); not the headline, which computes every "
+                    "tile as the reference does"}
+
+
 def landslide(args) -> int:
     from moses_2pdf_amd import Simulation
     from moses_2pdf_amd import scenarios as S
     sc = S.landslide_fractal(n=args.n, seed=2)
     print(f"# workload ready: {sc.nx}x{sc.ny}", file=sys.stderr, flush=True)
-    sim = Simulation(sc, precision=args.precision)
+    # the headline computes every tile, as the reference does; the dry-skipping rate of the
+    # same workload is reported beside it
+    sim = Simulation(sc, precision=args.precision, dry_skip=False)
     alg_bytes = {k: v * ELEM[args.precision] for k, v in PLANES.items()}
     sim.step(args.warmup)
     sim.sync()
@@ -289,6 +322,8 @@ def landslide(args) -> int:
         "state_finite": ok,
         "mass": [ms, mf],
     }
+    if not args.no_dry_skip:
+        out["dry_skip"] = dry_skip_run(sc, args, u, value)
     if not args.no_cpu_baseline:
         print("# cpu baseline (oracle) ...", file=sys.stderr, flush=True)
         del sim

@@ -58,7 +58,9 @@ typedef struct m2p_params {
   double dt_min;      /* reference 0.001 (landslide), 0.002 (inflow) */
   double dt_max;      /* reference 2.0 */
   int precision;      /* M2P_FP64 (default) or M2P_FP32 */
-  int reserved;       /* 0 */
+  int dry_skip;       /* 1 (default): tiles whose whole stencil is dry (every component of
+                       * every cell zero) are skipped; their results are exactly those of
+                       * computing them.  0: every tile is computed, as the reference does. */
 } m2p_params;
 
 /* Fills *p with the reference's values for `mode` (par_List values of the shipped
@@ -169,6 +171,11 @@ int m2p_restore(m2p_ctx* ctx, const char* path);
  * reached vstop), max_steps (< 0: none).  The kernels of a halted scenario return at
  * once, so it can stay in a batch or a captured graph.  Clears an earlier halt. */
 int m2p_set_stop(m2p_ctx* ctx, double t_end, double vstop, int max_steps);
+
+/* Switches dry-tile skipping (m2p_params.dry_skip) on (1) or off (0) from the next step.
+ * Results are the same either way; skipping only leaves out tiles whose inputs are all
+ * zero.  No reference counterpart. */
+int m2p_set_dry_skip(m2p_ctx* ctx, int on);
 
 /* Simulated time (scaled), steps done and whether the scenario has halted. */
 int m2p_status(m2p_ctx* ctx, double* t, int* nt, int* halted);

@@ -28,6 +28,7 @@ struct Member {
   const double* table;
   const double* loc;
   const int* dire;
+  DryMap dry;
   int ghost0, box0, ring0, stage0;   // first block of this member in each launch
   int box_n, ring_n, stage_n;        // and how many
 };

@@ -130,6 +130,7 @@ int rebuild(m2p_batch* b) {
     m.MT = c->dMT;
     m.M = c->dM;
     m.LIM = c->dLIM;
+    m.dry = c->dry;
     m.st = c->dSt;
     m.tf = c->dTf;
     m.itime = c->dItime;
@@ -238,7 +239,8 @@ struct Snapshot {
 };
 
 size_t snap_bytes(const m2p_ctx* c) {
-  return (16 * (size_t)c->P) * c->es + (c->dLIM ? (size_t)c->P : 0) + sizeof(DevState);
+  return (16 * (size_t)c->P) * c->es + (c->dLIM ? (size_t)c->P : 0) + sizeof(DevState) +
+         2 * (size_t)m2p::dry_flag_count(c->g);
 }
 
 int save(m2p_batch* b, Snapshot& sn, bool restore) {
@@ -256,7 +258,8 @@ int save(m2p_batch* b, Snapshot& sn, bool restore) {
       void* dev;
       size_t n;
     } parts[] = {{c->dU0, 7 * P * es}, {c->dU1, 7 * P * es}, {c->dONE, 2 * P * es},
-                 {c->dLIM, c->dLIM ? P : 0}, {c->dSt, sizeof(DevState)}};
+                 {c->dLIM, c->dLIM ? P : 0}, {c->dSt, sizeof(DevState)},
+                 {c->dZ, 2 * (size_t)m2p::dry_flag_count(c->g)}};
     for (auto& q : parts) {
       if (q.n == 0) continue;
       if (restore)

@@ -20,7 +20,7 @@ __global__ void kb_inflow(BatchDev B) {
   const Member& m = B.mem[blockIdx.x];
   if (threadIdx.x != 0 || m.mode != 1 || halted_pre(m.st)) return;
   inflow_body(m.g, (T*)m.U0, m.M, m.table, m.loc, m.dire, m.nloc, m.nsamp, m.phiS0,
-              m.st->inflowCount, m.LIM);
+              m.st->inflowCount, m.LIM, m.dry);
 }
 
 // post = 0: fill U0 before the CFL pass (previous-stage source U1); post = 1: fill U1
@@ -41,7 +41,8 @@ __global__ void __launch_bounds__(256) kb_cfl(BatchDev B) {
   const Member& m = B.mem[B.mapBox[blockIdx.x]];
   if (halted_pre(m.st)) return;
   const int lb = (int)blockIdx.x - m.box0;
-  block_max_store(cfl_box((const T*)m.U0, (const T*)m.MT, m.g, m.ph, lb % m.nbx, lb / m.nbx),
+  block_max_store(cfl_box((const T*)m.U0, (const T*)m.MT, m.g, m.ph, lb % m.nbx, lb / m.nbx,
+                          m.dry),
                   B.part + blockIdx.x);
 }
 
@@ -100,7 +101,7 @@ __global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per
   T* Uout = (T*)(IO == 0 ? m.U1 : m.U0);
   const double v2 = stage_tile<IO, T, TY>(F, m.g, m.ph, Uin, (const T*)m.U0, Uout, (T*)m.ONE,
                                           (const T*)m.MT, m.LIM,
-                                          IO == 0 ? st->dt_pred : st->dt_corr, tx, ty);
+                                          IO == 0 ? st->dt_pred : st->dt_corr, tx, ty, m.dry);
   if (IO == 1 && st->vstop2 > 0.0) block_max_store(v2, B.part + B.nBox + B.nRing + t);
 }
 

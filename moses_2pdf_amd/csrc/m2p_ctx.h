@@ -40,6 +40,8 @@ struct m2p_ctx {
   // the step kernels read (MT: dM itself in FP64, a float copy in FP32), in type `prec`
   void *dU0 = nullptr, *dU1 = nullptr, *dONE = nullptr, *dMT = nullptr;
   unsigned char* dLIM = nullptr;  // FP32 only: limiter memory flags, one byte per cell
+  unsigned char* dZ = nullptr;    // dry-tile flags of U0 and U1 (m2p::DryMap)
+  m2p::DryMap dry{};              // nseg 0 while dry-tile skipping is off
   double *dTf = nullptr, *dMass = nullptr;
   double* dPart = nullptr;  // per-block partial maxima of a step (m2p::partial_count)
   double *dTable = nullptr, *dLoc = nullptr, *dItime = nullptr;

@@ -170,6 +170,12 @@ int setup(m2p_ctx* c) {
   c->ph.varTheta = c->par.varTheta;
   c->ph.tande = std::tan(c->par.delta0 * 3.14159 / 180.);  // makeTopo11Kernel, L/K:298
   if ((rc = dalloc(c, &c->dPart, m2p::partial_count(c->g, c->prec)))) return rc;
+  const long long nz = m2p::dry_flag_count(c->g);
+  if ((rc = dalloc(c, &c->dZ, 2 * (size_t)nz))) return rc;
+  c->dry.z0 = c->dZ;
+  c->dry.z1 = c->dZ + nz;
+  c->dry.nseg = c->par.dry_skip ? m2p::dry_segments(c->g) : 0;
+  m2p::launch_dry_fill(c->g, c->dry, c->s);
 
   DevState st{};
   st.iter = 1;
@@ -217,6 +223,7 @@ int download_state(m2p_ctx* c, void* dsrc, size_t planes, double* u) {
 
 int upload_state(m2p_ctx* c, void* ddst, const double* u) {
   const size_t n = 7 * (size_t)c->P;
+  m2p::launch_dry_fill(c->g, c->dry, c->s);   // the flags no longer describe the state
   if (c->prec == M2P_FP64) {
     HIPCHK(c, hipMemcpyAsync(ddst, u, n * sizeof(double), hipMemcpyHostToDevice, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
@@ -299,7 +306,7 @@ void enqueue_step_t(m2p_ctx* c, int ntf) {
   if (c->mode == 1) {
     Timer t(c, "inflow");
     m2p::launch_inflow(c->g, U0, c->dM, c->dTable, c->dLoc, c->dDire, c->nloc, c->nsamp,
-                       c->par.phiS0, c->dSt, c->dLIM, c->s);
+                       c->par.phiS0, c->dSt, c->dLIM, c->dry, c->s);
   }
   {
     Timer t(c, "ghost");
@@ -307,7 +314,7 @@ void enqueue_step_t(m2p_ctx* c, int ntf) {
   }
   {
     Timer t(c, "cfl");
-    m2p::launch_cfl(c->g, c->ph, (const T*)U0, MT, c->dSt, c->dPart, c->s);
+    m2p::launch_cfl(c->g, c->ph, (const T*)U0, MT, c->dSt, c->dPart, c->dry, c->s);
   }
   {
     Timer t(c, "dt");
@@ -316,7 +323,7 @@ void enqueue_step_t(m2p_ctx* c, int ntf) {
   {
     Timer t(c, "stage0");
     m2p::launch_stage(0, c->g, c->ph, (const T*)U0, (const T*)U0, U1, ONE, MT, c->dLIM, c->dSt,
-                      c->dPart, c->s);
+                      c->dPart, c->dry, c->s);
   }
   {
     Timer t(c, "ghost");
@@ -325,7 +332,7 @@ void enqueue_step_t(m2p_ctx* c, int ntf) {
   {
     Timer t(c, "stage1");
     m2p::launch_stage(1, c->g, c->ph, (const T*)U1, (const T*)U0, U0, ONE, MT, c->dLIM, c->dSt,
-                      c->dPart, c->s);
+                      c->dPart, c->dry, c->s);
   }
   if (c->vstop_on) m2p::launch_vmax(c->dSt, c->g, c->prec, c->dPart, c->s);
 }
@@ -350,6 +357,11 @@ int parse_overlay(const char* ov, m2p_params& p) {
     size_t eq = kv.find('=');
     if (eq == std::string::npos) return -1;
     std::string k = kv.substr(0, eq), val = kv.substr(eq + 1);
+    if (k == "dry_skip") {
+      if (val == "0" || val == "1") p.dry_skip = val[0] - '0';
+      else return -1;
+      continue;
+    }
     if (k == "precision") {
       if (val == "fp64") p.precision = M2P_FP64;
       else if (val == "fp32") p.precision = M2P_FP32;
@@ -385,7 +397,7 @@ void m2p_default_params(int mode, m2p_params* p) {
   p->dt_min = mode == 0 ? 0.001 : 0.002;
   p->dt_max = 2.0;
   p->precision = M2P_FP64;
-  p->reserved = 0;
+  p->dry_skip = 1;
 }
 
 int m2p_open_grid(int mode, int NX, int NY, double cellsize, const double* topo,
@@ -399,6 +411,7 @@ int m2p_open_grid(int mode, int NX, int NY, double cellsize, const double* topo,
   if (mode == 0 && !depth) return M2P_EINVAL;
   if (mode == 1 && (nsamp < 1 || nloc < 1 || !t || !h || !u || !v || !loc)) return M2P_EINVAL;
   if (par->precision != M2P_FP64 && par->precision != M2P_FP32) return M2P_EINVAL;
+  if (par->dry_skip != 0 && par->dry_skip != 1) return M2P_EINVAL;
   m2p_ctx* c = new m2p_ctx();
   c->mode = mode;
   c->prec = par->precision;
@@ -806,6 +819,22 @@ int m2p_set_stop(m2p_ctx* c, double t_end, double vstop, int max_steps) {
   return 0;
 }
 
+int m2p_set_dry_skip(m2p_ctx* c, int on) {
+  if (!c || !c->s) return M2P_ESTATE;
+  if (on != 0 && on != 1) return fail(c, M2P_EINVAL, "dry_skip must be 0 or 1");
+  hipStream_t s = strm(c);
+  c->par.dry_skip = on;
+  c->dry.nseg = on ? m2p::dry_segments(c->g) : 0;
+  // while off, no kernel keeps the flags, so they start again from "maybe wet"
+  m2p::DryMap all = c->dry;
+  all.nseg = m2p::dry_segments(c->g);
+  m2p::launch_dry_fill(c->g, all, s);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(s));
+  if (c->batch) m2p_batch_mark_dirty(c->batch);
+  return 0;
+}
+
 int m2p_status(m2p_ctx* c, double* t, int* nt, int* halted) {
   if (!c || !c->s) return M2P_ESTATE;
   hipStream_t s = strm(c);
@@ -941,6 +970,7 @@ int m2p_restore(m2p_ctx* c, const char* path) {
   HIPCHK(c, hipMemcpyAsync(c->dU0, u0.data(), nst, hipMemcpyHostToDevice, c->s));
   HIPCHK(c, hipMemcpyAsync(c->dU1, u1.data(), nst, hipMemcpyHostToDevice, c->s));
   if (c->dLIM) HIPCHK(c, hipMemcpyAsync(c->dLIM, lim.data(), P, hipMemcpyHostToDevice, c->s));
+  m2p::launch_dry_fill(c->g, c->dry, c->s);
   HIPCHK(c, hipStreamSynchronize(c->s));
   c->outtime = ot;
   c->vstop_on = st.vstop2 > 0;

@@ -32,6 +32,25 @@ struct DevState {
   int v2_nt;          // step whose corrector v2 comes from (-1: none)
 };
 
+// Dry-tile skipping.  A stage or CFL tile whose every input cell is zero in every
+// component produces exactly +0 states, +0 limited momenta, a zero flow speed and the
+// 1e-10 wave-speed floor, so it can be skipped without changing a bit of the results.
+// Interior cells are grouped in row segments of 63 cells (the stage tile's width):
+// segment s of row r holds columns [MD + 63 s, MD + 63 s + 62].  z0[(r - MD) * nseg + s]
+// is 0 only if every component of every cell of that segment is zero in U0 (u^n after the
+// inflow), z1 likewise for U1 (the predictor's output); a nonzero flag promises nothing.
+// Every stage tile rewrites the flags of its rows, so after one step they are exact; any
+// other write to the state sets them all to 1.  nseg == 0 switches the skipping off.
+struct DryMap {
+  unsigned char* z0;
+  unsigned char* z1;
+  int nseg;
+};
+inline int dry_segments(const Grid& g) { return (g.nx + 62) / 63; }
+inline long long dry_flag_count(const Grid& g) { return (long long)g.ny * dry_segments(g); }
+// Sets every flag of both maps to 1 (nothing may be skipped in the next step).
+void launch_dry_fill(const Grid& g, const DryMap& dm, hipStream_t s);
+
 // A scenario is halted for the kernels that run before the dt kernel of a step
 // (inflow, ghost fill, CFL) once it has done halt_at steps, and for those after it once
 // the dt kernel has set `skip`.
@@ -56,7 +75,7 @@ void launch_convert(const double* a, T* b, long long n, hipStream_t s);
 template <typename T>
 void launch_inflow(const Grid& g, T* U, const double* M, const double* table, const double* loc,
                    const int* dire, int nloc, int nsamp, double phiS0, const DevState* st,
-                   unsigned char* LIM, hipStream_t s);
+                   unsigned char* LIM, const DryMap& dm, hipStream_t s);
 template <typename T>
 void launch_ghost(const Grid& g, int mode, T* Tg, const T* prev, const DevState* st, int post,
                   hipStream_t s);
@@ -65,7 +84,7 @@ void launch_ghost(const Grid& g, int mode, T* Tg, const T* prev, const DevState*
 size_t partial_count(const Grid& g, int prec);
 template <typename T>
 void launch_cfl(const Grid& g, const Phys& ph, const T* U, const T* M, DevState* st, double* part,
-                hipStream_t s);
+                const DryMap& dm, hipStream_t s);
 void launch_dt(const DevParams& dp, DevState* st, const double* tf, const double* itime,
                const Grid& g, int prec, const double* part, hipStream_t s);
 // After a corrector: the squared max flow speed of its tiles into DevState (stop speed).
@@ -73,7 +92,7 @@ void launch_vmax(DevState* st, const Grid& g, int prec, const double* part, hipS
 template <typename T>
 void launch_stage(int io, const Grid& g, const Phys& ph, const T* Uin, const T* Uzero, T* Uout,
                   T* ONE, const T* M, unsigned char* LIM, DevState* st, double* part,
-                  hipStream_t s);
+                  const DryMap& dm, hipStream_t s);
 int mass_blocks();
 template <typename T>
 void launch_mass(const Grid& g, const T* U, const double* M, double* out, hipStream_t s);

@@ -178,9 +178,16 @@ template <typename T>
 __global__ void k_inflow(Grid g, T* __restrict__ U, const double* __restrict__ M,
                          const double* __restrict__ table, const double* __restrict__ loc,
                          const int* __restrict__ dire, int nloc, int nsamp, double phiS0,
-                         const DevState* __restrict__ st, unsigned char* __restrict__ LIM) {
+                         const DevState* __restrict__ st, unsigned char* __restrict__ LIM,
+                         DryMap dm) {
   if (blockIdx.x != 0 || threadIdx.x != 0 || halted_pre(st)) return;
-  inflow_body(g, U, M, table, loc, dire, nloc, nsamp, phiS0, st->inflowCount, LIM);
+  inflow_body(g, U, M, table, loc, dire, nloc, nsamp, phiS0, st->inflowCount, LIM, dm);
+}
+
+__global__ void k_dry_fill(unsigned char* __restrict__ z, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    z[i] = 1;
 }
 
 template <typename T>
@@ -198,10 +205,12 @@ constexpr int CFL_BLOCKS = 2048;
 template <typename T>
 __global__ void __launch_bounds__(256) k_cfl(Grid g, Phys ph, const T* __restrict__ U,
                                              const T* __restrict__ M, DevState* __restrict__ st,
-                                             double* __restrict__ part, int nbx, int nbox) {
+                                             double* __restrict__ part, int nbx, int nbox,
+                                             DryMap dm) {
   if (halted_pre(st)) return;
   double m = 0.0;
-  for (int t = blockIdx.x; t < nbox; t += gridDim.x) m = fmax(m, cfl_box(U, M, g, ph, t % nbx, t / nbx));
+  for (int t = blockIdx.x; t < nbox; t += gridDim.x)
+    m = fmax(m, cfl_box(U, M, g, ph, t % nbx, t / nbx, dm));
   block_max_store(m, part + blockIdx.x);
 }
 
@@ -250,7 +259,7 @@ template <int IO, typename T, int TY, int W>
 __global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per_eu(W, W)))
     k_stage(Grid g, Phys ph, const T* __restrict__ Uin, const T* Uzero, T* Uout,
             T* __restrict__ ONE, const T* __restrict__ M, unsigned char* __restrict__ LIM,
-            DevState* __restrict__ st, double* __restrict__ part) {
+            DevState* __restrict__ st, double* __restrict__ part, DryMap dm) {
   __shared__ T F[ST_NQ][TY + 1][64];
   if (halted_post(st)) return;
   const int b = blockIdx.x;
@@ -259,7 +268,7 @@ __global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per
   const int t = (b % N_XCD) * per + b / N_XCD;
   if (!tile_of(g, TY, t, tx, ty)) return;
   const double v2 = stage_tile<IO, T, TY>(F, g, ph, Uin, Uzero, Uout, ONE, M, LIM,
-                                          IO == 0 ? st->dt_pred : st->dt_corr, tx, ty);
+                                          IO == 0 ? st->dt_pred : st->dt_corr, tx, ty, dm);
   if (IO == 1 && st->vstop2 > 0.0) block_max_store(v2, part + t);
 }
 
@@ -358,8 +367,16 @@ template void launch_convert<float>(const double*, float*, long long, hipStream_
 template <typename T>
 void launch_inflow(const Grid& g, T* U, const double* M, const double* table, const double* loc,
                    const int* dire, int nloc, int nsamp, double phiS0, const DevState* st,
-                   unsigned char* LIM, hipStream_t s) {
-  k_inflow<T><<<1, 64, 0, s>>>(g, U, M, table, loc, dire, nloc, nsamp, phiS0, st, LIM);
+                   unsigned char* LIM, const DryMap& dm, hipStream_t s) {
+  k_inflow<T><<<1, 64, 0, s>>>(g, U, M, table, loc, dire, nloc, nsamp, phiS0, st, LIM, dm);
+}
+
+void launch_dry_fill(const Grid& g, const DryMap& dm, hipStream_t s) {
+  if (dm.nseg <= 0) return;
+  const long long n = dry_flag_count(g);
+  const unsigned nb = (unsigned)std::min<long long>((n + 255) / 256, 1024);
+  k_dry_fill<<<nb, 256, 0, s>>>(dm.z0, n);
+  k_dry_fill<<<nb, 256, 0, s>>>(dm.z1, n);
 }
 
 template <typename T>
@@ -380,9 +397,10 @@ size_t partial_count(const Grid& g, int prec) {
 
 template <typename T>
 void launch_cfl(const Grid& g, const Phys& ph, const T* U, const T* M, DevState* st, double* part,
-                hipStream_t s) {
+                const DryMap& dm, hipStream_t s) {
   const dim3 b(64, 4);
-  k_cfl<T><<<cfl_box_blocks(g), b, 0, s>>>(g, ph, U, M, st, part, (g.nx + 63) / 64, cfl_boxes(g));
+  k_cfl<T><<<cfl_box_blocks(g), b, 0, s>>>(g, ph, U, M, st, part, (g.nx + 63) / 64, cfl_boxes(g),
+                                          dm);
   k_cfl_ring<T><<<cfl_ring_blocks(g), 64, 0, s>>>(g, ph, U, M, st, part + cfl_box_blocks(g));
 }
 
@@ -402,15 +420,15 @@ void launch_vmax(DevState* st, const Grid& g, int prec, const double* part, hipS
 template <typename T>
 void launch_stage(int io, const Grid& g, const Phys& ph, const T* Uin, const T* Uzero, T* Uout,
                   T* ONE, const T* M, unsigned char* LIM, DevState* st, double* part,
-                  hipStream_t s) {
+                  const DryMap& dm, hipStream_t s) {
   constexpr int TY = PlainStage<T>::TY, W = PlainStage<T>::W;
   const dim3 b(64, TY + 1);
   const dim3 gr(((stage_tiles(g, TY) + N_XCD - 1) / N_XCD) * N_XCD);
   double* sp = part + cfl_box_blocks(g) + cfl_ring_blocks(g);
   if (io == 0)
-    k_stage<0, T, TY, W><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp);
+    k_stage<0, T, TY, W><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm);
   else
-    k_stage<1, T, TY, W><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp);
+    k_stage<1, T, TY, W><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm);
 }
 
 int mass_blocks() { return 512; }
@@ -423,13 +441,14 @@ void launch_mass(const Grid& g, const T* U, const double* M, double* out, hipStr
 #define M2P_INSTANTIATE(T)                                                                    \
   template void launch_inflow<T>(const Grid&, T*, const double*, const double*, const double*, \
                                  const int*, int, int, double, const DevState*,                \
-                                 unsigned char*, hipStream_t);                                 \
+                                 unsigned char*, const DryMap&, hipStream_t);                  \
   template void launch_ghost<T>(const Grid&, int, T*, const T*, const DevState*, int,         \
                                 hipStream_t);                                                 \
   template void launch_cfl<T>(const Grid&, const Phys&, const T*, const T*, DevState*, double*, \
-                              hipStream_t);                                                    \
+                              const DryMap&, hipStream_t);                                     \
   template void launch_stage<T>(int, const Grid&, const Phys&, const T*, const T*, T*, T*,     \
-                                const T*, unsigned char*, DevState*, double*, hipStream_t);    \
+                                const T*, unsigned char*, DevState*, double*, const DryMap&,   \
+                                hipStream_t);                                                  \
   template void launch_mass<T>(const Grid&, const T*, const double*, double*, hipStream_t);
 M2P_INSTANTIATE(double)
 M2P_INSTANTIATE(float)

@@ -23,7 +23,8 @@ template <typename T>
 __device__ __forceinline__ void inflow_body(const Grid& g, T* __restrict__ U, const double* __restrict__ M,
                                             const double* __restrict__ table, const double* __restrict__ loc,
                                             const int* __restrict__ dire, int nloc, int nsamp,
-                                            double phiS0, int ic, unsigned char* __restrict__ LIM) {
+                                            double phiS0, int ic, unsigned char* __restrict__ LIM,
+                                            const DryMap& dm) {
   const long long P = g.P;
   for (int i = 0; i < nloc; i++) {
     const int direX = dire[i * 3 + 1], direY = dire[i * 3 + 2];
@@ -51,6 +52,9 @@ __device__ __forceinline__ void inflow_body(const Grid& g, T* __restrict__ U, co
         U[5 * P + c] = (T)(Vin * (u3 + H0));
         U[6 * P + c] = (T)(u0 / (u0 + u3 + H0));
         if (LIM) LIM[c] = 0;
+        const int r = yloc + nn, cc = xloc + mm;
+        if (dm.nseg > 0 && r >= MD && r < g.ny + MD && cc >= MD && cc < g.nx + MD)
+          dm.z0[(long long)(r - MD) * dm.nseg + (cc - MD) / 63] = 1;
       }
   }
 }
@@ -220,10 +224,51 @@ __device__ __forceinline__ double block_max_of(const double* __restrict__ p, int
   return b;
 }
 
-// one 64x4 block of interior faces (both directions) starting at box column bx*64, row by*4
+// True when components 0..5 of every cell of U in rows [ra, rb] x columns [ca, cb] (clipped
+// to the padded grid) are zero: interior cells through the row-segment flags z, ghost cells
+// by reading them (the ghost fill that ran before the caller derived them from interior
+// cells, or for the lagging corners from the other buffer).  NaN counts as wet.  Every
+// thread of the block must call it; tid in [0, nthr).
+template <typename T>
+__device__ __forceinline__ bool dry_region(const T* __restrict__ U, const Grid& g,
+                                           const unsigned char* __restrict__ z, int nseg, int ra,
+                                           int rb, int ca, int cb, int tid, int nthr) {
+  ra = max(ra, 0);
+  rb = min(rb, g.nyd - 1);
+  ca = max(ca, 0);
+  cb = min(cb, g.nxd - 1);
+  const int xmax = g.nx + MD - 1, ymax = g.ny + MD - 1;
+  int wet = 0;
+  const int ia = max(ra, MD), ib = min(rb, ymax);
+  const int sa = (max(ca, MD) - MD) / 63, sb = (min(cb, xmax) - MD) / 63;
+  const int ns = sb - sa + 1, nf = (ib - ia + 1) * ns;
+  for (int e = tid; e < nf; e += nthr) wet |= z[(long long)(ia + e / ns - MD) * nseg + sa + e % ns];
+  const int nc = cb - ca + 1;
+  const int ncell = (ra >= MD && rb <= ymax && ca >= MD && cb <= xmax) ? 0 : (rb - ra + 1) * nc;
+  for (int e = tid; e < ncell; e += nthr) {
+    const int r = ra + e / nc, c = ca + e % nc;
+    if (r >= MD && r <= ymax && c >= MD && c <= xmax) continue;
+    const long long i = idx(g, r, c);
+#pragma unroll
+    for (int k = 0; k < 6; k++) wet |= (U[k * g.P + i] != T(0)) ? 1 : 0;
+  }
+  return !__syncthreads_or(wet);
+}
+
+// one 64x4 block of interior faces (both directions) starting at box column bx*64, row by*4.
+// With dry-tile skipping, a box whose stencil (columns c-1..c+2 of its x-faces, rows r-1..r+2
+// of its y-faces) is all zero returns what its faces would: the wave-speed floor 1e-10 over
+// dx and dy.  Every thread of the block must call it.
 template <typename T>
 __device__ __forceinline__ double cfl_box(const T* __restrict__ U, const T* __restrict__ M,
-                                          const Grid& g, const Phys& ph, int bx, int by) {
+                                          const Grid& g, const Phys& ph, int bx, int by,
+                                          const DryMap& dm) {
+  if (dm.nseg > 0) {
+    const int clo = MD + 64 * bx, rlo = MD + 4 * by;
+    if (dry_region(U, g, dm.z0, dm.nseg, rlo - 1, rlo + 5, clo - 1, clo + 65,
+                   threadIdx.y * blockDim.x + threadIdx.x, blockDim.x * blockDim.y))
+      return fmax((double)T(1.0e-10) / g.dx, (double)T(1.0e-10) / g.dy);
+  }
   const int c = bx * 64 + threadIdx.x + MD;
   const int r = by * 4 + threadIdx.y + MD;
   double m = 0.0;
@@ -412,13 +457,46 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
                                            const T* __restrict__ Uin, const T* Uzero, T* Uout,
                                            T* __restrict__ ONE, const T* __restrict__ M,
                                            unsigned char* __restrict__ LIM, double dt_d, int tx,
-                                           int ty) {
+                                           int ty, const DryMap& dm) {
   const int lane = threadIdx.x;
   const int w = threadIdx.y;
   const int c0 = tx * ST_TX + MD;
   const int r0 = ty * TY + MD;
   const int xmax = g.nx + MD - 1, ymax = g.ny + MD - 1;
   const long long P = g.P;
+
+  // Dry-tile skipping.  The tile reads cells [r0-2, r0+TY+1] x [c0-2, c0+64] of Uin (face
+  // stencils, the boundary overrides of edge tiles and the viscous term's tangential slopes)
+  // and its own cells of Uzero (and of ONE, which is zero where U1 is).  When all of them
+  // are zero, every output is +0 and the flow speed 0.
+  if (dm.nseg > 0) {
+    const int tid = w * 64 + lane;
+    const int rown = min(TY, ymax - r0 + 1);           // output rows of this tile
+    unsigned char* zown = IO == 0 ? dm.z1 : dm.z0;     // their flags
+    int own = 0;
+    if (tid < rown) own = zown[(long long)(r0 + tid - MD) * dm.nseg + tx];
+    // the corrector also needs uzero = U0 zero on its cells
+    const int own_wet = __syncthreads_or(own);
+    if (!(IO == 1 && own_wet) &&
+        dry_region(Uin, g, IO == 0 ? dm.z0 : dm.z1, dm.nseg, r0 - 2, r0 + TY + 1, c0 - 2,
+                   c0 + 64, tid, 64 * (TY + 1))) {
+      // Predictor: U1 and ONE of the tile's cells must read zero; skip the stores when the
+      // tile's U1 flags say they already do (ONE is zero wherever U1 is).  Corrector: U0
+      // is updated in place and already zero.
+      if (IO == 0 && own_wet) {
+        const int r = r0 + w, c = c0 + lane;
+        if (w < TY && lane < ST_TX && r <= ymax && c <= xmax) {
+          const long long i = idx(g, r, c);
+#pragma unroll
+          for (int k = 0; k < 7; k++) Uout[k * P + i] = T(0);
+          ONE[i] = T(0);
+          ONE[P + i] = T(0);
+        }
+        if (w < rown && lane == 0) zown[(long long)(r - MD) * dm.nseg + tx] = 0;
+      }
+      return 0.0;
+    }
+  }
 
   // x faces (r0+w, c0-1+lane), faces (r, c)-(r, c+1).  Only tiles in the first tile column
   // hold the overridden face at column 2; the others form their face states in registers
@@ -558,6 +636,18 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
     const unsigned nb = (kx ? 0u : 1u) | (ky ? 0u : 2u);
     LIM[i] = (unsigned char)(IO == 0 ? ((f & 3u) | (nb << 2)) : nb);
   }
+  if (IO == 1) {
+    un[4] = lim4;
+    un[5] = lim5;
+  }
+  if (dm.nseg > 0) {
+    // this row segment's flag of the output state (lane 0 holds column c0, always active)
+    bool nz = false;
+#pragma unroll
+    for (int k = 0; k < 7; k++) nz = nz || (un[k] != T(0));
+    const unsigned long long wet = __ballot(nz);
+    if (lane == 0) (IO == 0 ? dm.z1 : dm.z0)[(long long)(r - MD) * dm.nseg + tx] = wet != 0ull;
+  }
   if (IO == 0) {
 #pragma unroll
     for (int k = 0; k < 7; k++) Uout[k * P + i] = un[k];
@@ -565,8 +655,6 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
     ONE[P + i] = lim5;
     return 0.0;
   } else {
-    un[4] = lim4;
-    un[5] = lim5;
 #pragma unroll
     for (int k = 0; k < 7; k++) Uout[k * P + i] = un[k];
     if ((double)un[0] + (double)un[3] <= WET_SPEED) return 0.0;

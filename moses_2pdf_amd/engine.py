@@ -28,7 +28,7 @@ EXPORTS = ["m2p_default_params", "m2p_open", "m2p_open_grid", "m2p_run", "m2p_st
            "m2p_get_state", "m2p_set_state", "m2p_dims", "m2p_mass", "m2p_scalars",
            "m2p_get_field", "m2p_profile", "m2p_profile_get", "m2p_last_error", "m2p_close",
            "m2p_set_stop", "m2p_status", "m2p_footprint", "m2p_checkpoint", "m2p_restore",
-           "m2p_run_steps",
+           "m2p_run_steps", "m2p_set_dry_skip",
            "m2p_batch_create", "m2p_batch_add", "m2p_batch_remove", "m2p_batch_size",
            "m2p_batch_step", "m2p_batch_sync", "m2p_batch_tune", "m2p_batch_set", "m2p_batch_get",
            "m2p_batch_last_error", "m2p_batch_destroy"]
@@ -42,7 +42,7 @@ class Params(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
                 ("total_sim", "each_out", "delta0", "Cd", "N_R", "varTheta", "phiS0", "cfl",
                  "rho_ratio", "dt_min", "dt_max")] + [("precision", ctypes.c_int),
-                                                       ("reserved", ctypes.c_int)]
+                                                       ("dry_skip", ctypes.c_int)]
 
 
 def lib():
@@ -81,6 +81,7 @@ def lib():
     L.m2p_close.restype = None
     llp = ctypes.POINTER(ctypes.c_longlong)
     L.m2p_set_stop.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    L.m2p_set_dry_skip.argtypes = [vp, ctypes.c_int]
     L.m2p_checkpoint.argtypes = [vp, ctypes.c_char_p]
     L.m2p_restore.argtypes = [vp, ctypes.c_char_p]
     L.m2p_run_steps.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, ip, dp]
@@ -123,18 +124,22 @@ def default_params(mode: int) -> Params:
 class Simulation:
     """One scenario resident on the GPU.  ``precision`` is "fp64" (the reference's
     arithmetic) or "fp32" (FP32-bulk: float state and per-step arithmetic, FP64 metric
-    setup, CFL max, dt control and mass totals)."""
+    setup, CFL max, dt control and mass totals).  ``dry_skip`` leaves out tiles whose
+    inputs are all zero (the results are the same; see m2p_params.dry_skip)."""
 
     def __init__(self, sc=None, *, par_list: Optional[str] = None, mode: Optional[int] = None,
-                 overlay: Optional[str] = None, precision: str = "fp64"):
+                 overlay: Optional[str] = None, precision: str = "fp64", dry_skip: bool = True):
         L = lib()
         self._h = ctypes.c_void_p()
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
         self.precision = precision
+        self._dry_skip = bool(dry_skip)
         if par_list is not None:
             if precision != "fp64":
                 overlay = ((overlay + ";") if overlay else "") + "precision=" + precision
+            if not dry_skip:
+                overlay = ((overlay + ";") if overlay else "") + "dry_skip=0"
             rc = L.m2p_open(par_list.encode(), int(mode), overlay.encode() if overlay else None,
                             ctypes.byref(self._h))
         else:
@@ -154,6 +159,7 @@ class Simulation:
         p.delta0, p.Cd, p.N_R, p.varTheta, p.phiS0 = sc.delta0, sc.Cd, sc.N_R, sc.varTheta, sc.phiS0
         p.cfl, p.rho_ratio, p.dt_min, p.dt_max = sc.cfl, sc.rho_ratio, sc.dt_floor(), sc.dt_max
         p.precision = PRECISIONS[self.precision]
+        p.dry_skip = 1 if self._dry_skip else 0
         f32 = lambda a: None if a is None else np.ascontiguousarray(
             np.asarray(a, np.float64).astype(np.float32).astype(np.float64))
         topo = f32(sc.topo)
@@ -201,6 +207,10 @@ class Simulation:
         """Halting rules counted from now (see m2p_set_stop)."""
         self._check(lib().m2p_set_stop(self._h, float(t_end), float(vstop), int(max_steps)),
                     "m2p_set_stop")
+
+    def set_dry_skip(self, on: bool) -> None:
+        """Dry-tile skipping on or off from the next step (see m2p_set_dry_skip)."""
+        self._check(lib().m2p_set_dry_skip(self._h, 1 if on else 0), "m2p_set_dry_skip")
 
     def status(self):
         """(scaled time, steps done, halted)."""

@@ -29,12 +29,12 @@ def _full(h):
     return h
 
 
-def run_plain(scs, steps, precision="fp64", keep=True):
+def run_plain(scs, steps, precision="fp64", keep=True, dry_skip=True):
     """Each scenario alone, one after another, `steps` m2p_step steps each."""
     t0 = time.perf_counter()
     out = []
     for k, sc in enumerate(scs):
-        with Simulation(sc, precision=precision) as s:
+        with Simulation(sc, precision=precision, dry_skip=dry_skip) as s:
             s.step(steps)
             s.sync()
             if keep:
@@ -46,11 +46,12 @@ def run_plain(scs, steps, precision="fp64", keep=True):
     return time.perf_counter() - t0, out
 
 
-def run_batch(scs, steps, precision, tune_steps=3, t_end=None, keep=True, graph_steps=None):
+def run_batch(scs, steps, precision, tune_steps=3, t_end=None, keep=True, graph_steps=None,
+              dry_skip=True):
     """All scenarios in one batch.  With t_end (per member), each member halts at its own
     end time and the batch steps until every member has halted (steps is then a cap)."""
     t0 = time.perf_counter()
-    sims = [Simulation(sc, precision=precision) for sc in scs]
+    sims = [Simulation(sc, precision=precision, dry_skip=dry_skip) for sc in scs]
     t_open = time.perf_counter() - t0
     b = Batch(precision)
     for s in sims:

@@ -89,8 +89,9 @@ def _peak(req: Request, precision: str) -> int:
 class Server:
     def __init__(self, precision: str = "fp32", hbm_budget: float = 200e9, tick_steps: int = 100,
                  vstop: float = 0.01, max_steps: int = 8000, max_active: int = 256,
-                 keep_state: bool = False, graph_steps: int = 25):
+                 keep_state: bool = False, graph_steps: int = 25, dry_skip: bool = True):
         self.precision = precision
+        self.dry_skip = dry_skip
         self.hbm_budget = hbm_budget
         self.tick_steps = tick_steps
         self.vstop = vstop
@@ -119,7 +120,7 @@ class Server:
                     break
                 queue.pop(0)
                 start = now()
-                sim = Simulation(req.scenario, precision=self.precision)
+                sim = Simulation(req.scenario, precision=self.precision, dry_skip=self.dry_skip)
                 sim.set_stop(vstop=self.vstop, max_steps=self.max_steps)
                 b.add(sim)
                 held += steady
@@ -150,7 +151,7 @@ class Server:
 
 def serve_alone(requests: List[Request], precision: str = "fp64", vstop: float = 0.01,
                 max_steps: int = 8000, tick_steps: int = 100, realtime: bool = True,
-                keep_state: bool = False) -> List[Result]:
+                keep_state: bool = False, dry_skip: bool = True) -> List[Result]:
     """The one-at-a-time path: requests in arrival order, each alone on the device with
     m2p_step, the same stop rule checked every tick_steps steps."""
     results = []
@@ -163,7 +164,7 @@ def serve_alone(requests: List[Request], precision: str = "fp64", vstop: float =
             else:
                 t0 -= req.arrival - now()
         start = now()
-        with Simulation(req.scenario, precision=precision) as sim:
+        with Simulation(req.scenario, precision=precision, dry_skip=dry_skip) as sim:
             sim.set_stop(vstop=vstop, max_steps=max_steps)
             while True:
                 sim.step(tick_steps)
