@@ -51,6 +51,8 @@ struct m2p_batch {
   size_t capMap = 0;
   double* dPart = nullptr;
   size_t capPart = 0;
+  int* dPlan = nullptr;   // stage plans under dry-tile skipping
+  size_t capPlan = 0;
   m2p::BatchDev B{};
   bool any_inflow = false;
   hipGraph_t graph = nullptr;
@@ -107,7 +109,7 @@ int rebuild(m2p_batch* b) {
   std::vector<m2p::Member> M(n);
   std::vector<int> mg, mb, mr, ms;
   b->any_inflow = false;
-  bool any_vstop = false;
+  bool any_vstop = false, any_dry = false;
   for (int k = 0; k < n; k++) {
     m2p_ctx* c = b->mem[ord[k]];
     m2p::Member& m = M[k];
@@ -139,6 +141,7 @@ int rebuild(m2p_batch* b) {
     m.dire = c->dDire;
     if (c->mode == 1) b->any_inflow = true;
     if (c->vstop_on) any_vstop = true;
+    if (c->dry.nseg > 0) any_dry = true;
     m.box_n = m.nbx * m2p::box_blocks_y(c->g);
     m.ring_n = m2p::ring_blocks(c->g);
     m.stage_n = m2p::stage_tiles(c->g, b->ty);
@@ -172,6 +175,15 @@ int rebuild(m2p_batch* b) {
     b->capPart = std::max(npart, 2 * b->capPart);
     BCHK(b, hipMalloc((void**)&b->dPart, b->capPart * sizeof(double)));
   }
+  const int per8 = (int)((ms.size() + 7) / 8);
+  const size_t nplan = 16 + 16 * (size_t)per8;
+  if (nplan > b->capPlan) {
+    if (b->dPlan) (void)hipFree(b->dPlan);
+    b->dPlan = nullptr;
+    b->capPlan = std::max(nplan, 2 * b->capPlan);
+    BCHK(b, hipMalloc((void**)&b->dPlan, b->capPlan * sizeof(int)));
+    BCHK(b, hipMemsetAsync(b->dPlan, 0, b->capPlan * sizeof(int), b->s));
+  }
   std::vector<int> all;
   all.reserve(nmap);
   for (auto* v : {&mg, &mb, &mr, &ms}) all.insert(all.end(), v->begin(), v->end());
@@ -192,6 +204,13 @@ int rebuild(m2p_batch* b) {
   B.nStage = (int)ms.size();
   B.remap = b->remap;
   B.vmax = any_vstop ? 1 : 0;
+  B.plan_on = any_dry ? 1 : 0;
+  B.plan_cnt = b->dPlan;
+  for (int io = 0; io < 2; io++) {
+    B.plan[io].cnt = b->dPlan + 8 * io;
+    B.plan[io].list = b->dPlan + 16 + 8 * (size_t)per8 * io;
+    B.plan[io].per = b->remap ? per8 : (int)ms.size();
+  }
   b->dirty = false;
   return 0;
 }
@@ -431,6 +450,7 @@ void m2p_batch_destroy(m2p_batch* b) {
   if (b->dMem) (void)hipFree(b->dMem);
   if (b->dMap) (void)hipFree(b->dMap);
   if (b->dPart) (void)hipFree(b->dPart);
+  if (b->dPlan) (void)hipFree(b->dPlan);
   if (b->s) (void)hipStreamDestroy(b->s);
   delete b;
 }

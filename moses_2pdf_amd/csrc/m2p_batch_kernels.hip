@@ -40,9 +40,9 @@ template <typename T>
 __global__ void __launch_bounds__(256) kb_cfl(BatchDev B) {
   const Member& m = B.mem[B.mapBox[blockIdx.x]];
   if (halted_pre(m.st)) return;
-  const int lb = (int)blockIdx.x - m.box0;
-  block_max_store(cfl_box((const T*)m.U0, (const T*)m.MT, m.g, m.ph, lb % m.nbx, lb / m.nbx,
-                          m.dry),
+  const int lb = (int)blockIdx.x - m.box0, bx = lb % m.nbx, by = lb / m.nbx;
+  const bool dry = m.dry.nseg > 0 && cfl_box_dry((const T*)m.U0, m.g, m.dry, bx, by);
+  block_max_store(dry ? cfl_floor<T>(m.g) : cfl_box((const T*)m.U0, (const T*)m.MT, m.g, m.ph, bx, by),
                   B.part + blockIdx.x);
 }
 
@@ -60,6 +60,7 @@ __global__ void __launch_bounds__(64) kb_ring(BatchDev B) {
 template <typename T>
 __global__ void __launch_bounds__(256) kb_dt(BatchDev B) {
   const Member& m = B.mem[blockIdx.x];
+  if (B.plan_on && blockIdx.x == 0 && threadIdx.x < 16) B.plan_cnt[threadIdx.x] = 0;
   __shared__ int halted;
   if (threadIdx.x == 0) halted = dt_halted(m.st) ? 1 : 0;
   __syncthreads();
@@ -81,16 +82,44 @@ __global__ void __launch_bounds__(256) kb_vmax(BatchDev B) {
   }
 }
 
+// One wave per stage tile of the batch: its plan entry (see TilePlan).
+template <int IO, typename T, int TY>
+__global__ void __launch_bounds__(256) kb_plan(BatchDev B) {
+  const int t = blockIdx.x * 4 + threadIdx.y;
+  if (t >= B.nStage) return;
+  const Member& m = B.mem[B.mapStage[t]];
+  if (halted_post(m.st)) return;
+  int tx, ty;
+  if (!tile_of(m.g, TY, t - m.stage0, tx, ty)) return;
+  const int e = plan_tile<IO, T, TY>(m.g, (const T*)(IO == 0 ? m.U0 : m.U1), m.dry, tx, ty, t,
+                                     threadIdx.x);
+  if (threadIdx.x != 0) return;
+  const TilePlan& pl = B.plan[IO];
+  if (e >= 0) {
+    const int x = B.remap ? t / pl.per : 0;
+    pl.list[x * pl.per + atomicAdd(&pl.cnt[x], 1)] = e;
+  } else if (IO == 1) {
+    B.part[B.nBox + B.nRing + t] = 0.0;
+  }
+}
+
 template <int IO, typename T, int TY, int W>
 __global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per_eu(W, W)))
     kb_stage(BatchDev B) {
   __shared__ T F[ST_NQ][TY + 1][64];
   const int b = blockIdx.x;
   int t = b;
-  if (B.remap) {
+  if (B.plan_on) {   // dry-tile skipping: the plan's entry j of band x
+    const TilePlan& pl = B.plan[IO];
+    const int x = B.remap ? b % N_XCD : 0, j = B.remap ? b / N_XCD : b;
+    if (j >= pl.cnt[x]) return;
+    t = pl.list[x * pl.per + j];
+  } else if (B.remap) {
     const int per = (B.nStage + N_XCD - 1) / N_XCD;
     t = (b % N_XCD) * per + b / N_XCD;
   }
+  const bool zero = (t & PLAN_ZERO) != 0;
+  t &= ~PLAN_ZERO;
   if (t >= B.nStage) return;
   const Member& m = B.mem[B.mapStage[t]];
   DevState* st = m.st;
@@ -99,6 +128,10 @@ __global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per
   if (!tile_of(m.g, TY, t - m.stage0, tx, ty)) return;
   const T* Uin = (const T*)(IO == 0 ? m.U0 : m.U1);
   T* Uout = (T*)(IO == 0 ? m.U1 : m.U0);
+  if (IO == 0 && zero) {
+    stage_zero<T, TY>(m.g, Uout, (T*)m.ONE, m.dry, tx, ty);
+    return;
+  }
   const double v2 = stage_tile<IO, T, TY>(F, m.g, m.ph, Uin, (const T*)m.U0, Uout, (T*)m.ONE,
                                           (const T*)m.MT, m.LIM,
                                           IO == 0 ? st->dt_pred : st->dt_corr, tx, ty, m.dry);
@@ -109,10 +142,14 @@ template <typename T, int TY, int W>
 static void stage_pair(const BatchDev& B, int io, hipStream_t s) {
   const dim3 blk(64, TY + 1);
   const unsigned nb = B.remap ? (unsigned)(((B.nStage + N_XCD - 1) / N_XCD) * N_XCD) : (unsigned)B.nStage;
-  if (io == 0)
+  const unsigned np = (unsigned)((B.nStage + 3) / 4);
+  if (io == 0) {
+    if (B.plan_on) kb_plan<0, T, TY><<<np, dim3(64, 4), 0, s>>>(B);
     kb_stage<0, T, TY, W><<<nb, blk, 0, s>>>(B);
-  else
+  } else {
+    if (B.plan_on) kb_plan<1, T, TY><<<np, dim3(64, 4), 0, s>>>(B);
     kb_stage<1, T, TY, W><<<nb, blk, 0, s>>>(B);
+  }
 }
 
 // The compiled (tile height, waves per SIMD) variants; the list in stage_variants() must

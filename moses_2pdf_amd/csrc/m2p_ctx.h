@@ -42,6 +42,8 @@ struct m2p_ctx {
   unsigned char* dLIM = nullptr;  // FP32 only: limiter memory flags, one byte per cell
   unsigned char* dZ = nullptr;    // dry-tile flags of U0 and U1 (m2p::DryMap)
   m2p::DryMap dry{};              // nseg 0 while dry-tile skipping is off
+  int* dPlan = nullptr;           // tile plans of the two stages and their 16 counters
+  m2p::TilePlan plan[2]{};
   double *dTf = nullptr, *dMass = nullptr;
   double* dPart = nullptr;  // per-block partial maxima of a step (m2p::partial_count)
   double *dTable = nullptr, *dLoc = nullptr, *dItime = nullptr;

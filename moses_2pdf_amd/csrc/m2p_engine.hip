@@ -99,13 +99,16 @@ void release(m2p_ctx* c) {
   if (c->dMT == c->dM) c->dMT = nullptr;
   void* ps[] = {c->dTopo, c->dDepth, c->dPhi, c->dM,     c->dU0,   c->dU1,    c->dONE, c->dMT,
                 c->dTf,   c->dMass,  c->dTable, c->dLoc, c->dItime, c->dDire, c->dSt,  c->dLIM,
-                c->dPart};
+                c->dPart, c->dZ,     c->dPlan};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   c->dTopo = c->dDepth = c->dPhi = c->dM = c->dTf = c->dMass = c->dTable = c->dLoc = c->dItime = nullptr;
   c->dU0 = c->dU1 = c->dONE = c->dMT = nullptr;
   c->dLIM = nullptr;
   c->dPart = nullptr;
+  c->dZ = nullptr;
+  c->dPlan = nullptr;
+  c->dry = m2p::DryMap{};
   c->dDire = nullptr;
   c->dSt = nullptr;
   if (c->s) (void)hipStreamDestroy(c->s);
@@ -174,8 +177,15 @@ int setup(m2p_ctx* c) {
   if ((rc = dalloc(c, &c->dZ, 2 * (size_t)nz))) return rc;
   c->dry.z0 = c->dZ;
   c->dry.z1 = c->dZ + nz;
-  c->dry.nseg = c->par.dry_skip ? m2p::dry_segments(c->g) : 0;
+  c->dry.nseg = c->par.dry_skip && m2p::dry_capable(c->g) ? m2p::dry_segments(c->g) : 0;
   m2p::launch_dry_fill(c->g, c->dry, c->s);
+  const int per = (m2p::plain_stage_tiles(c->g, c->prec) + 7) / 8;
+  if ((rc = dalloc(c, &c->dPlan, 16 + 16 * (size_t)per))) return rc;
+  for (int io = 0; io < 2; io++) {
+    c->plan[io].cnt = c->dPlan + 8 * io;
+    c->plan[io].list = c->dPlan + 16 + 8 * (size_t)per * io;
+    c->plan[io].per = per;
+  }
 
   DevState st{};
   st.iter = 1;
@@ -318,12 +328,15 @@ void enqueue_step_t(m2p_ctx* c, int ntf) {
   }
   {
     Timer t(c, "dt");
-    m2p::launch_dt(dp, c->dSt, c->dTf, c->dItime, c->g, c->prec, c->dPart, c->s);
+    m2p::launch_dt(dp, c->dSt, c->dTf, c->dItime, c->g, c->prec, c->dPart,
+                   c->dry.nseg > 0 ? c->dPlan : nullptr, c->s);
   }
   {
     Timer t(c, "stage0");
+    if (c->dry.nseg > 0)
+      m2p::launch_plan(0, c->g, (const T*)U0, c->dry, c->plan[0], c->dPart, c->dSt, c->s);
     m2p::launch_stage(0, c->g, c->ph, (const T*)U0, (const T*)U0, U1, ONE, MT, c->dLIM, c->dSt,
-                      c->dPart, c->dry, c->s);
+                      c->dPart, c->dry, c->dry.nseg > 0 ? c->plan[0] : m2p::TilePlan{}, c->s);
   }
   {
     Timer t(c, "ghost");
@@ -331,8 +344,10 @@ void enqueue_step_t(m2p_ctx* c, int ntf) {
   }
   {
     Timer t(c, "stage1");
+    if (c->dry.nseg > 0)
+      m2p::launch_plan(1, c->g, (const T*)U1, c->dry, c->plan[1], c->dPart, c->dSt, c->s);
     m2p::launch_stage(1, c->g, c->ph, (const T*)U1, (const T*)U0, U0, ONE, MT, c->dLIM, c->dSt,
-                      c->dPart, c->dry, c->s);
+                      c->dPart, c->dry, c->dry.nseg > 0 ? c->plan[1] : m2p::TilePlan{}, c->s);
   }
   if (c->vstop_on) m2p::launch_vmax(c->dSt, c->g, c->prec, c->dPart, c->s);
 }
@@ -824,7 +839,7 @@ int m2p_set_dry_skip(m2p_ctx* c, int on) {
   if (on != 0 && on != 1) return fail(c, M2P_EINVAL, "dry_skip must be 0 or 1");
   hipStream_t s = strm(c);
   c->par.dry_skip = on;
-  c->dry.nseg = on ? m2p::dry_segments(c->g) : 0;
+  c->dry.nseg = on && m2p::dry_capable(c->g) ? m2p::dry_segments(c->g) : 0;
   // while off, no kernel keeps the flags, so they start again from "maybe wet"
   m2p::DryMap all = c->dry;
   all.nseg = m2p::dry_segments(c->g);
@@ -854,6 +869,7 @@ int m2p_footprint(int mode, int NX, int NY, int precision, int nsamp, int nloc,
   const long long full = (long long)m2p::MC_COUNT * P * 8, cells = (long long)m2p::MC_CELL_END * P * 8;
   long long b = 4LL * m2p::mass_blocks() * 8 + (long long)sizeof(DevState) + 4096;
   b += (P / 256 + P / 441 + 4LL * (NX + NY) + 64) * 8;   // per-block partial maxima
+  b += 2LL * NY * ((NX + 62) / 63) + 4 * (16 + 16 * (P / 441 / 8 + 1));   // dry flags, plans
   if (precision == M2P_FP64)
     b += full + 16 * P * 8;
   else

@@ -47,9 +47,26 @@ struct DryMap {
   int nseg;
 };
 inline int dry_segments(const Grid& g) { return (g.nx + 62) / 63; }
+// The dry test infers the ghost cells from the interior cells two deep along the boundary,
+// so it needs at least two interior rows and columns.
+inline bool dry_capable(const Grid& g) { return g.nx >= 2 && g.ny >= 2; }
 inline long long dry_flag_count(const Grid& g) { return (long long)g.ny * dry_segments(g); }
 // Sets every flag of both maps to 1 (nothing may be skipped in the next step).
 void launch_dry_fill(const Grid& g, const DryMap& dm, hipStream_t s);
+
+// Tile plan of one stage launch under dry-tile skipping.  A plan kernel tests every tile
+// (one wave per tile) and lists, per XCD band of the stage grid (band x holds tiles
+// [x * per, (x + 1) * per)), the tiles the stage must visit: those to compute and, for the
+// predictor, dry tiles whose U1 and ONE must be cleared (entry | PLAN_ZERO).  Block j of
+// band x of the stage takes entry j of list x; blocks past the list's end return at once,
+// so dry tiles cost no more than a block launch.  The counters are zeroed by the dt kernel
+// of the step.
+constexpr int PLAN_ZERO = 1 << 30;
+struct TilePlan {
+  int* list;   // 8 * per entries
+  int* cnt;    // 8 counters
+  int per;     // band size: ceil(tiles / 8)
+};
 
 // A scenario is halted for the kernels that run before the dt kernel of a step
 // (inflow, ghost fill, CFL) once it has done halt_at steps, and for those after it once
@@ -85,14 +102,20 @@ size_t partial_count(const Grid& g, int prec);
 template <typename T>
 void launch_cfl(const Grid& g, const Phys& ph, const T* U, const T* M, DevState* st, double* part,
                 const DryMap& dm, hipStream_t s);
+// plan_cnt: the 16 plan counters of the step's two stages (zeroed here), or null.
 void launch_dt(const DevParams& dp, DevState* st, const double* tf, const double* itime,
-               const Grid& g, int prec, const double* part, hipStream_t s);
+               const Grid& g, int prec, const double* part, int* plan_cnt, hipStream_t s);
+// Tiles of the single-scenario stage launch, and the plan kernel before it.
+int plain_stage_tiles(const Grid& g, int prec);
+template <typename T>
+void launch_plan(int io, const Grid& g, const T* Uin, const DryMap& dm, const TilePlan& pl,
+                 double* part, const DevState* st, hipStream_t s);
 // After a corrector: the squared max flow speed of its tiles into DevState (stop speed).
 void launch_vmax(DevState* st, const Grid& g, int prec, const double* part, hipStream_t s);
 template <typename T>
 void launch_stage(int io, const Grid& g, const Phys& ph, const T* Uin, const T* Uzero, T* Uout,
                   T* ONE, const T* M, unsigned char* LIM, DevState* st, double* part,
-                  const DryMap& dm, hipStream_t s);
+                  const DryMap& dm, const TilePlan& pl, hipStream_t s);
 int mass_blocks();
 template <typename T>
 void launch_mass(const Grid& g, const T* U, const double* M, double* out, hipStream_t s);

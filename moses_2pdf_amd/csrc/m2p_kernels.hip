@@ -209,8 +209,36 @@ __global__ void __launch_bounds__(256) k_cfl(Grid g, Phys ph, const T* __restric
                                              DryMap dm) {
   if (halted_pre(st)) return;
   double m = 0.0;
-  for (int t = blockIdx.x; t < nbox; t += gridDim.x)
-    m = fmax(m, cfl_box(U, M, g, ph, t % nbx, t / nbx, dm));
+  if (dm.nseg == 0) {
+    for (int t = blockIdx.x; t < nbox; t += gridDim.x) m = fmax(m, cfl_box(U, M, g, ph, t % nbx, t / nbx));
+    block_max_store(m, part + blockIdx.x);
+    return;
+  }
+  // Dry-tile skipping: the dry tests of up to 8 boxes of this block in one round of loads
+  // (32 threads per box), then only the wet boxes are computed.
+  __shared__ int wetk[8];
+  const int tid = threadIdx.y * 64 + threadIdx.x;
+  const double fl = cfl_floor<T>(g);
+  for (int t0 = blockIdx.x; t0 < nbox; t0 += 8 * gridDim.x) {
+    __syncthreads();   // every thread is done with the previous group's wetk
+    if (tid < 8) wetk[tid] = 0;
+    __syncthreads();
+    {
+      const int k = tid >> 5, t = t0 + k * gridDim.x;
+      if (t < nbox) {
+        const DryRegion d = box_region(g, t % nbx, t / nbx);
+        int wet = 0;
+        for (int e = tid & 31; e < d.n; e += 32) wet |= dry_item_wet(d, U, g, dm.z0, dm.nseg, e);
+        if (wet) atomicOr(&wetk[k], 1);
+      }
+    }
+    __syncthreads();
+    for (int k = 0; k < 8; k++) {
+      const int t = t0 + k * gridDim.x;
+      if (t >= nbox) break;
+      m = fmax(m, wetk[k] ? cfl_box(U, M, g, ph, t % nbx, t / nbx) : fl);
+    }
+  }
   block_max_store(m, part + blockIdx.x);
 }
 
@@ -228,11 +256,13 @@ __global__ void __launch_bounds__(64) k_cfl_ring(Grid g, Phys ph, const T* __res
 __global__ void __launch_bounds__(256) k_dt(DevParams dp, DevState* __restrict__ st,
                                             const double* __restrict__ tf,
                                             const double* __restrict__ itime,
-                                            const double* __restrict__ part, int ncfl) {
+                                            const double* __restrict__ part, int ncfl,
+                                            int* __restrict__ plan_cnt) {
   __shared__ int halted;
   if (threadIdx.x == 0) halted = dt_halted(st) ? 1 : 0;
   __syncthreads();
   if (halted) return;
+  if (plan_cnt && threadIdx.x < 16) plan_cnt[threadIdx.x] = 0;
   const double mx = block_max_of(part, ncfl);
   if (threadIdx.x == 0) dt_body(dp, st, tf, itime, mx);
 }
@@ -259,17 +289,48 @@ template <int IO, typename T, int TY, int W>
 __global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per_eu(W, W)))
     k_stage(Grid g, Phys ph, const T* __restrict__ Uin, const T* Uzero, T* Uout,
             T* __restrict__ ONE, const T* __restrict__ M, unsigned char* __restrict__ LIM,
-            DevState* __restrict__ st, double* __restrict__ part, DryMap dm) {
+            DevState* __restrict__ st, double* __restrict__ part, DryMap dm, TilePlan pl) {
   __shared__ T F[ST_NQ][TY + 1][64];
   if (halted_post(st)) return;
   const int b = blockIdx.x;
   const int per = (stage_tiles(g, TY) + N_XCD - 1) / N_XCD;
-  int tx, ty;
-  const int t = (b % N_XCD) * per + b / N_XCD;
-  if (!tile_of(g, TY, t, tx, ty)) return;
+  int tx, ty, t;
+  if (pl.list) {   // dry-tile skipping: the plan's entry j of band x
+    const int x = b % N_XCD, j = b / N_XCD;
+    if (j >= pl.cnt[x]) return;
+    t = pl.list[x * pl.per + j];
+  } else {
+    t = (b % N_XCD) * per + b / N_XCD;
+  }
+  if (!tile_of(g, TY, t & ~PLAN_ZERO, tx, ty)) return;
+  if (IO == 0 && (t & PLAN_ZERO)) {
+    stage_zero<T, TY>(g, Uout, ONE, dm, tx, ty);
+    return;
+  }
   const double v2 = stage_tile<IO, T, TY>(F, g, ph, Uin, Uzero, Uout, ONE, M, LIM,
                                           IO == 0 ? st->dt_pred : st->dt_corr, tx, ty, dm);
   if (IO == 1 && st->vstop2 > 0.0) block_max_store(v2, part + t);
+}
+
+// One wave per stage tile: its plan entry into the list of its XCD band (tiles that need
+// nothing have no entry; in the corrector their flow-speed partial is set to 0).
+template <int IO, typename T, int TY>
+__global__ void __launch_bounds__(256) k_plan(Grid g, const T* __restrict__ Uin, DryMap dm,
+                                              TilePlan pl, double* __restrict__ spart,
+                                              const DevState* __restrict__ st) {
+  if (halted_post(st)) return;
+  const int t = blockIdx.x * 4 + threadIdx.y;
+  if (t >= stage_tiles(g, TY)) return;
+  int tx, ty;
+  tile_of(g, TY, t, tx, ty);
+  const int e = plan_tile<IO, T, TY>(g, Uin, dm, tx, ty, t, threadIdx.x);
+  if (threadIdx.x != 0) return;
+  if (e >= 0) {
+    const int x = t / pl.per;
+    pl.list[x * pl.per + atomicAdd(&pl.cnt[x], 1)] = e;
+  } else if (IO == 1) {
+    spart[t] = 0.0;
+  }
 }
 
 // Sum of u0*Det and u3*Det over interior cells with double-double accumulation; one
@@ -405,11 +466,26 @@ void launch_cfl(const Grid& g, const Phys& ph, const T* U, const T* M, DevState*
 }
 
 void launch_dt(const DevParams& dp, DevState* st, const double* tf, const double* itime,
-               const Grid& g, int prec, const double* part, hipStream_t s) {
+               const Grid& g, int prec, const double* part, int* plan_cnt, hipStream_t s) {
+  (void)prec;
   const int ncfl = cfl_box_blocks(g) + cfl_ring_blocks(g);
-  const int ty = prec == 1 ? PlainStage<float>::TY : PlainStage<double>::TY;
-  (void)ty;
-  k_dt<<<1, 256, 0, s>>>(dp, st, tf, itime, part, ncfl);
+  k_dt<<<1, 256, 0, s>>>(dp, st, tf, itime, part, ncfl, plan_cnt);
+}
+
+int plain_stage_tiles(const Grid& g, int prec) {
+  return stage_tiles(g, prec == 1 ? PlainStage<float>::TY : PlainStage<double>::TY);
+}
+
+template <typename T>
+void launch_plan(int io, const Grid& g, const T* Uin, const DryMap& dm, const TilePlan& pl,
+                 double* part, const DevState* st, hipStream_t s) {
+  constexpr int TY = PlainStage<T>::TY;
+  const unsigned nb = (unsigned)((stage_tiles(g, TY) + 3) / 4);
+  double* sp = part + cfl_box_blocks(g) + cfl_ring_blocks(g);
+  if (io == 0)
+    k_plan<0, T, TY><<<nb, dim3(64, 4), 0, s>>>(g, Uin, dm, pl, sp, st);
+  else
+    k_plan<1, T, TY><<<nb, dim3(64, 4), 0, s>>>(g, Uin, dm, pl, sp, st);
 }
 
 void launch_vmax(DevState* st, const Grid& g, int prec, const double* part, hipStream_t s) {
@@ -420,15 +496,15 @@ void launch_vmax(DevState* st, const Grid& g, int prec, const double* part, hipS
 template <typename T>
 void launch_stage(int io, const Grid& g, const Phys& ph, const T* Uin, const T* Uzero, T* Uout,
                   T* ONE, const T* M, unsigned char* LIM, DevState* st, double* part,
-                  const DryMap& dm, hipStream_t s) {
+                  const DryMap& dm, const TilePlan& pl, hipStream_t s) {
   constexpr int TY = PlainStage<T>::TY, W = PlainStage<T>::W;
   const dim3 b(64, TY + 1);
   const dim3 gr(((stage_tiles(g, TY) + N_XCD - 1) / N_XCD) * N_XCD);
   double* sp = part + cfl_box_blocks(g) + cfl_ring_blocks(g);
   if (io == 0)
-    k_stage<0, T, TY, W><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm);
+    k_stage<0, T, TY, W><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
   else
-    k_stage<1, T, TY, W><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm);
+    k_stage<1, T, TY, W><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
 }
 
 int mass_blocks() { return 512; }
@@ -448,7 +524,9 @@ void launch_mass(const Grid& g, const T* U, const double* M, double* out, hipStr
                               const DryMap&, hipStream_t);                                     \
   template void launch_stage<T>(int, const Grid&, const Phys&, const T*, const T*, T*, T*,     \
                                 const T*, unsigned char*, DevState*, double*, const DryMap&,   \
-                                hipStream_t);                                                  \
+                                const TilePlan&, hipStream_t);                                 \
+  template void launch_plan<T>(int, const Grid&, const T*, const DryMap&, const TilePlan&,     \
+                               double*, const DevState*, hipStream_t);                         \
   template void launch_mass<T>(const Grid&, const T*, const double*, double*, hipStream_t);
 M2P_INSTANTIATE(double)
 M2P_INSTANTIATE(float)

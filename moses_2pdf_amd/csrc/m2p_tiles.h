@@ -224,51 +224,100 @@ __device__ __forceinline__ double block_max_of(const double* __restrict__ p, int
   return b;
 }
 
-// True when components 0..5 of every cell of U in rows [ra, rb] x columns [ca, cb] (clipped
-// to the padded grid) are zero: interior cells through the row-segment flags z, ghost cells
-// by reading them (the ghost fill that ran before the caller derived them from interior
-// cells, or for the lagging corners from the other buffer).  NaN counts as wet.  Every
-// thread of the block must call it; tid in [0, nthr).
-template <typename T>
-__device__ __forceinline__ bool dry_region(const T* __restrict__ U, const Grid& g,
-                                           const unsigned char* __restrict__ z, int nseg, int ra,
-                                           int rb, int ca, int cb, int tid, int nthr) {
+// The dry test of a region [ra, rb] x [ca, cb] of U (clipped to the padded grid) as a list
+// of items, each one load and test: first the row-segment flags of its interior cells,
+// then the corner ghost cells it holds (components 0..5).  The other ghost cells need no
+// test: the ghost fill that ran before the caller made them 0.5-weighted sums of the
+// interior cells within two cells of the boundary in the same row or column (momenta 0),
+// and for any region of a stage tile or CFL box (at least 7 rows and 67 columns) those
+// cells lie in the region's interior part, so their flags are among the items.  The corner
+// blocks (rows and columns both ghost) may lag a stage behind, so they are read.
+struct DryRegion {
+  int ia, sa, ns, nf;            // interior rows from ia, segments [sa, sa+ns), nf flag items
+  int rt, nrt, rbo, ncg, nrg;    // corner rows: [rt, rt+nrt) and [rbo, ...), nrg in all
+  int cl, ncl, cr;               // corner columns: [cl, cl+ncl) and [cr, ...), ncg in all
+  int n;                         // items
+};
+
+__device__ __forceinline__ DryRegion dry_region_of(const Grid& g, int ra, int rb, int ca, int cb) {
+  DryRegion d;
   ra = max(ra, 0);
   rb = min(rb, g.nyd - 1);
   ca = max(ca, 0);
   cb = min(cb, g.nxd - 1);
   const int xmax = g.nx + MD - 1, ymax = g.ny + MD - 1;
+  d.ia = max(ra, MD);
+  const int ib = min(rb, ymax);
+  d.sa = (max(ca, MD) - MD) / 63;
+  d.ns = (min(cb, xmax) - MD) / 63 - d.sa + 1;
+  d.nf = (ib - d.ia + 1) * d.ns;
+  d.rt = ra;
+  d.nrt = max(0, min(rb, MD - 1) - ra + 1);
+  d.rbo = max(ra, ymax + 1);
+  d.nrg = d.nrt + max(0, rb - d.rbo + 1);
+  d.cl = ca;
+  d.ncl = max(0, min(cb, MD - 1) - ca + 1);
+  d.cr = max(ca, xmax + 1);
+  d.ncg = d.ncl + max(0, cb - d.cr + 1);
+  d.n = d.nf + d.nrg * d.ncg;
+  return d;
+}
+
+// 1 if item e of the region is wet (NaN counts as wet)
+template <typename T>
+__device__ __forceinline__ int dry_item_wet(const DryRegion& d, const T* __restrict__ U,
+                                            const Grid& g, const unsigned char* __restrict__ z,
+                                            int nseg, int e) {
+  if (e < d.nf) return z[(long long)(d.ia + e / d.ns - MD) * nseg + d.sa + e % d.ns] != 0;
+  e -= d.nf;
+  const int ir = e / d.ncg, ic = e % d.ncg;
+  const int r = ir < d.nrt ? d.rt + ir : d.rbo + (ir - d.nrt);
+  const int c = ic < d.ncl ? d.cl + ic : d.cr + (ic - d.ncl);
+  const long long i = idx(g, r, c);
   int wet = 0;
-  const int ia = max(ra, MD), ib = min(rb, ymax);
-  const int sa = (max(ca, MD) - MD) / 63, sb = (min(cb, xmax) - MD) / 63;
-  const int ns = sb - sa + 1, nf = (ib - ia + 1) * ns;
-  for (int e = tid; e < nf; e += nthr) wet |= z[(long long)(ia + e / ns - MD) * nseg + sa + e % ns];
-  const int nc = cb - ca + 1;
-  const int ncell = (ra >= MD && rb <= ymax && ca >= MD && cb <= xmax) ? 0 : (rb - ra + 1) * nc;
-  for (int e = tid; e < ncell; e += nthr) {
-    const int r = ra + e / nc, c = ca + e % nc;
-    if (r >= MD && r <= ymax && c >= MD && c <= xmax) continue;
-    const long long i = idx(g, r, c);
 #pragma unroll
-    for (int k = 0; k < 6; k++) wet |= (U[k * g.P + i] != T(0)) ? 1 : 0;
-  }
+  for (int k = 0; k < 6; k++) wet |= (U[k * g.P + i] != T(0)) ? 1 : 0;
+  return wet;
+}
+
+// True when the region holds only zeros.  Every thread of the block must call it; tid in
+// [0, nthr).
+template <typename T>
+__device__ __forceinline__ bool dry_region(const T* __restrict__ U, const Grid& g,
+                                           const unsigned char* __restrict__ z, int nseg, int ra,
+                                           int rb, int ca, int cb, int tid, int nthr) {
+  const DryRegion d = dry_region_of(g, ra, rb, ca, cb);
+  int wet = 0;
+  for (int e = tid; e < d.n; e += nthr) wet |= dry_item_wet(d, U, g, z, nseg, e);
   return !__syncthreads_or(wet);
 }
 
-// one 64x4 block of interior faces (both directions) starting at box column bx*64, row by*4.
-// With dry-tile skipping, a box whose stencil (columns c-1..c+2 of its x-faces, rows r-1..r+2
-// of its y-faces) is all zero returns what its faces would: the wave-speed floor 1e-10 over
-// dx and dy.  Every thread of the block must call it.
+// CFL boxes: 64x4 blocks of interior faces (both directions), box (bx, by) starting at
+// column MD + 64 bx, row MD + 4 by.  A box's faces read columns c-1..c+2 (x-faces) and rows
+// r-1..r+2 (y-faces): the stencil [rlo-1, rlo+5] x [clo-1, clo+65].  A box whose stencil
+// is all zero gives the wave-speed floor 1e-10 over dx and dy on every face.
+template <typename T>
+__device__ __forceinline__ double cfl_floor(const Grid& g) {
+  return fmax((double)T(1.0e-10) / g.dx, (double)T(1.0e-10) / g.dy);
+}
+
+__device__ __forceinline__ DryRegion box_region(const Grid& g, int bx, int by) {
+  const int clo = MD + 64 * bx, rlo = MD + 4 * by;
+  return dry_region_of(g, rlo - 1, rlo + 5, clo - 1, clo + 65);
+}
+
+// Dry test of one box by the whole block.
+template <typename T>
+__device__ __forceinline__ bool cfl_box_dry(const T* __restrict__ U, const Grid& g,
+                                            const DryMap& dm, int bx, int by) {
+  const int clo = MD + 64 * bx, rlo = MD + 4 * by;
+  return dry_region(U, g, dm.z0, dm.nseg, rlo - 1, rlo + 5, clo - 1, clo + 65,
+                    threadIdx.y * blockDim.x + threadIdx.x, blockDim.x * blockDim.y);
+}
+
 template <typename T>
 __device__ __forceinline__ double cfl_box(const T* __restrict__ U, const T* __restrict__ M,
-                                          const Grid& g, const Phys& ph, int bx, int by,
-                                          const DryMap& dm) {
-  if (dm.nseg > 0) {
-    const int clo = MD + 64 * bx, rlo = MD + 4 * by;
-    if (dry_region(U, g, dm.z0, dm.nseg, rlo - 1, rlo + 5, clo - 1, clo + 65,
-                   threadIdx.y * blockDim.x + threadIdx.x, blockDim.x * blockDim.y))
-      return fmax((double)T(1.0e-10) / g.dx, (double)T(1.0e-10) / g.dy);
-  }
+                                          const Grid& g, const Phys& ph, int bx, int by) {
   const int c = bx * 64 + threadIdx.x + MD;
   const int r = by * 4 + threadIdx.y + MD;
   double m = 0.0;
@@ -465,39 +514,6 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
   const int xmax = g.nx + MD - 1, ymax = g.ny + MD - 1;
   const long long P = g.P;
 
-  // Dry-tile skipping.  The tile reads cells [r0-2, r0+TY+1] x [c0-2, c0+64] of Uin (face
-  // stencils, the boundary overrides of edge tiles and the viscous term's tangential slopes)
-  // and its own cells of Uzero (and of ONE, which is zero where U1 is).  When all of them
-  // are zero, every output is +0 and the flow speed 0.
-  if (dm.nseg > 0) {
-    const int tid = w * 64 + lane;
-    const int rown = min(TY, ymax - r0 + 1);           // output rows of this tile
-    unsigned char* zown = IO == 0 ? dm.z1 : dm.z0;     // their flags
-    int own = 0;
-    if (tid < rown) own = zown[(long long)(r0 + tid - MD) * dm.nseg + tx];
-    // the corrector also needs uzero = U0 zero on its cells
-    const int own_wet = __syncthreads_or(own);
-    if (!(IO == 1 && own_wet) &&
-        dry_region(Uin, g, IO == 0 ? dm.z0 : dm.z1, dm.nseg, r0 - 2, r0 + TY + 1, c0 - 2,
-                   c0 + 64, tid, 64 * (TY + 1))) {
-      // Predictor: U1 and ONE of the tile's cells must read zero; skip the stores when the
-      // tile's U1 flags say they already do (ONE is zero wherever U1 is).  Corrector: U0
-      // is updated in place and already zero.
-      if (IO == 0 && own_wet) {
-        const int r = r0 + w, c = c0 + lane;
-        if (w < TY && lane < ST_TX && r <= ymax && c <= xmax) {
-          const long long i = idx(g, r, c);
-#pragma unroll
-          for (int k = 0; k < 7; k++) Uout[k * P + i] = T(0);
-          ONE[i] = T(0);
-          ONE[P + i] = T(0);
-        }
-        if (w < rown && lane == 0) zown[(long long)(r - MD) * dm.nseg + tx] = 0;
-      }
-      return 0.0;
-    }
-  }
-
   // x faces (r0+w, c0-1+lane), faces (r, c)-(r, c+1).  Only tiles in the first tile column
   // hold the overridden face at column 2; the others form their face states in registers
   // and skip the LDS round trip (the branch is uniform over the block).
@@ -662,6 +678,49 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
     const double sx = (double)usxnew, sy = (double)usynew;
     return fmax(sx * sx + sy * sy, ufx * ufx + ufy * ufy);
   }
+}
+
+// Dry-tile skipping.  A stage tile reads cells [r0-2, r0+TY+1] x [c0-2, c0+64] of its
+// input (face stencils, the boundary overrides of edge tiles and the viscous term's
+// tangential slopes) and its own cells of uzero (and of ONE, which is zero wherever U1
+// is).  When all of them are zero, every output is +0 and the flow speed 0.
+// plan_tile is called by every lane of one wave and returns the tile's plan entry: t to
+// compute it, t | PLAN_ZERO (predictor) to clear its U1 and ONE because its U1 flags say
+// they may hold values of an earlier step, or -1 for nothing to do.
+template <int IO, typename T, int TY>
+__device__ __forceinline__ int plan_tile(const Grid& g, const T* __restrict__ Uin,
+                                         const DryMap& dm, int tx, int ty, int t, int lane) {
+  if (dm.nseg == 0) return t;
+  const int c0 = tx * ST_TX + MD, r0 = ty * TY + MD, ymax = g.ny + MD - 1;
+  const DryRegion d = dry_region_of(g, r0 - 2, r0 + TY + 1, c0 - 2, c0 + 64);
+  int wet = 0;
+  for (int e = lane; e < d.n; e += 64)
+    wet |= dry_item_wet(d, Uin, g, IO == 0 ? dm.z0 : dm.z1, dm.nseg, e);
+  const unsigned char* zown = IO == 0 ? dm.z1 : dm.z0;   // flags of the tile's output rows
+  const int rown = min(TY, ymax - r0 + 1);
+  const int own = lane < rown ? zown[(long long)(r0 + lane - MD) * dm.nseg + tx] : 0;
+  const bool anywet = __ballot(wet) != 0ull, anyown = __ballot(own) != 0ull;
+  if (anywet || (IO == 1 && anyown)) return t;   // the corrector also needs uzero zero
+  if (IO == 0 && anyown) return t | PLAN_ZERO;
+  return -1;
+}
+
+// A dry predictor tile whose U1 and ONE may hold values of an earlier step: writes the +0
+// that computing it would, and its U1 flags.
+template <typename T, int TY>
+__device__ __forceinline__ void stage_zero(const Grid& g, T* Uout, T* __restrict__ ONE,
+                                           const DryMap& dm, int tx, int ty) {
+  const int lane = threadIdx.x, w = threadIdx.y;
+  const int r = ty * TY + MD + w, c = tx * ST_TX + MD + lane;
+  if (w >= TY || r > g.ny + MD - 1) return;
+  if (lane < ST_TX && c <= g.nx + MD - 1) {
+    const long long i = idx(g, r, c), P = g.P;
+#pragma unroll
+    for (int k = 0; k < 7; k++) Uout[k * P + i] = T(0);
+    ONE[i] = T(0);
+    ONE[P + i] = T(0);
+  }
+  if (lane == 0) dm.z1[(long long)(r - MD) * dm.nseg + tx] = 0;
 }
 
 // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).  Block b runs on
