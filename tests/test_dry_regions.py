@@ -75,4 +75,4 @@ def test_dry_regions_cover_their_stencils():
                 clo, rlo = MD + 64 * bx, MD + 4 * by
                 check_region(nx, ny, rlo - 1, rlo + 5, clo - 1, clo + 65)
                 n += 1
-    assert n > 5000
+    assert n > 1000
