@@ -46,8 +46,8 @@ struct BatchDev {
   int nmem, nGhost, nBox, nRing, nStage;
   int remap;   // stage blocks dealt to the 8 XCDs in contiguous bands
   int vmax;    // some member has a stop speed: reduce corrector speeds after each step
-  // dry-tile skipping (some member has it on): the stage plans and their 16 counters,
-  // zeroed by block 0 of the dt kernel
+  // dry-tile skipping (some member has it on): the stage plans (8 bands whatever remap
+  // says) and their PLAN_COUNTERS counters, zeroed by block 0 of the dt kernel
   int plan_on;
   TilePlan plan[2];
   int* plan_cnt;

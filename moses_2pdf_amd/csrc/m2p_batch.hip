@@ -176,7 +176,7 @@ int rebuild(m2p_batch* b) {
     BCHK(b, hipMalloc((void**)&b->dPart, b->capPart * sizeof(double)));
   }
   const int per8 = (int)((ms.size() + 7) / 8);
-  const size_t nplan = 16 + 16 * (size_t)per8;
+  const size_t nplan = m2p::PLAN_COUNTERS + 16 * (size_t)per8;
   if (nplan > b->capPlan) {
     if (b->dPlan) (void)hipFree(b->dPlan);
     b->dPlan = nullptr;
@@ -208,8 +208,8 @@ int rebuild(m2p_batch* b) {
   B.plan_cnt = b->dPlan;
   for (int io = 0; io < 2; io++) {
     B.plan[io].cnt = b->dPlan + 8 * io;
-    B.plan[io].list = b->dPlan + 16 + 8 * (size_t)per8 * io;
-    B.plan[io].per = b->remap ? per8 : (int)ms.size();
+    B.plan[io].list = b->dPlan + m2p::PLAN_COUNTERS + 8 * (size_t)per8 * io;
+    B.plan[io].per = per8;
   }
   b->dirty = false;
   return 0;

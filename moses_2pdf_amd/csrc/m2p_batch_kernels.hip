@@ -60,7 +60,7 @@ __global__ void __launch_bounds__(64) kb_ring(BatchDev B) {
 template <typename T>
 __global__ void __launch_bounds__(256) kb_dt(BatchDev B) {
   const Member& m = B.mem[blockIdx.x];
-  if (B.plan_on && blockIdx.x == 0 && threadIdx.x < 16) B.plan_cnt[threadIdx.x] = 0;
+  if (B.plan_on && blockIdx.x == 0 && threadIdx.x < PLAN_COUNTERS) B.plan_cnt[threadIdx.x] = 0;
   __shared__ int halted;
   if (threadIdx.x == 0) halted = dt_halted(m.st) ? 1 : 0;
   __syncthreads();
@@ -85,41 +85,43 @@ __global__ void __launch_bounds__(256) kb_vmax(BatchDev B) {
 // One wave per stage tile of the batch: its plan entry (see TilePlan).
 template <int IO, typename T, int TY>
 __global__ void __launch_bounds__(256) kb_plan(BatchDev B) {
-  const int t = blockIdx.x * 4 + threadIdx.y;
-  if (t >= B.nStage) return;
-  const Member& m = B.mem[B.mapStage[t]];
-  if (halted_post(m.st)) return;
-  int tx, ty;
-  if (!tile_of(m.g, TY, t - m.stage0, tx, ty)) return;
-  const int e = plan_tile<IO, T, TY>(m.g, (const T*)(IO == 0 ? m.U0 : m.U1), m.dry, tx, ty, t,
-                                     threadIdx.x);
-  if (threadIdx.x != 0) return;
-  const TilePlan& pl = B.plan[IO];
-  if (e >= 0) {
-    const int x = B.remap ? t / pl.per : 0;
-    pl.list[x * pl.per + atomicAdd(&pl.cnt[x], 1)] = e;
-  } else if (IO == 1) {
-    B.part[B.nBox + B.nRing + t] = 0.0;
+  int t = (blockIdx.x * 4 + threadIdx.y) * (64 / PLAN_LANES) + threadIdx.x / PLAN_LANES;
+  int tx = 0, ty = 0;
+  const Member* m = nullptr;
+  if (t < B.nStage) {
+    m = &B.mem[B.mapStage[t]];
+    if (halted_post(m->st) || !tile_of(m->g, TY, t - m->stage0, tx, ty)) t = -1;
+  } else {
+    t = -1;
   }
+  const Grid g = m ? m->g : Grid{};
+  const DryMap dm = m ? m->dry : DryMap{};
+  const T* Uin = m ? (const T*)(IO == 0 ? m->U0 : m->U1) : nullptr;
+  const int e = plan_tile<IO, T, TY>(g, Uin, dm, tx, ty, t, threadIdx.x % PLAN_LANES);
+  const TilePlan& pl = B.plan[IO];
+  const bool leader = t >= 0 && threadIdx.x % PLAN_LANES == 0;
+  plan_append(pl, leader, e, t >= 0 ? t / pl.per : 0);
+  if (IO == 1 && leader && e < 0) B.part[B.nBox + B.nRing + t] = 0.0;
 }
 
-template <int IO, typename T, int TY, int W>
+// PLANNED: dry-tile skipping, one block per plan-list slot (halted members have no
+// entries); a kernel of its own, so the dense kernel keeps its registers.
+template <int IO, typename T, int TY, int W, bool PLANNED>
 __global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per_eu(W, W)))
     kb_stage(BatchDev B) {
   __shared__ T F[ST_NQ][TY + 1][64];
   const int b = blockIdx.x;
   int t = b;
-  if (B.plan_on) {   // dry-tile skipping: the plan's entry j of band x
-    const TilePlan& pl = B.plan[IO];
-    const int x = B.remap ? b % N_XCD : 0, j = B.remap ? b / N_XCD : b;
-    if (j >= pl.cnt[x]) return;
-    t = pl.list[x * pl.per + j];
+  bool zero = false;
+  if (PLANNED) {
+    const int e = plan_entry(B.plan[IO]);
+    if (e < 0) return;
+    t = e & ~PLAN_ZERO;
+    zero = IO == 0 && (e & PLAN_ZERO);
   } else if (B.remap) {
     const int per = (B.nStage + N_XCD - 1) / N_XCD;
     t = (b % N_XCD) * per + b / N_XCD;
   }
-  const bool zero = (t & PLAN_ZERO) != 0;
-  t &= ~PLAN_ZERO;
   if (t >= B.nStage) return;
   const Member& m = B.mem[B.mapStage[t]];
   DevState* st = m.st;
@@ -128,7 +130,7 @@ __global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per
   if (!tile_of(m.g, TY, t - m.stage0, tx, ty)) return;
   const T* Uin = (const T*)(IO == 0 ? m.U0 : m.U1);
   T* Uout = (T*)(IO == 0 ? m.U1 : m.U0);
-  if (IO == 0 && zero) {
+  if (zero) {
     stage_zero<T, TY>(m.g, Uout, (T*)m.ONE, m.dry, tx, ty);
     return;
   }
@@ -141,14 +143,22 @@ __global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per
 template <typename T, int TY, int W>
 static void stage_pair(const BatchDev& B, int io, hipStream_t s) {
   const dim3 blk(64, TY + 1);
-  const unsigned nb = B.remap ? (unsigned)(((B.nStage + N_XCD - 1) / N_XCD) * N_XCD) : (unsigned)B.nStage;
-  const unsigned np = (unsigned)((B.nStage + 3) / 4);
-  if (io == 0) {
-    if (B.plan_on) kb_plan<0, T, TY><<<np, dim3(64, 4), 0, s>>>(B);
-    kb_stage<0, T, TY, W><<<nb, blk, 0, s>>>(B);
+  const unsigned nb = B.plan_on ? (unsigned)plan_blocks(B.nStage)
+                   : B.remap ? (unsigned)(((B.nStage + N_XCD - 1) / N_XCD) * N_XCD) : (unsigned)B.nStage;
+  const unsigned np = (unsigned)((B.nStage + 4 * (64 / PLAN_LANES) - 1) / (4 * (64 / PLAN_LANES)));
+  if (B.plan_on) {
+    if (io == 0) {
+      kb_plan<0, T, TY><<<np, dim3(64, 4), 0, s>>>(B);
+      kb_stage<0, T, TY, W, true><<<nb, blk, 0, s>>>(B);
+    } else {
+      kb_plan<1, T, TY><<<np, dim3(64, 4), 0, s>>>(B);
+      kb_stage<1, T, TY, W, true><<<nb, blk, 0, s>>>(B);
+    }
   } else {
-    if (B.plan_on) kb_plan<1, T, TY><<<np, dim3(64, 4), 0, s>>>(B);
-    kb_stage<1, T, TY, W><<<nb, blk, 0, s>>>(B);
+    if (io == 0)
+      kb_stage<0, T, TY, W, false><<<nb, blk, 0, s>>>(B);
+    else
+      kb_stage<1, T, TY, W, false><<<nb, blk, 0, s>>>(B);
   }
 }
 

@@ -180,10 +180,10 @@ int setup(m2p_ctx* c) {
   c->dry.nseg = c->par.dry_skip && m2p::dry_capable(c->g) ? m2p::dry_segments(c->g) : 0;
   m2p::launch_dry_fill(c->g, c->dry, c->s);
   const int per = (m2p::plain_stage_tiles(c->g, c->prec) + 7) / 8;
-  if ((rc = dalloc(c, &c->dPlan, 16 + 16 * (size_t)per))) return rc;
+  if ((rc = dalloc(c, &c->dPlan, m2p::PLAN_COUNTERS + 16 * (size_t)per))) return rc;
   for (int io = 0; io < 2; io++) {
     c->plan[io].cnt = c->dPlan + 8 * io;
-    c->plan[io].list = c->dPlan + 16 + 8 * (size_t)per * io;
+    c->plan[io].list = c->dPlan + m2p::PLAN_COUNTERS + 8 * (size_t)per * io;
     c->plan[io].per = per;
   }
 

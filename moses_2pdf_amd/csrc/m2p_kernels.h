@@ -55,18 +55,22 @@ inline long long dry_flag_count(const Grid& g) { return (long long)g.ny * dry_se
 void launch_dry_fill(const Grid& g, const DryMap& dm, hipStream_t s);
 
 // Tile plan of one stage launch under dry-tile skipping.  A plan kernel tests every tile
-// (one wave per tile) and lists, per XCD band of the stage grid (band x holds tiles
-// [x * per, (x + 1) * per)), the tiles the stage must visit: those to compute and, for the
-// predictor, dry tiles whose U1 and ONE must be cleared (entry | PLAN_ZERO).  Block j of
-// band x of the stage takes entry j of list x; blocks past the list's end return at once,
-// so dry tiles cost no more than a block launch.  The counters are zeroed by the dt kernel
-// of the step.
+// (8 lanes per tile) and lists, per XCD band (band x holds tiles [x * per, (x + 1) * per)),
+// the tiles the stage must visit: those to compute and, for the predictor, dry tiles whose
+// U1 and ONE must be cleared (entry | PLAN_ZERO).  Block b of the stage runs on XCD b % 8
+// and takes entry b / 8 of list b % 8; blocks past the list's end return at once, so a dry
+// tile costs one block launch, and the hardware's block dispatch balances the wet ones
+// (a bounded grid striding over the lists was measured slower: mixed tile costs left it
+// unbalanced).  The list lengths are zeroed by the dt kernel of the step.
 constexpr int PLAN_ZERO = 1 << 30;
+constexpr int PLAN_COUNTERS = 16;   // the 8 list lengths of each of the two stages
 struct TilePlan {
   int* list;   // 8 * per entries
-  int* cnt;    // 8 counters
+  int* cnt;    // 8 list lengths
   int per;     // band size: ceil(tiles / 8)
 };
+// Blocks of a planned stage launch: one per list slot.
+inline int plan_blocks(int tiles) { return 8 * ((tiles + 7) / 8 > 0 ? (tiles + 7) / 8 : 1); }
 
 // A scenario is halted for the kernels that run before the dt kernel of a step
 // (inflow, ghost fill, CFL) once it has done halt_at steps, and for those after it once
@@ -102,7 +106,7 @@ size_t partial_count(const Grid& g, int prec);
 template <typename T>
 void launch_cfl(const Grid& g, const Phys& ph, const T* U, const T* M, DevState* st, double* part,
                 const DryMap& dm, hipStream_t s);
-// plan_cnt: the 16 plan counters of the step's two stages (zeroed here), or null.
+// plan_cnt: the PLAN_COUNTERS plan counters of the step's two stages (zeroed here), or null.
 void launch_dt(const DevParams& dp, DevState* st, const double* tf, const double* itime,
                const Grid& g, int prec, const double* part, int* plan_cnt, hipStream_t s);
 // Tiles of the single-scenario stage launch, and the plan kernel before it.

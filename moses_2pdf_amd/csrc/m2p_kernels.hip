@@ -262,7 +262,7 @@ __global__ void __launch_bounds__(256) k_dt(DevParams dp, DevState* __restrict__
   if (threadIdx.x == 0) halted = dt_halted(st) ? 1 : 0;
   __syncthreads();
   if (halted) return;
-  if (plan_cnt && threadIdx.x < 16) plan_cnt[threadIdx.x] = 0;
+  if (plan_cnt && threadIdx.x < PLAN_COUNTERS) plan_cnt[threadIdx.x] = 0;
   const double mx = block_max_of(part, ncfl);
   if (threadIdx.x == 0) dt_body(dp, st, tf, itime, mx);
 }
@@ -285,7 +285,9 @@ template <typename T> struct PlainStage;
 template <> struct PlainStage<double> { static constexpr int TY = 7, W = 4; };
 template <> struct PlainStage<float> { static constexpr int TY = 7, W = 6; };
 
-template <int IO, typename T, int TY, int W>
+// PLANNED: dry-tile skipping, one block per plan-list slot (a kernel of its own, so the
+// dense kernel keeps its registers).
+template <int IO, typename T, int TY, int W, bool PLANNED>
 __global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per_eu(W, W)))
     k_stage(Grid g, Phys ph, const T* __restrict__ Uin, const T* Uzero, T* Uout,
             T* __restrict__ ONE, const T* __restrict__ M, unsigned char* __restrict__ LIM,
@@ -293,20 +295,23 @@ __global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per
   __shared__ T F[ST_NQ][TY + 1][64];
   if (halted_post(st)) return;
   const int b = blockIdx.x;
-  const int per = (stage_tiles(g, TY) + N_XCD - 1) / N_XCD;
-  int tx, ty, t;
-  if (pl.list) {   // dry-tile skipping: the plan's entry j of band x
-    const int x = b % N_XCD, j = b / N_XCD;
-    if (j >= pl.cnt[x]) return;
-    t = pl.list[x * pl.per + j];
+  int t;
+  if (PLANNED) {   // dry-tile skipping: this block's plan entry
+    const int e = plan_entry(pl);
+    if (e < 0) return;
+    t = e & ~PLAN_ZERO;
+    if (IO == 0 && (e & PLAN_ZERO)) {
+      int tx, ty;
+      tile_of(g, TY, t, tx, ty);
+      stage_zero<T, TY>(g, Uout, ONE, dm, tx, ty);
+      return;
+    }
   } else {
+    const int per = (stage_tiles(g, TY) + N_XCD - 1) / N_XCD;
     t = (b % N_XCD) * per + b / N_XCD;
   }
-  if (!tile_of(g, TY, t & ~PLAN_ZERO, tx, ty)) return;
-  if (IO == 0 && (t & PLAN_ZERO)) {
-    stage_zero<T, TY>(g, Uout, ONE, dm, tx, ty);
-    return;
-  }
+  int tx, ty;
+  if (!tile_of(g, TY, t, tx, ty)) return;
   const double v2 = stage_tile<IO, T, TY>(F, g, ph, Uin, Uzero, Uout, ONE, M, LIM,
                                           IO == 0 ? st->dt_pred : st->dt_corr, tx, ty, dm);
   if (IO == 1 && st->vstop2 > 0.0) block_max_store(v2, part + t);
@@ -319,18 +324,13 @@ __global__ void __launch_bounds__(256) k_plan(Grid g, const T* __restrict__ Uin,
                                               TilePlan pl, double* __restrict__ spart,
                                               const DevState* __restrict__ st) {
   if (halted_post(st)) return;
-  const int t = blockIdx.x * 4 + threadIdx.y;
-  if (t >= stage_tiles(g, TY)) return;
-  int tx, ty;
-  tile_of(g, TY, t, tx, ty);
-  const int e = plan_tile<IO, T, TY>(g, Uin, dm, tx, ty, t, threadIdx.x);
-  if (threadIdx.x != 0) return;
-  if (e >= 0) {
-    const int x = t / pl.per;
-    pl.list[x * pl.per + atomicAdd(&pl.cnt[x], 1)] = e;
-  } else if (IO == 1) {
-    spart[t] = 0.0;
-  }
+  int t = (blockIdx.x * 4 + threadIdx.y) * (64 / PLAN_LANES) + threadIdx.x / PLAN_LANES;
+  int tx = 0, ty = 0;
+  if (!tile_of(g, TY, t, tx, ty)) t = -1;
+  const int e = plan_tile<IO, T, TY>(g, Uin, dm, tx, ty, t, threadIdx.x % PLAN_LANES);
+  const bool leader = t >= 0 && threadIdx.x % PLAN_LANES == 0;
+  plan_append(pl, leader, e, t >= 0 ? t / pl.per : 0);
+  if (IO == 1 && leader && e < 0) spart[t] = 0.0;
 }
 
 // Sum of u0*Det and u3*Det over interior cells with double-double accumulation; one
@@ -480,7 +480,7 @@ template <typename T>
 void launch_plan(int io, const Grid& g, const T* Uin, const DryMap& dm, const TilePlan& pl,
                  double* part, const DevState* st, hipStream_t s) {
   constexpr int TY = PlainStage<T>::TY;
-  const unsigned nb = (unsigned)((stage_tiles(g, TY) + 3) / 4);
+  const unsigned nb = (unsigned)((stage_tiles(g, TY) + 4 * (64 / PLAN_LANES) - 1) / (4 * (64 / PLAN_LANES)));
   double* sp = part + cfl_box_blocks(g) + cfl_ring_blocks(g);
   if (io == 0)
     k_plan<0, T, TY><<<nb, dim3(64, 4), 0, s>>>(g, Uin, dm, pl, sp, st);
@@ -499,12 +499,20 @@ void launch_stage(int io, const Grid& g, const Phys& ph, const T* Uin, const T* 
                   const DryMap& dm, const TilePlan& pl, hipStream_t s) {
   constexpr int TY = PlainStage<T>::TY, W = PlainStage<T>::W;
   const dim3 b(64, TY + 1);
-  const dim3 gr(((stage_tiles(g, TY) + N_XCD - 1) / N_XCD) * N_XCD);
+  const dim3 gr(pl.list ? plan_blocks(stage_tiles(g, TY))
+                        : ((stage_tiles(g, TY) + N_XCD - 1) / N_XCD) * N_XCD);
   double* sp = part + cfl_box_blocks(g) + cfl_ring_blocks(g);
-  if (io == 0)
-    k_stage<0, T, TY, W><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
-  else
-    k_stage<1, T, TY, W><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
+  if (pl.list) {
+    if (io == 0)
+      k_stage<0, T, TY, W, true><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
+    else
+      k_stage<1, T, TY, W, true><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
+  } else {
+    if (io == 0)
+      k_stage<0, T, TY, W, false><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
+    else
+      k_stage<1, T, TY, W, false><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
+  }
 }
 
 int mass_blocks() { return 512; }
@@ -536,7 +544,7 @@ M2P_INSTANTIATE(float)
 hipError_t probe_kernels() {
   hipFuncAttributes a;
   return hipFuncGetAttributes(&a, reinterpret_cast<const void*>(
-                                   &k_stage<0, double, PlainStage<double>::TY, PlainStage<double>::W>));
+                                   &k_stage<0, double, PlainStage<double>::TY, PlainStage<double>::W, false>));
 }
 
 }  // namespace m2p

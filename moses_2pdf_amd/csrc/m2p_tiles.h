@@ -684,22 +684,32 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
 // input (face stencils, the boundary overrides of edge tiles and the viscous term's
 // tangential slopes) and its own cells of uzero (and of ONE, which is zero wherever U1
 // is).  When all of them are zero, every output is +0 and the flow speed 0.
-// plan_tile is called by every lane of one wave and returns the tile's plan entry: t to
-// compute it, t | PLAN_ZERO (predictor) to clear its U1 and ONE because its U1 flags say
-// they may hold values of an earlier step, or -1 for nothing to do.
+// plan_tile is called by the PLAN_LANES lanes of one lane group (aligned within a wave,
+// lane index q in the group) and returns, in every lane of the group, the tile's plan
+// entry: t to compute it, t | PLAN_ZERO (predictor) to clear its U1 and ONE because its U1
+// flags say they may hold values of an earlier step, or -1 for nothing to do.  A group
+// whose tile does not exist passes t < 0 (it still takes part in the wave's ballots).
+constexpr int PLAN_LANES = 8;
+
 template <int IO, typename T, int TY>
 __device__ __forceinline__ int plan_tile(const Grid& g, const T* __restrict__ Uin,
-                                         const DryMap& dm, int tx, int ty, int t, int lane) {
+                                         const DryMap& dm, int tx, int ty, int t, int q) {
+  int wet = 0, own = 0;
+  if (t >= 0 && dm.nseg > 0) {
+    const int c0 = tx * ST_TX + MD, r0 = ty * TY + MD, ymax = g.ny + MD - 1;
+    const DryRegion d = dry_region_of(g, r0 - 2, r0 + TY + 1, c0 - 2, c0 + 64);
+#pragma unroll 3
+    for (int e = q; e < d.n; e += PLAN_LANES)
+      wet |= dry_item_wet(d, Uin, g, IO == 0 ? dm.z0 : dm.z1, dm.nseg, e);
+    const unsigned char* zown = IO == 0 ? dm.z1 : dm.z0;   // flags of the tile's output rows
+    const int rown = min(TY, ymax - r0 + 1);
+    for (int k = q; k < rown; k += PLAN_LANES)
+      own |= zown[(long long)(r0 + k - MD) * dm.nseg + tx];
+  }
+  const unsigned long long grp = ((1ull << PLAN_LANES) - 1) << (threadIdx.x & ~(PLAN_LANES - 1) & 63);
+  const bool anywet = (__ballot(wet) & grp) != 0ull, anyown = (__ballot(own != 0) & grp) != 0ull;
+  if (t < 0) return -1;
   if (dm.nseg == 0) return t;
-  const int c0 = tx * ST_TX + MD, r0 = ty * TY + MD, ymax = g.ny + MD - 1;
-  const DryRegion d = dry_region_of(g, r0 - 2, r0 + TY + 1, c0 - 2, c0 + 64);
-  int wet = 0;
-  for (int e = lane; e < d.n; e += 64)
-    wet |= dry_item_wet(d, Uin, g, IO == 0 ? dm.z0 : dm.z1, dm.nseg, e);
-  const unsigned char* zown = IO == 0 ? dm.z1 : dm.z0;   // flags of the tile's output rows
-  const int rown = min(TY, ymax - r0 + 1);
-  const int own = lane < rown ? zown[(long long)(r0 + lane - MD) * dm.nseg + tx] : 0;
-  const bool anywet = __ballot(wet) != 0ull, anyown = __ballot(own) != 0ull;
   if (anywet || (IO == 1 && anyown)) return t;   // the corrector also needs uzero zero
   if (IO == 0 && anyown) return t | PLAN_ZERO;
   return -1;
@@ -721,6 +731,36 @@ __device__ __forceinline__ void stage_zero(const Grid& g, T* Uout, T* __restrict
     ONE[P + i] = T(0);
   }
   if (lane == 0) dm.z1[(long long)(r - MD) * dm.nseg + tx] = 0;
+}
+
+// The plan entry of a planned stage block: entry blockIdx.x / 8 of list blockIdx.x % 8, or
+// -1 past the list's end.
+__device__ __forceinline__ int plan_entry(const TilePlan& pl) {
+  const int x = blockIdx.x % 8, j = blockIdx.x / 8;
+  return j < pl.cnt[x] ? pl.list[x * pl.per + j] : -1;
+}
+
+// Appends the plan entries of the lane groups of one wave (e >= 0 in a group's leader lane,
+// band x) to the band lists with one atomic per band present in the wave (a wave's tiles
+// are consecutive, so usually one or two).  Every lane of the wave calls it.
+__device__ __forceinline__ void plan_append(const TilePlan& pl, bool leader, int e, int x) {
+  const int lane = threadIdx.x & 63;
+  bool pend = leader && e >= 0;
+  while (true) {
+    const unsigned long long any = __ballot(pend);
+    if (!any) return;
+    const int first = __ffsll((long long)any) - 1;
+    const int xf = __shfl(x, first, 64);
+    const bool mine = pend && x == xf;
+    const unsigned long long m = __ballot(mine);
+    int base = 0;
+    if (lane == first) base = atomicAdd(&pl.cnt[xf], __popcll(m));
+    base = __shfl(base, first, 64);
+    if (mine) {
+      pl.list[xf * pl.per + base + __popcll(m & ((1ull << lane) - 1))] = e;
+      pend = false;
+    }
+  }
 }
 
 // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).  Block b runs on
