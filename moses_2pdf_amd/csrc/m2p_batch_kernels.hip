@@ -36,14 +36,54 @@ __global__ void __launch_bounds__(GHOST_LINES) kb_ghost(BatchDev B, int post) {
   ghost_line(m.g, m.mode, U, prev, line, (int)blockIdx.y);
 }
 
+// Blocks stride over the batch's CFL boxes, 8 at a time: one round of loads tests all 8
+// (32 threads per box; members without dry-tile skipping are always computed), then each
+// box's maximum goes to its own partial slot (a dry box's is the wave-speed floor).
+// Halted members' slots are left alone: their dt kernel does not read them.
+constexpr int BATCH_CFL_BLOCKS = 4096;
+
 template <typename T>
 __global__ void __launch_bounds__(256) kb_cfl(BatchDev B) {
-  const Member& m = B.mem[B.mapBox[blockIdx.x]];
-  if (halted_pre(m.st)) return;
-  const int lb = (int)blockIdx.x - m.box0, bx = lb % m.nbx, by = lb / m.nbx;
-  const bool dry = m.dry.nseg > 0 && cfl_box_dry((const T*)m.U0, m.g, m.dry, bx, by);
-  block_max_store(dry ? cfl_floor<T>(m.g) : cfl_box((const T*)m.U0, (const T*)m.MT, m.g, m.ph, bx, by),
-                  B.part + blockIdx.x);
+  __shared__ int state[8];   // 0: dry, 1: compute, 2: halted
+  const int tid = threadIdx.y * 64 + threadIdx.x;
+  for (int t0 = blockIdx.x; t0 < B.nBox; t0 += 8 * gridDim.x) {
+    __syncthreads();   // every thread is done with the previous group's state
+    if (tid < 8) state[tid] = 0;
+    __syncthreads();
+    {
+      const int k = tid >> 5, t = t0 + k * gridDim.x;
+      if (t < B.nBox) {
+        const Member& m = B.mem[B.mapBox[t]];
+        if (halted_pre(m.st)) {
+          if ((tid & 31) == 0) state[k] = 2;
+        } else if (m.dry.nseg == 0) {
+          if ((tid & 31) == 0) state[k] = 1;
+        } else {
+          const int lb = t - m.box0;
+          const DryRegion d = box_region(m.g, lb % m.nbx, lb / m.nbx);
+          int wet = 0;
+          for (int e = tid & 31; e < d.n; e += 32)
+            wet |= dry_item_wet(d, (const T*)m.U0, m.g, m.dry.z0, m.dry.nseg, e);
+          if (wet) atomicOr(&state[k], 1);
+        }
+      }
+    }
+    __syncthreads();
+    for (int k = 0; k < 8; k++) {
+      const int t = t0 + k * gridDim.x;
+      if (t >= B.nBox) break;
+      const int s = state[k];
+      if (s == 2) continue;
+      const Member& m = B.mem[B.mapBox[t]];
+      if (s == 0) {
+        if (tid == 0) B.part[t] = cfl_floor<T>(m.g);
+        continue;
+      }
+      const int lb = t - m.box0;
+      block_max_store(cfl_box((const T*)m.U0, (const T*)m.MT, m.g, m.ph, lb % m.nbx, lb / m.nbx),
+                      B.part + t);
+    }
+  }
 }
 
 template <typename T>
@@ -190,7 +230,7 @@ hipError_t launch_batch_steps(const BatchDev& B, int ty, int w, bool any_inflow,
   for (int n = 0; n < nsteps; n++) {
     if (any_inflow) kb_inflow<T><<<B.nmem, 64, 0, s>>>(B);
     kb_ghost<T><<<dim3(B.nGhost, MN), GHOST_LINES, 0, s>>>(B, 0);
-    kb_cfl<T><<<B.nBox, dim3(64, 4), 0, s>>>(B);
+    kb_cfl<T><<<B.nBox < BATCH_CFL_BLOCKS ? B.nBox : BATCH_CFL_BLOCKS, dim3(64, 4), 0, s>>>(B);
     kb_ring<T><<<B.nRing, 64, 0, s>>>(B);
     kb_dt<T><<<B.nmem, 256, 0, s>>>(B);
     if (!launch_stage_variant<T>(B, ty, w, 0, s)) return hipErrorInvalidValue;

@@ -183,8 +183,9 @@ __device__ __forceinline__ double face_cfl(const T* __restrict__ U, const T* __r
 }
 
 // Max over the block of a per-thread value (fmax: NaN ignored unless all are NaN), written
-// by thread 0 to *out.  Every thread of the block must call it.  Each block writes its own
-// slot and the dt kernel reduces the slots, so no two blocks contend for one address.
+// by thread 0 to *out.  Every thread of the block must call it; it may be called again
+// right away (it ends with a barrier).  Each block writes its own slot and the dt kernel
+// reduces the slots, so no two blocks contend for one address.
 __device__ __forceinline__ void block_max_store(double m, double* out) {
   m = wave_max(m);
   __shared__ double part[16];
@@ -197,6 +198,7 @@ __device__ __forceinline__ void block_max_store(double m, double* out) {
     for (int w = 1; w < nw; w++) b = fmax(b, part[w]);
     *out = b;
   }
+  __syncthreads();
 }
 
 // fmax of p[0..n) by all threads of a 1-D block; every thread gets the result.  The max
