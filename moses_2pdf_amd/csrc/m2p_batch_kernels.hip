@@ -91,7 +91,7 @@ __global__ void __launch_bounds__(64) kb_ring(BatchDev B) {
   const Member& m = B.mem[B.mapRing[blockIdx.x]];
   if (halted_pre(m.st)) return;
   const int t = ((int)blockIdx.x - m.ring0) * 64 + threadIdx.x;
-  double v = cfl_ring_face((const T*)m.U0, (const T*)m.MT, m.g, m.ph, t);
+  double v = cfl_ring_face((const T*)m.U0, (const T*)m.MT, m.g, m.ph, t, m.dry);
   v = wave_max(v);
   if (threadIdx.x == 0) B.part[B.nBox + blockIdx.x] = v;
 }

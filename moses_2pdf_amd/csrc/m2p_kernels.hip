@@ -245,9 +245,9 @@ __global__ void __launch_bounds__(256) k_cfl(Grid g, Phys ph, const T* __restric
 template <typename T>
 __global__ void __launch_bounds__(64) k_cfl_ring(Grid g, Phys ph, const T* __restrict__ U,
                                                  const T* __restrict__ M, DevState* __restrict__ st,
-                                                 double* __restrict__ part) {
+                                                 double* __restrict__ part, DryMap dm) {
   if (halted_pre(st)) return;
-  double m = cfl_ring_face(U, M, g, ph, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+  double m = cfl_ring_face(U, M, g, ph, (int)(blockIdx.x * blockDim.x + threadIdx.x), dm);
   m = wave_max(m);
   if (threadIdx.x == 0) part[blockIdx.x] = m;
 }
@@ -462,7 +462,7 @@ void launch_cfl(const Grid& g, const Phys& ph, const T* U, const T* M, DevState*
   const dim3 b(64, 4);
   k_cfl<T><<<cfl_box_blocks(g), b, 0, s>>>(g, ph, U, M, st, part, (g.nx + 63) / 64, cfl_boxes(g),
                                           dm);
-  k_cfl_ring<T><<<cfl_ring_blocks(g), 64, 0, s>>>(g, ph, U, M, st, part + cfl_box_blocks(g));
+  k_cfl_ring<T><<<cfl_ring_blocks(g), 64, 0, s>>>(g, ph, U, M, st, part + cfl_box_blocks(g), dm);
 }
 
 void launch_dt(const DevParams& dp, DevState* st, const double* tf, const double* itime,

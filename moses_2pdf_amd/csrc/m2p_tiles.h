@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <type_traits>
 
 #include "m2p_kernels.h"
@@ -283,12 +284,14 @@ __device__ __forceinline__ int dry_item_wet(const DryRegion& d, const T* __restr
 }
 
 // True when the region holds only zeros.  Every thread of the block must call it; tid in
-// [0, nthr).
+// [0, nthr).  A region of more than max_items items is not tested (false).
 template <typename T>
 __device__ __forceinline__ bool dry_region(const T* __restrict__ U, const Grid& g,
                                            const unsigned char* __restrict__ z, int nseg, int ra,
-                                           int rb, int ca, int cb, int tid, int nthr) {
+                                           int rb, int ca, int cb, int tid, int nthr,
+                                           int max_items = INT_MAX) {
   const DryRegion d = dry_region_of(g, ra, rb, ca, cb);
+  if (d.n > max_items) return false;
   int wet = 0;
   for (int e = tid; e < d.n; e += nthr) wet |= dry_item_wet(d, U, g, z, nseg, e);
   return !__syncthreads_or(wet);
@@ -337,15 +340,14 @@ __host__ __device__ __forceinline__ int ring_threads(const Grid& g) {
   return 2 * (4 * (g.nxd - 3) + 4 * g.ny);
 }
 
-template <typename T>
-__device__ __forceinline__ double cfl_ring_face(const T* __restrict__ U, const T* __restrict__ M,
-                                                const Grid& g, const Phys& ph, int t) {
-  const int f = t >> 1, dir = t & 1;
+// Position of ring thread t's face; false past the ring.
+__device__ __forceinline__ bool ring_face_pos(const Grid& g, int t, int& r, int& c, int& dir) {
+  const int f = t >> 1;
+  dir = t & 1;
   const int w = g.nxd - 3;        // columns 1..N-3
   const int nrow = 4 * w;         // 4 ghost face rows
   const int ncol = 4 * g.ny;      // 4 ghost face columns on interior rows
-  if (f >= nrow + ncol) return 0.0;
-  int r, c;
+  if (f >= nrow + ncol) return false;
   if (f < nrow) {
     const int k = f / w;
     r = (k < 2) ? 1 + k : g.ny + MD + (k - 2);
@@ -355,7 +357,44 @@ __device__ __forceinline__ double cfl_ring_face(const T* __restrict__ U, const T
     r = MD + q % g.ny;
     c = (k < 2) ? 1 + k : g.nx + MD + (k - 2);
   }
-  return face_cfl(U, M, g, ph, dir, r, c, true);
+  return true;
+}
+
+// The CFL value of ring thread t (0 past the ring).  Called by every lane of a one-wave
+// block.  With dry-tile skipping the wave first tests the cells its faces read: a ring face
+// on a ghost row or column takes its states from rows 3,4 (ny+2,ny+1) and columns 3,4
+// (nx+2,nx+1) instead (Interfaces2Kernel), and base_states reads a row r' or column c'
+// and the cells from r'-1 to r'+2 (c'-1 to c'+2); so the wave's faces read within the
+// bounding box of those effective rows and columns, widened by 1 before and 2 after.  When
+// it is dry, each face gives the floor 1e-10 over its spacing, as computing it would.  A
+// wave whose faces lie on two strips far apart (its box would span the grid) is not tested.
+template <typename T>
+__device__ __forceinline__ double cfl_ring_face(const T* __restrict__ U, const T* __restrict__ M,
+                                                const Grid& g, const Phys& ph, int t,
+                                                const DryMap& dm) {
+  int r = 0, c = 0, dir = 0;
+  const bool valid = ring_face_pos(g, t, r, c, dir);
+  if (dm.nseg > 0) {
+    int ra = INT_MAX, rb = INT_MIN, ca = INT_MAX, cb = INT_MIN;
+    if (valid) {
+      const bool gr = r <= 2 || r >= g.ny + MD, gc = c <= 2 || c >= g.nx + MD;
+      ra = gr ? (r <= 2 ? MD : g.ny + MD - 2) : r;
+      rb = gr ? (r <= 2 ? MD + 1 : g.ny + MD - 1) : r;
+      ca = gc ? (c <= 2 ? MD : g.nx + MD - 2) : c;
+      cb = gc ? (c <= 2 ? MD + 1 : g.nx + MD - 1) : c;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      ra = min(ra, __shfl_xor(ra, off, 64));
+      rb = max(rb, __shfl_xor(rb, off, 64));
+      ca = min(ca, __shfl_xor(ca, off, 64));
+      cb = max(cb, __shfl_xor(cb, off, 64));
+    }
+    if (rb >= ra && dry_region(U, g, dm.z0, dm.nseg, ra - 1, rb + 2, ca - 1, cb + 2,
+                               (int)(threadIdx.x & 63), 64, 4 * 64))
+      return valid ? (double)T(1.0e-10) / (dir == 0 ? g.dx : g.dy) : 0.0;
+  }
+  return valid ? face_cfl(U, M, g, ph, dir, r, c, true) : 0.0;
 }
 
 // CFL3Kernel + the host's dt control + CFL4Kernel (L/K:2006-2036, L/R:978-991; inflow

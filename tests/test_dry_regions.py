@@ -76,3 +76,59 @@ def test_dry_regions_cover_their_stencils():
                 check_region(nx, ny, rlo - 1, rlo + 5, clo - 1, clo + 65)
                 n += 1
     assert n > 1000
+
+
+def ring_face_pos(nx, ny, t):
+    nxd = nx + 2 * MD
+    f, d = t >> 1, t & 1
+    w = nxd - 3
+    nrow, ncol = 4 * w, 4 * ny
+    if f >= nrow + ncol:
+        return None
+    if f < nrow:
+        k = f // w
+        r = 1 + k if k < 2 else ny + MD + (k - 2)
+        c = 1 + f % w
+    else:
+        q = f - nrow
+        k = q // ny
+        r = MD + q % ny
+        c = 1 + k if k < 2 else nx + MD + (k - 2)
+    return r, c, d
+
+
+def ring_face_reads(nx, ny, r, c, d):
+    """Cells final_states_edge reads for face (r, c, dir d): Interfaces2's row and column
+    overrides, then base_states' 4-cell stencil."""
+    rows = [MD, MD + 1] if r <= 2 else [ny + MD - 1, ny + MD - 2] if r >= ny + MD else [r]
+    cols = [MD, MD + 1] if c <= 2 else [nx + MD - 1, nx + MD - 2] if c >= nx + MD else [c]
+    out = set()
+    for rr in rows:
+        for cc in cols:
+            for k in range(-1, 3):
+                out.add((rr, cc + k) if d == 0 else (rr + k, cc))
+    return out
+
+
+def test_ring_dry_regions_cover_their_faces():
+    for nx, ny in [(2, 2), (5, 3), (40, 70), (63, 63), (130, 64), (257, 129)]:
+        nthreads = 2 * (4 * (nx + 2 * MD - 3) + 4 * ny)
+        for b in range((nthreads + 63) // 64):
+            faces = [ring_face_pos(nx, ny, t) for t in range(64 * b, 64 * b + 64)]
+            faces = [f for f in faces if f]
+            ra = rb = ca = cb = None
+            for r, c, d in faces:
+                gr, gc = r <= 2 or r >= ny + MD, c <= 2 or c >= nx + MD
+                lo_r = (MD if r <= 2 else ny + MD - 2) if gr else r
+                hi_r = (MD + 1 if r <= 2 else ny + MD - 1) if gr else r
+                lo_c = (MD if c <= 2 else nx + MD - 2) if gc else c
+                hi_c = (MD + 1 if c <= 2 else nx + MD - 1) if gc else c
+                ra = lo_r if ra is None else min(ra, lo_r)
+                rb = hi_r if rb is None else max(rb, hi_r)
+                ca = lo_c if ca is None else min(ca, lo_c)
+                cb = hi_c if cb is None else max(cb, hi_c)
+            reg = (ra - 1, rb + 2, ca - 1, cb + 2)
+            for r, c, d in faces:
+                for (rr, cc) in ring_face_reads(nx, ny, r, c, d):
+                    assert reg[0] <= rr <= reg[1] and reg[2] <= cc <= reg[3]
+            check_region(nx, ny, *reg)
