@@ -174,7 +174,7 @@ __global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per
     stage_zero<T, TY>(m.g, Uout, (T*)m.ONE, m.dry, tx, ty);
     return;
   }
-  const double v2 = stage_tile<IO, T, TY>(F, m.g, m.ph, Uin, (const T*)m.U0, Uout, (T*)m.ONE,
+  const double v2 = stage_tile<IO, T, TY, PLANNED>(F, m.g, m.ph, Uin, (const T*)m.U0, Uout, (T*)m.ONE,
                                           (const T*)m.MT, m.LIM,
                                           IO == 0 ? st->dt_pred : st->dt_corr, tx, ty, m.dry);
   if (IO == 1 && st->vstop2 > 0.0) block_max_store(v2, B.part + B.nBox + B.nRing + t);

@@ -542,7 +542,9 @@ __device__ __forceinline__ void flux_store(T (*Tl)[TY + 1][64], int w, int lane,
 // if its flow depth exceeds WET_SPEED (0 otherwise, and for IO = 0).
 constexpr double WET_SPEED = 1.0e-2;  // scaled depth (1 mm)
 
-template <int IO, typename T, int TY>
+// FLAGS: write the dry-tile flags of the output rows (only the planned stage kernels run
+// with dry-tile skipping on; the others are compiled without the flag code).
+template <int IO, typename T, int TY, bool FLAGS>
 __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, const Phys& ph,
                                            const T* __restrict__ Uin, const T* Uzero, T* Uout,
                                            T* __restrict__ ONE, const T* __restrict__ M,
@@ -697,7 +699,7 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
     un[4] = lim4;
     un[5] = lim5;
   }
-  if (dm.nseg > 0) {
+  if (FLAGS && dm.nseg > 0) {
     // this row segment's flag of the output state (lane 0 holds column c0, always active)
     bool nz = false;
 #pragma unroll
