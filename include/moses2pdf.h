@@ -71,8 +71,8 @@ void m2p_default_params(int mode, m2p_params* p);
  * par_List at `par_list` (9 lines landslide, 11 lines inflow; '#' starts a comment) and
  * the files it names (relative paths resolve against the current directory, as in the
  * reference), parses them with the reference's precision rules and uploads them.
- * `overlay` is NULL or "key=value;key=value" over cfl, rho_ratio, dt_min, dt_max and
- * precision (fp64 | fp32). */
+ * `overlay` is NULL or "key=value;key=value" over cfl, rho_ratio, dt_min, dt_max,
+ * precision (fp64 | fp32) and dry_skip (0 | 1). */
 int m2p_open(const char* par_list, int mode, const char* overlay, m2p_ctx** out);
 
 /* In-memory constructor (no files).  topo[NY*NX] and depth[NY*NX] hold the grid values
