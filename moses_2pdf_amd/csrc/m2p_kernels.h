@@ -109,6 +109,8 @@ void launch_cfl(const Grid& g, const Phys& ph, const T* U, const T* M, DevState*
 // plan_cnt: the PLAN_COUNTERS plan counters of the step's two stages (zeroed here), or null.
 void launch_dt(const DevParams& dp, DevState* st, const double* tf, const double* itime,
                const Grid& g, int prec, const double* part, int* plan_cnt, hipStream_t s);
+// Offset of the corrector tiles' partials in a scenario's partial array (after the CFL ones).
+size_t stage_partials_offset(const Grid& g);
 // Tiles of the single-scenario stage launch, and the plan kernel before it.
 int plain_stage_tiles(const Grid& g, int prec);
 template <typename T>

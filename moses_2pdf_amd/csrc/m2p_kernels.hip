@@ -4,7 +4,8 @@
 // (MoSES_2PDF_landslide/adNOC_2Pb_runKernels.cu:646-1330).  Here one step is a handful of
 // launches on one stream, with dt kept on the device:
 //   ghost(U0) -> cfl(U0) -> dt -> stage<0>(U0 -> U1) -> ghost(U1) -> stage<1>(U1,U0 -> U0)
-// (+ the inflow injection first in inflow mode).  A stage kernel fuses the reference's
+// (+ the inflow injection first in inflow mode).  The stage kernels are in m2p_stage.hip
+// (compiled with their own scheduling option).  A stage kernel fuses the reference's
 // TVD slopes, interface states, positivity, local speeds, central-upwind fluxes, pressure
 // and viscous terms, sources, update, wet/dry clipping and the velocity-crossing limiter
 // (L/K:1000-2937).  The per-step kernels are templated on the state's scalar type (double:
@@ -279,44 +280,6 @@ __global__ void __launch_bounds__(256) k_vmax(DevState* __restrict__ st,
   }
 }
 
-// Tile height and the waves-per-SIMD cap of the single-scenario stage kernel: FP64 is held
-// to 128 VGPRs (4 waves); the FP32 body needs ~80 VGPRs and runs at 6.
-template <typename T> struct PlainStage;
-template <> struct PlainStage<double> { static constexpr int TY = 7, W = 4; };
-template <> struct PlainStage<float> { static constexpr int TY = 7, W = 6; };
-
-// PLANNED: dry-tile skipping, one block per plan-list slot (a kernel of its own, so the
-// dense kernel keeps its registers).
-template <int IO, typename T, int TY, int W, bool PLANNED>
-__global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per_eu(W, W)))
-    k_stage(Grid g, Phys ph, const T* __restrict__ Uin, const T* Uzero, T* Uout,
-            T* __restrict__ ONE, const T* __restrict__ M, unsigned char* __restrict__ LIM,
-            DevState* __restrict__ st, double* __restrict__ part, DryMap dm, TilePlan pl) {
-  __shared__ T F[ST_NQ][TY + 1][64];
-  if (halted_post(st)) return;
-  const int b = blockIdx.x;
-  int t;
-  if (PLANNED) {   // dry-tile skipping: this block's plan entry
-    const int e = plan_entry(pl);
-    if (e < 0) return;
-    t = e & ~PLAN_ZERO;
-    if (IO == 0 && (e & PLAN_ZERO)) {
-      int tx, ty;
-      tile_of(g, TY, t, tx, ty);
-      stage_zero<T, TY>(g, Uout, ONE, dm, tx, ty);
-      return;
-    }
-  } else {
-    const int per = (stage_tiles(g, TY) + N_XCD - 1) / N_XCD;
-    t = (b % N_XCD) * per + b / N_XCD;
-  }
-  int tx, ty;
-  if (!tile_of(g, TY, t, tx, ty)) return;
-  const double v2 = stage_tile<IO, T, TY, PLANNED>(F, g, ph, Uin, Uzero, Uout, ONE, M, LIM,
-                                                   IO == 0 ? st->dt_pred : st->dt_corr, tx, ty, dm);
-  if (IO == 1 && st->vstop2 > 0.0) block_max_store(v2, part + t);
-}
-
 // One wave per stage tile: its plan entry into the list of its XCD band (tiles that need
 // nothing have no entry; in the corrector their flow-speed partial is set to 0).
 template <int IO, typename T, int TY>
@@ -493,26 +456,8 @@ void launch_vmax(DevState* st, const Grid& g, int prec, const double* part, hipS
   k_vmax<<<1, 256, 0, s>>>(st, part + cfl_box_blocks(g) + cfl_ring_blocks(g), stage_tiles(g, ty));
 }
 
-template <typename T>
-void launch_stage(int io, const Grid& g, const Phys& ph, const T* Uin, const T* Uzero, T* Uout,
-                  T* ONE, const T* M, unsigned char* LIM, DevState* st, double* part,
-                  const DryMap& dm, const TilePlan& pl, hipStream_t s) {
-  constexpr int TY = PlainStage<T>::TY, W = PlainStage<T>::W;
-  const dim3 b(64, TY + 1);
-  const dim3 gr(pl.list ? plan_blocks(stage_tiles(g, TY))
-                        : ((stage_tiles(g, TY) + N_XCD - 1) / N_XCD) * N_XCD);
-  double* sp = part + cfl_box_blocks(g) + cfl_ring_blocks(g);
-  if (pl.list) {
-    if (io == 0)
-      k_stage<0, T, TY, W, true><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
-    else
-      k_stage<1, T, TY, W, true><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
-  } else {
-    if (io == 0)
-      k_stage<0, T, TY, W, false><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
-    else
-      k_stage<1, T, TY, W, false><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
-  }
+size_t stage_partials_offset(const Grid& g) {
+  return (size_t)cfl_box_blocks(g) + cfl_ring_blocks(g);
 }
 
 int mass_blocks() { return 512; }
@@ -530,21 +475,10 @@ void launch_mass(const Grid& g, const T* U, const double* M, double* out, hipStr
                                 hipStream_t);                                                 \
   template void launch_cfl<T>(const Grid&, const Phys&, const T*, const T*, DevState*, double*, \
                               const DryMap&, hipStream_t);                                     \
-  template void launch_stage<T>(int, const Grid&, const Phys&, const T*, const T*, T*, T*,     \
-                                const T*, unsigned char*, DevState*, double*, const DryMap&,   \
-                                const TilePlan&, hipStream_t);                                 \
   template void launch_plan<T>(int, const Grid&, const T*, const DryMap&, const TilePlan&,     \
                                double*, const DevState*, hipStream_t);                         \
   template void launch_mass<T>(const Grid&, const T*, const double*, double*, hipStream_t);
 M2P_INSTANTIATE(double)
 M2P_INSTANTIATE(float)
-
-// Confirms the library holds a code object for the current device (a missing gfx950 image
-// otherwise surfaces only at the first launch).
-hipError_t probe_kernels() {
-  hipFuncAttributes a;
-  return hipFuncGetAttributes(&a, reinterpret_cast<const void*>(
-                                   &k_stage<0, double, PlainStage<double>::TY, PlainStage<double>::W, false>));
-}
 
 }  // namespace m2p

@@ -1,0 +1,82 @@
+// m2p_stage.hip -- the single-scenario stage kernels (predictor and corrector) and their
+// launcher.  They sit in a translation unit of their own so that the build can give them
+// a scheduling strategy (max-memory-clause) that helps them and slows the CFL kernel.
+// The device body is stage_tile (m2p_tiles.h), shared with the batched kernels.
+#include <hip/hip_runtime.h>
+
+#include "m2p_kernels.h"
+#include "m2p_numerics.h"
+#include "m2p_tiles.h"
+
+namespace m2p {
+
+// PLANNED: dry-tile skipping, one block per plan-list slot (a kernel of its own, so the
+// dense kernel keeps its registers).
+template <int IO, typename T, int TY, int W, bool PLANNED>
+__global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per_eu(W, W)))
+    k_stage(Grid g, Phys ph, const T* __restrict__ Uin, const T* Uzero, T* Uout,
+            T* __restrict__ ONE, const T* __restrict__ M, unsigned char* __restrict__ LIM,
+            DevState* __restrict__ st, double* __restrict__ part, DryMap dm, TilePlan pl) {
+  __shared__ T F[ST_NQ][TY + 1][64];
+  if (halted_post(st)) return;
+  const int b = blockIdx.x;
+  int t;
+  if (PLANNED) {   // dry-tile skipping: this block's plan entry
+    const int e = plan_entry(pl);
+    if (e < 0) return;
+    t = e & ~PLAN_ZERO;
+    if (IO == 0 && (e & PLAN_ZERO)) {
+      int tx, ty;
+      tile_of(g, TY, t, tx, ty);
+      stage_zero<T, TY>(g, Uout, ONE, dm, tx, ty);
+      return;
+    }
+  } else {
+    const int per = (stage_tiles(g, TY) + N_XCD - 1) / N_XCD;
+    t = (b % N_XCD) * per + b / N_XCD;
+  }
+  int tx, ty;
+  if (!tile_of(g, TY, t, tx, ty)) return;
+  const double v2 = stage_tile<IO, T, TY, PLANNED>(F, g, ph, Uin, Uzero, Uout, ONE, M, LIM,
+                                                   IO == 0 ? st->dt_pred : st->dt_corr, tx, ty, dm);
+  if (IO == 1 && st->vstop2 > 0.0) block_max_store(v2, part + t);
+}
+
+template <typename T>
+void launch_stage(int io, const Grid& g, const Phys& ph, const T* Uin, const T* Uzero, T* Uout,
+                  T* ONE, const T* M, unsigned char* LIM, DevState* st, double* part,
+                  const DryMap& dm, const TilePlan& pl, hipStream_t s) {
+  constexpr int TY = PlainStage<T>::TY, W = PlainStage<T>::W;
+  const dim3 b(64, TY + 1);
+  const dim3 gr(pl.list ? plan_blocks(stage_tiles(g, TY))
+                        : ((stage_tiles(g, TY) + N_XCD - 1) / N_XCD) * N_XCD);
+  double* sp = part + stage_partials_offset(g);
+  if (pl.list) {
+    if (io == 0)
+      k_stage<0, T, TY, W, true><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
+    else
+      k_stage<1, T, TY, W, true><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
+  } else {
+    if (io == 0)
+      k_stage<0, T, TY, W, false><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
+    else
+      k_stage<1, T, TY, W, false><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
+  }
+}
+
+template void launch_stage<double>(int, const Grid&, const Phys&, const double*, const double*,
+                                   double*, double*, const double*, unsigned char*, DevState*,
+                                   double*, const DryMap&, const TilePlan&, hipStream_t);
+template void launch_stage<float>(int, const Grid&, const Phys&, const float*, const float*,
+                                  float*, float*, const float*, unsigned char*, DevState*,
+                                  double*, const DryMap&, const TilePlan&, hipStream_t);
+
+// Confirms the library holds a code object for the current device (a missing gfx950 image
+// otherwise surfaces only at the first launch).
+hipError_t probe_kernels() {
+  hipFuncAttributes a;
+  return hipFuncGetAttributes(&a, reinterpret_cast<const void*>(
+                                   &k_stage<0, double, PlainStage<double>::TY, PlainStage<double>::W, false>));
+}
+
+}  // namespace m2p

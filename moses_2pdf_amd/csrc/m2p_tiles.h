@@ -806,6 +806,12 @@ __device__ __forceinline__ void plan_append(const TilePlan& pl, bool leader, int
   }
 }
 
+// Tile height and the waves-per-SIMD cap of the single-scenario stage kernel: FP64 is held
+// to 128 VGPRs (4 waves); the FP32 body needs ~80 VGPRs and runs at 6.
+template <typename T> struct PlainStage;
+template <> struct PlainStage<double> { static constexpr int TY = 7, W = 4; };
+template <> struct PlainStage<float> { static constexpr int TY = 7, W = 6; };
+
 // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).  Block b runs on
 // XCD b % 8; it is given tile number (b % 8) * per + b / 8 in row-major tile order, so
 // each XCD sweeps a contiguous band of tile rows and vertically adjacent tiles, which
