@@ -64,6 +64,11 @@ int ring_blocks(const Grid& g);
 int box_blocks_x(const Grid& g);
 int box_blocks_y(const Grid& g);
 
+// Enqueues the plan (with dry-tile skipping) and the stage kernel of stage io for every
+// member; false when no kernel is compiled for (ty, w).
+template <typename T>
+bool launch_batch_stage(const BatchDev& B, int ty, int w, int io, hipStream_t s);
+
 // Enqueues `nsteps` time steps of every member (halted members return at once).
 template <typename T>
 hipError_t launch_batch_steps(const BatchDev& B, int ty, int w, bool any_inflow, int nsteps,

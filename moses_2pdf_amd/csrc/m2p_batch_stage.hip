@@ -1,0 +1,121 @@
+// m2p_batch_stage.hip -- the batched stage kernels (plan and stage, every compiled tile
+// shape) and their launcher, in a translation unit of their own so that the build can give
+// them the max-memory-clause scheduling strategy, as the single-scenario stage kernels.
+// Compiled once per scalar type: M2P_BATCH_T is double or float.
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "m2p_batch.h"
+#include "m2p_tiles.h"
+
+#ifndef M2P_BATCH_T
+#error "M2P_BATCH_T must be double or float"
+#endif
+
+namespace m2p {
+
+// One wave per stage tile of the batch: its plan entry (see TilePlan).
+template <int IO, typename T, int TY>
+__global__ void __launch_bounds__(256) kb_plan(BatchDev B) {
+  int t = (blockIdx.x * 4 + threadIdx.y) * (64 / PLAN_LANES) + threadIdx.x / PLAN_LANES;
+  int tx = 0, ty = 0;
+  const Member* m = nullptr;
+  if (t < B.nStage) {
+    m = &B.mem[B.mapStage[t]];
+    if (halted_post(m->st) || !tile_of(m->g, TY, t - m->stage0, tx, ty)) t = -1;
+  } else {
+    t = -1;
+  }
+  const Grid g = m ? m->g : Grid{};
+  const DryMap dm = m ? m->dry : DryMap{};
+  const T* Uin = m ? (const T*)(IO == 0 ? m->U0 : m->U1) : nullptr;
+  const int e = plan_tile<IO, T, TY>(g, Uin, dm, tx, ty, t, threadIdx.x % PLAN_LANES);
+  const TilePlan& pl = B.plan[IO];
+  const bool leader = t >= 0 && threadIdx.x % PLAN_LANES == 0;
+  plan_append(pl, leader, e, t >= 0 ? t / pl.per : 0);
+  if (IO == 1 && leader && e < 0) B.part[B.nBox + B.nRing + t] = 0.0;
+}
+
+// PLANNED: dry-tile skipping, one block per plan-list slot (halted members have no
+// entries); a kernel of its own, so the dense kernel keeps its registers.
+template <int IO, typename T, int TY, int W, bool PLANNED>
+__global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per_eu(W, W)))
+    kb_stage(BatchDev B) {
+  __shared__ T F[ST_NQ][TY + 1][64];
+  const int b = blockIdx.x;
+  int t = b;
+  bool zero = false;
+  if (PLANNED) {
+    const int e = plan_entry(B.plan[IO]);
+    if (e < 0) return;
+    t = e & ~PLAN_ZERO;
+    zero = IO == 0 && (e & PLAN_ZERO);
+  } else if (B.remap) {
+    const int per = (B.nStage + N_XCD - 1) / N_XCD;
+    t = (b % N_XCD) * per + b / N_XCD;
+  }
+  if (t >= B.nStage) return;
+  const Member& m = B.mem[B.mapStage[t]];
+  DevState* st = m.st;
+  if (halted_post(st)) return;
+  int tx, ty;
+  if (!tile_of(m.g, TY, t - m.stage0, tx, ty)) return;
+  const T* Uin = (const T*)(IO == 0 ? m.U0 : m.U1);
+  T* Uout = (T*)(IO == 0 ? m.U1 : m.U0);
+  if (zero) {
+    stage_zero<T, TY>(m.g, Uout, (T*)m.ONE, m.dry, tx, ty);
+    return;
+  }
+  const double v2 = stage_tile<IO, T, TY, PLANNED>(F, m.g, m.ph, Uin, (const T*)m.U0, Uout, (T*)m.ONE,
+                                          (const T*)m.MT, m.LIM,
+                                          IO == 0 ? st->dt_pred : st->dt_corr, tx, ty, m.dry);
+  if (IO == 1 && st->vstop2 > 0.0) block_max_store(v2, B.part + B.nBox + B.nRing + t);
+}
+
+template <typename T, int TY, int W>
+static void stage_pair(const BatchDev& B, int io, hipStream_t s) {
+  const dim3 blk(64, TY + 1);
+  const unsigned nb = B.plan_on ? (unsigned)plan_blocks(B.nStage)
+                   : B.remap ? (unsigned)(((B.nStage + N_XCD - 1) / N_XCD) * N_XCD) : (unsigned)B.nStage;
+  const unsigned np = (unsigned)((B.nStage + 4 * (64 / PLAN_LANES) - 1) / (4 * (64 / PLAN_LANES)));
+  if (B.plan_on) {
+    if (io == 0) {
+      kb_plan<0, T, TY><<<np, dim3(64, 4), 0, s>>>(B);
+      kb_stage<0, T, TY, W, true><<<nb, blk, 0, s>>>(B);
+    } else {
+      kb_plan<1, T, TY><<<np, dim3(64, 4), 0, s>>>(B);
+      kb_stage<1, T, TY, W, true><<<nb, blk, 0, s>>>(B);
+    }
+  } else {
+    if (io == 0)
+      kb_stage<0, T, TY, W, false><<<nb, blk, 0, s>>>(B);
+    else
+      kb_stage<1, T, TY, W, false><<<nb, blk, 0, s>>>(B);
+  }
+}
+
+// The compiled (tile height, waves per SIMD) variants; the list in stage_variants() must
+// match.
+template <typename T>
+bool launch_batch_stage(const BatchDev& B, int ty, int w, int io, hipStream_t s) {
+  if constexpr (std::is_same<T, double>::value) {
+    if (ty == 7 && w == 4) stage_pair<double, 7, 4>(B, io, s);
+    else if (ty == 3 && w == 4) stage_pair<double, 3, 4>(B, io, s);
+    else if (ty == 7 && w == 3) stage_pair<double, 7, 3>(B, io, s);
+    else return false;
+  } else {
+    if (ty == 7 && w == 6) stage_pair<float, 7, 6>(B, io, s);
+    else if (ty == 7 && w == 4) stage_pair<float, 7, 4>(B, io, s);
+    else if (ty == 3 && w == 6) stage_pair<float, 3, 6>(B, io, s);
+    else if (ty == 15 && w == 4) stage_pair<float, 15, 4>(B, io, s);
+    else if (ty == 3 && w == 8) stage_pair<float, 3, 8>(B, io, s);
+    else if (ty == 7 && w == 8) stage_pair<float, 7, 8>(B, io, s);
+    else return false;
+  }
+  return true;
+}
+
+template bool launch_batch_stage<M2P_BATCH_T>(const BatchDev&, int, int, int, hipStream_t);
+
+}  // namespace m2p
