@@ -1,12 +1,17 @@
 // m2p_stage.hip -- the single-scenario stage kernels (predictor and corrector) and their
-// launcher.  They sit in a translation unit of their own so that the build can give them
-// a scheduling strategy (max-memory-clause) that helps them and slows the CFL kernel.
-// The device body is stage_tile (m2p_tiles.h), shared with the batched kernels.
+// launcher.  They sit in a translation unit of their own, compiled once per scalar type
+// (M2P_STAGE_T is double or float), so that the build can give each the LLVM scheduling
+// strategy that suits it (a different one suits the CFL kernel).  The device body is
+// stage_tile (m2p_tiles.h), shared with the batched kernels.
 #include <hip/hip_runtime.h>
 
 #include "m2p_kernels.h"
 #include "m2p_numerics.h"
 #include "m2p_tiles.h"
+
+#ifndef M2P_STAGE_T
+#error "M2P_STAGE_T must be double or float"
+#endif
 
 namespace m2p {
 
@@ -64,13 +69,12 @@ void launch_stage(int io, const Grid& g, const Phys& ph, const T* Uin, const T* 
   }
 }
 
-template void launch_stage<double>(int, const Grid&, const Phys&, const double*, const double*,
-                                   double*, double*, const double*, unsigned char*, DevState*,
-                                   double*, const DryMap&, const TilePlan&, hipStream_t);
-template void launch_stage<float>(int, const Grid&, const Phys&, const float*, const float*,
-                                  float*, float*, const float*, unsigned char*, DevState*,
-                                  double*, const DryMap&, const TilePlan&, hipStream_t);
+template void launch_stage<M2P_STAGE_T>(int, const Grid&, const Phys&, const M2P_STAGE_T*,
+                                        const M2P_STAGE_T*, M2P_STAGE_T*, M2P_STAGE_T*,
+                                        const M2P_STAGE_T*, unsigned char*, DevState*, double*,
+                                        const DryMap&, const TilePlan&, hipStream_t);
 
+#ifdef M2P_STAGE_PROBE   // in the FP64 object only
 // Confirms the library holds a code object for the current device (a missing gfx950 image
 // otherwise surfaces only at the first launch).
 hipError_t probe_kernels() {
@@ -78,5 +82,7 @@ hipError_t probe_kernels() {
   return hipFuncGetAttributes(&a, reinterpret_cast<const void*>(
                                    &k_stage<0, double, PlainStage<double>::TY, PlainStage<double>::W, false>));
 }
+
+#endif
 
 }  // namespace m2p
