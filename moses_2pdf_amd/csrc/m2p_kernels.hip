@@ -201,7 +201,7 @@ __global__ void k_ghost(Grid g, int mode, T* __restrict__ Tg, const T* __restric
 
 // Blocks stride over the 64x4 face boxes (at most CFL_BLOCKS blocks), so the dt kernel
 // reduces few partials.
-constexpr int CFL_BLOCKS = 2048;
+constexpr int CFL_BLOCKS = 4096;
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_cfl(Grid g, Phys ph, const T* __restrict__ U,
