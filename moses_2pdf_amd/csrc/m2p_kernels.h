@@ -39,8 +39,9 @@ struct DevState {
 // segment s of row r holds columns [MD + 63 s, MD + 63 s + 62].  z0[(r - MD) * nseg + s]
 // is 0 only if every component of every cell of that segment is zero in U0 (u^n after the
 // inflow), z1 likewise for U1 (the predictor's output); a nonzero flag promises nothing.
-// Every stage tile rewrites the flags of its rows, so after one step they are exact; any
-// other write to the state sets them all to 1.  nseg == 0 switches the skipping off.
+// Every stage tile that is computed or cleared rewrites the flags of its rows (a skipped
+// tile's rows stay zero, as its flags say), so after one step they are exact; any other
+// write to the state sets them all to 1.  nseg == 0 switches the skipping off.
 struct DryMap {
   unsigned char* z0;
   unsigned char* z1;

@@ -311,15 +311,6 @@ __device__ __forceinline__ DryRegion box_region(const Grid& g, int bx, int by) {
   return dry_region_of(g, rlo - 1, rlo + 5, clo - 1, clo + 65);
 }
 
-// Dry test of one box by the whole block.
-template <typename T>
-__device__ __forceinline__ bool cfl_box_dry(const T* __restrict__ U, const Grid& g,
-                                            const DryMap& dm, int bx, int by) {
-  const int clo = MD + 64 * bx, rlo = MD + 4 * by;
-  return dry_region(U, g, dm.z0, dm.nseg, rlo - 1, rlo + 5, clo - 1, clo + 65,
-                    threadIdx.y * blockDim.x + threadIdx.x, blockDim.x * blockDim.y);
-}
-
 template <typename T>
 __device__ __forceinline__ double cfl_box(const T* __restrict__ U, const T* __restrict__ M,
                                           const Grid& g, const Phys& ph, int bx, int by) {
