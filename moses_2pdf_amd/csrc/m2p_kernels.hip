@@ -199,19 +199,25 @@ __global__ void k_ghost(Grid g, int mode, T* __restrict__ Tg, const T* __restric
   ghost_line(g, mode, Tg, prev, line, (int)blockIdx.y);
 }
 
-// Blocks stride over the 64x4 face boxes (at most CFL_BLOCKS blocks), so the dt kernel
-// reduces few partials.
+// Blocks [0, nbb) stride over the 64x4 face boxes (at most CFL_BLOCKS blocks, so the dt
+// kernel reduces few partials); the blocks after them take the ring faces, 256 ring threads
+// each (one wave per 32 faces), in the same launch.
 constexpr int CFL_BLOCKS = 4096;
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_cfl(Grid g, Phys ph, const T* __restrict__ U,
                                              const T* __restrict__ M, DevState* __restrict__ st,
                                              double* __restrict__ part, int nbx, int nbox,
-                                             DryMap dm) {
+                                             int nbb, DryMap dm) {
   if (halted_pre(st)) return;
   double m = 0.0;
+  if ((int)blockIdx.x >= nbb) {   // ring faces
+    const int t = ((int)blockIdx.x - nbb) * 256 + threadIdx.y * 64 + threadIdx.x;
+    block_max_store(cfl_ring_face(U, M, g, ph, t, dm), part + blockIdx.x);
+    return;
+  }
   if (dm.nseg == 0) {
-    for (int t = blockIdx.x; t < nbox; t += gridDim.x) m = fmax(m, cfl_box(U, M, g, ph, t % nbx, t / nbx));
+    for (int t = blockIdx.x; t < nbox; t += nbb) m = fmax(m, cfl_box(U, M, g, ph, t % nbx, t / nbx));
     block_max_store(m, part + blockIdx.x);
     return;
   }
@@ -220,12 +226,12 @@ __global__ void __launch_bounds__(256) k_cfl(Grid g, Phys ph, const T* __restric
   __shared__ int wetk[8];
   const int tid = threadIdx.y * 64 + threadIdx.x;
   const double fl = cfl_floor<T>(g);
-  for (int t0 = blockIdx.x; t0 < nbox; t0 += 8 * gridDim.x) {
+  for (int t0 = blockIdx.x; t0 < nbox; t0 += 8 * nbb) {
     __syncthreads();   // every thread is done with the previous group's wetk
     if (tid < 8) wetk[tid] = 0;
     __syncthreads();
     {
-      const int k = tid >> 5, t = t0 + k * gridDim.x;
+      const int k = tid >> 5, t = t0 + k * nbb;
       if (t < nbox) {
         const DryRegion d = box_region(g, t % nbx, t / nbx);
         int wet = 0;
@@ -235,22 +241,12 @@ __global__ void __launch_bounds__(256) k_cfl(Grid g, Phys ph, const T* __restric
     }
     __syncthreads();
     for (int k = 0; k < 8; k++) {
-      const int t = t0 + k * gridDim.x;
+      const int t = t0 + k * nbb;
       if (t >= nbox) break;
       m = fmax(m, wetk[k] ? cfl_box(U, M, g, ph, t % nbx, t / nbx) : fl);
     }
   }
   block_max_store(m, part + blockIdx.x);
-}
-
-template <typename T>
-__global__ void __launch_bounds__(64) k_cfl_ring(Grid g, Phys ph, const T* __restrict__ U,
-                                                 const T* __restrict__ M, DevState* __restrict__ st,
-                                                 double* __restrict__ part, DryMap dm) {
-  if (halted_pre(st)) return;
-  double m = cfl_ring_face(U, M, g, ph, (int)(blockIdx.x * blockDim.x + threadIdx.x), dm);
-  m = wave_max(m);
-  if (threadIdx.x == 0) part[blockIdx.x] = m;
 }
 
 // One block: the max over the CFL partials [0, ncfl), then the dt control in thread 0.
@@ -412,7 +408,7 @@ void launch_ghost(const Grid& g, int mode, T* Tg, const T* prev, const DevState*
 
 static int cfl_boxes(const Grid& g) { return ((g.nx + 63) / 64) * ((g.ny + 3) / 4); }
 static int cfl_box_blocks(const Grid& g) { return std::min(cfl_boxes(g), CFL_BLOCKS); }
-static int cfl_ring_blocks(const Grid& g) { return (ring_threads(g) + 63) / 64; }
+static int cfl_ring_blocks(const Grid& g) { return (ring_threads(g) + 255) / 256; }
 
 size_t partial_count(const Grid& g, int prec) {
   const int ty = prec == 1 ? PlainStage<float>::TY : PlainStage<double>::TY;
@@ -423,9 +419,8 @@ template <typename T>
 void launch_cfl(const Grid& g, const Phys& ph, const T* U, const T* M, DevState* st, double* part,
                 const DryMap& dm, hipStream_t s) {
   const dim3 b(64, 4);
-  k_cfl<T><<<cfl_box_blocks(g), b, 0, s>>>(g, ph, U, M, st, part, (g.nx + 63) / 64, cfl_boxes(g),
-                                          dm);
-  k_cfl_ring<T><<<cfl_ring_blocks(g), 64, 0, s>>>(g, ph, U, M, st, part + cfl_box_blocks(g), dm);
+  k_cfl<T><<<cfl_box_blocks(g) + cfl_ring_blocks(g), b, 0, s>>>(
+      g, ph, U, M, st, part, (g.nx + 63) / 64, cfl_boxes(g), cfl_box_blocks(g), dm);
 }
 
 void launch_dt(const DevParams& dp, DevState* st, const double* tf, const double* itime,

@@ -162,9 +162,9 @@ __device__ __forceinline__ double wave_max(double v) {
 // Flux1..6Kernel + CFL1/2Kernel + reduceKernel (L/K:1426-1848, 1938-2004) for the dt of
 // the predictor: the max over faces [1, N-3]^2 of em/dx, em/dy (solid and fluid).  Faces
 // outside that square have a zero face-averaged metric in the reference, so their em is
-// the 1e-10 floor, which never exceeds these.  Result: atomicMax on the IEEE bits of a
-// positive double.  k_cfl covers the faces of the interior box [3, ny+2] x [3, nx+2]
-// (InterfacesKernel states); k_cfl_ring the rest of [1, N-3]^2, where Interfaces2Kernel
+// the 1e-10 floor, which never exceeds these.  Each block's maximum goes to its own slot,
+// which the dt kernel reduces.  k_cfl covers the faces of the interior box [3, ny+2] x [3, nx+2]
+// (InterfacesKernel states); its ring blocks the rest of [1, N-3]^2, where Interfaces2Kernel
 // overrides the states.  Speeds are formed in T; the quotient by dx and the max in FP64.
 template <typename T>
 __device__ __forceinline__ double face_cfl(const T* __restrict__ U, const T* __restrict__ M,
@@ -297,6 +297,19 @@ __device__ __forceinline__ bool dry_region(const T* __restrict__ U, const Grid& 
   return !__syncthreads_or(wet);
 }
 
+// The same test by one wave on its own (every lane of the wave calls it; lane in [0, 64)).
+template <typename T>
+__device__ __forceinline__ bool dry_region_wave(const T* __restrict__ U, const Grid& g,
+                                                const unsigned char* __restrict__ z, int nseg,
+                                                int ra, int rb, int ca, int cb, int lane,
+                                                int max_items) {
+  const DryRegion d = dry_region_of(g, ra, rb, ca, cb);
+  if (d.n > max_items) return false;
+  int wet = 0;
+  for (int e = lane; e < d.n; e += 64) wet |= dry_item_wet(d, U, g, z, nseg, e);
+  return __ballot(wet) == 0ull;
+}
+
 // CFL boxes: 64x4 blocks of interior faces (both directions), box (bx, by) starting at
 // column MD + 64 bx, row MD + 4 by.  A box's faces read columns c-1..c+2 (x-faces) and rows
 // r-1..r+2 (y-faces): the stencil [rlo-1, rlo+5] x [clo-1, clo+65].  A box whose stencil
@@ -351,8 +364,9 @@ __device__ __forceinline__ bool ring_face_pos(const Grid& g, int t, int& r, int&
   return true;
 }
 
-// The CFL value of ring thread t (0 past the ring).  Called by every lane of a one-wave
-// block.  With dry-tile skipping the wave first tests the cells its faces read: a ring face
+// The CFL value of ring thread t (0 past the ring).  Called by every lane of a wave, whose
+// threads are consecutive.  With dry-tile skipping the wave first tests the cells its faces
+// read: a ring face
 // on a ghost row or column takes its states from rows 3,4 (ny+2,ny+1) and columns 3,4
 // (nx+2,nx+1) instead (Interfaces2Kernel), and base_states reads a row r' or column c'
 // and the cells from r'-1 to r'+2 (c'-1 to c'+2); so the wave's faces read within the
@@ -381,8 +395,8 @@ __device__ __forceinline__ double cfl_ring_face(const T* __restrict__ U, const T
       ca = min(ca, __shfl_xor(ca, off, 64));
       cb = max(cb, __shfl_xor(cb, off, 64));
     }
-    if (rb >= ra && dry_region(U, g, dm.z0, dm.nseg, ra - 1, rb + 2, ca - 1, cb + 2,
-                               (int)(threadIdx.x & 63), 64, 4 * 64))
+    if (rb >= ra && dry_region_wave(U, g, dm.z0, dm.nseg, ra - 1, rb + 2, ca - 1, cb + 2,
+                                    (int)(threadIdx.x & 63), 4 * 64))
       return valid ? (double)T(1.0e-10) / (dir == 0 ? g.dx : g.dy) : 0.0;
   }
   return valid ? face_cfl(U, M, g, ph, dir, r, c, true) : 0.0;
