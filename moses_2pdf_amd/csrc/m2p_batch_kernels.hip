@@ -39,19 +39,32 @@ __global__ void __launch_bounds__(GHOST_LINES) kb_ghost(BatchDev B, int post) {
 // Blocks stride over the batch's CFL boxes, 8 at a time: one round of loads tests all 8
 // (32 threads per box; members without dry-tile skipping are always computed), then each
 // box's maximum goes to its own partial slot (a dry box's is the wave-speed floor).
-// Halted members' slots are left alone: their dt kernel does not read them.
+// Halted members' slots are left alone: their dt kernel does not read them.  The blocks
+// after the first nbb take the ring faces in the same launch: each of their 4 waves is one
+// ring block of the block map (one member's 32 ring faces), with its own slot.
 constexpr int BATCH_CFL_BLOCKS = 4096;
 
 template <typename T>
-__global__ void __launch_bounds__(256) kb_cfl(BatchDev B) {
+__global__ void __launch_bounds__(256) kb_cfl(BatchDev B, int nbb) {
   __shared__ int state[8];   // 0: dry, 1: compute, 2: halted
   const int tid = threadIdx.y * 64 + threadIdx.x;
-  for (int t0 = blockIdx.x; t0 < B.nBox; t0 += 8 * gridDim.x) {
+  if ((int)blockIdx.x >= nbb) {   // ring faces, one ring block per wave
+    const int rb = ((int)blockIdx.x - nbb) * 4 + threadIdx.y;
+    if (rb >= B.nRing) return;
+    const Member& m = B.mem[B.mapRing[rb]];
+    if (halted_pre(m.st)) return;
+    const int t = (rb - m.ring0) * 64 + threadIdx.x;
+    double v = cfl_ring_face((const T*)m.U0, (const T*)m.MT, m.g, m.ph, t, m.dry);
+    v = wave_max(v);
+    if (threadIdx.x == 0) B.part[B.nBox + rb] = v;
+    return;
+  }
+  for (int t0 = blockIdx.x; t0 < B.nBox; t0 += 8 * nbb) {
     __syncthreads();   // every thread is done with the previous group's state
     if (tid < 8) state[tid] = 0;
     __syncthreads();
     {
-      const int k = tid >> 5, t = t0 + k * gridDim.x;
+      const int k = tid >> 5, t = t0 + k * nbb;
       if (t < B.nBox) {
         const Member& m = B.mem[B.mapBox[t]];
         if (halted_pre(m.st)) {
@@ -70,7 +83,7 @@ __global__ void __launch_bounds__(256) kb_cfl(BatchDev B) {
     }
     __syncthreads();
     for (int k = 0; k < 8; k++) {
-      const int t = t0 + k * gridDim.x;
+      const int t = t0 + k * nbb;
       if (t >= B.nBox) break;
       const int s = state[k];
       if (s == 2) continue;
@@ -84,16 +97,6 @@ __global__ void __launch_bounds__(256) kb_cfl(BatchDev B) {
                       B.part + t);
     }
   }
-}
-
-template <typename T>
-__global__ void __launch_bounds__(64) kb_ring(BatchDev B) {
-  const Member& m = B.mem[B.mapRing[blockIdx.x]];
-  if (halted_pre(m.st)) return;
-  const int t = ((int)blockIdx.x - m.ring0) * 64 + threadIdx.x;
-  double v = cfl_ring_face((const T*)m.U0, (const T*)m.MT, m.g, m.ph, t, m.dry);
-  v = wave_max(v);
-  if (threadIdx.x == 0) B.part[B.nBox + blockIdx.x] = v;
 }
 
 // One block per member: its CFL and corrector partials, then the dt control.
@@ -129,8 +132,10 @@ hipError_t launch_batch_steps(const BatchDev& B, int ty, int w, bool any_inflow,
   for (int n = 0; n < nsteps; n++) {
     if (any_inflow) kb_inflow<T><<<B.nmem, 64, 0, s>>>(B);
     kb_ghost<T><<<dim3(B.nGhost, MN), GHOST_LINES, 0, s>>>(B, 0);
-    kb_cfl<T><<<B.nBox < BATCH_CFL_BLOCKS ? B.nBox : BATCH_CFL_BLOCKS, dim3(64, 4), 0, s>>>(B);
-    kb_ring<T><<<B.nRing, 64, 0, s>>>(B);
+    {
+      const int nbb = B.nBox < BATCH_CFL_BLOCKS ? B.nBox : BATCH_CFL_BLOCKS;
+      kb_cfl<T><<<nbb + (B.nRing + 3) / 4, dim3(64, 4), 0, s>>>(B, nbb);
+    }
     kb_dt<T><<<B.nmem, 256, 0, s>>>(B);
     if (!launch_batch_stage<T>(B, ty, w, 0, s)) return hipErrorInvalidValue;
     kb_ghost<T><<<dim3(B.nGhost, MN), GHOST_LINES, 0, s>>>(B, 1);
