@@ -133,6 +133,13 @@ def test_run_writes_result2pb_byte_exact(tmp_path, name, sc, mode):
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert "Time used on GPU:" in r.stdout
     check_dir(os.path.join(root, "result2Pb"), files)
+    # every tile computed, as the reference does (overlay dry_skip=0): the same files
+    for f in os.listdir(os.path.join(root, "result2Pb")):
+        os.remove(os.path.join(root, "result2Pb", f))
+    r = subprocess.run([exe, "dry_skip=0", "./result2Pb/"], cwd=root, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    check_dir(os.path.join(root, "result2Pb"), files)
 
 
 def test_missing_par_list_is_an_error_not_an_exit(tmp_path):
