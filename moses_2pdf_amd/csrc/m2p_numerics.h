@@ -282,13 +282,14 @@ struct FaceFlux {
   T D4, D5;
 };
 
-// Cell values the viscous term needs on both sides of the face (Flux8Kernel
-// L/K:2182-2205): fluid momenta and depth, and the tangential limited slopes of the
-// fluid momenta.
+// Cell quotients the viscous term needs on both sides of the face (Flux8Kernel
+// L/K:2182-2205), each over the fluid depth u3 + H0: the fluid momenta u4, u5 and their
+// limited slopes along the face (uy for x-faces, ux for y-faces).  They depend on the cell
+// only, so a cell shared by two faces can hand the same values to both.
 template <typename T>
 struct PDSide {
-  T u3, u4, u5;   // cell state
-  T t4, t5;       // slope of u4, u5 along the face (uy for x-faces, ux for y-faces)
+  T q4, q5;   // u4 / (u3 + H0), u5 / (u3 + H0)
+  T s4, s5;   // slope(u4) / (u3 + H0), slope(u5) / (u3 + H0)
 };
 
 template <typename T>
@@ -299,7 +300,6 @@ __device__ __forceinline__ void face_flux(int dir, const T L[6], const T R[6], c
   const T M = (dir == 0) ? f.j11 * f.j11 + f.j12 * f.j12 + f.j13 * f.j13
                          : f.j21 * f.j21 + f.j22 * f.j22 + f.j23 * f.j23;
   const T Drtio1 = T(ph.Drtio1);
-  const T h0 = T(H0);
   // Flux6Kernel L/K:1809-1843
   const T epsilon0 = T(1.0);
   const T kk = T(0.5) * epsilon0 * f.cav * f.det * M;
@@ -341,15 +341,15 @@ __device__ __forceinline__ void face_flux(int dir, const T L[6], const T R[6], c
   const T dx = T(g.dx), dy = T(g.dy);
   T dudx, dvdx, dudy, dvdy;
   if (dir == 0) {
-    dudx = qdiv(qdiv(pr.u4, pr.u3 + h0) - qdiv(pl.u4, pl.u3 + h0), dx);
-    dvdx = qdiv(qdiv(pr.u5, pr.u3 + h0) - qdiv(pl.u5, pl.u3 + h0), dx);
-    dudy = qdiv(T(0.5) * (qdiv(pr.t4, pr.u3 + h0) + qdiv(pl.t4, pl.u3 + h0)), dy);
-    dvdy = qdiv(T(0.5) * (qdiv(pr.t5, pr.u3 + h0) + qdiv(pl.t5, pl.u3 + h0)), dy);
+    dudx = qdiv(pr.q4 - pl.q4, dx);
+    dvdx = qdiv(pr.q5 - pl.q5, dx);
+    dudy = qdiv(T(0.5) * (pr.s4 + pl.s4), dy);
+    dvdy = qdiv(T(0.5) * (pr.s5 + pl.s5), dy);
   } else {
-    dudx = qdiv(T(0.5) * (qdiv(pr.t4, pr.u3 + h0) + qdiv(pl.t4, pl.u3 + h0)), dx);
-    dvdx = qdiv(T(0.5) * (qdiv(pr.t5, pr.u3 + h0) + qdiv(pl.t5, pl.u3 + h0)), dx);
-    dudy = qdiv(qdiv(pr.u4, pr.u3 + h0) - qdiv(pl.u4, pl.u3 + h0), dy);
-    dvdy = qdiv(qdiv(pr.u5, pr.u3 + h0) - qdiv(pl.u5, pl.u3 + h0), dy);
+    dudx = qdiv(T(0.5) * (pr.s4 + pl.s4), dx);
+    dvdx = qdiv(T(0.5) * (pr.s5 + pl.s5), dx);
+    dudy = qdiv(pr.q4 - pl.q4, dy);
+    dvdy = qdiv(pr.q5 - pl.q5, dy);
   }
   // Flux9Kernel L/K:2242-2273
   const T duxidxi = f.j11 * dudx + f.j12 * dvdx - qdiv(f.j13 * (f.j31 * dudx + f.j32 * dvdx), f.j33);

@@ -454,18 +454,21 @@ __device__ __forceinline__ void dt_body(const DevParams& dp, DevState* __restric
 }
 
 
-// Slope-carrying cell values for the viscous term: u3, u4, u5 and the limited slopes of
-// u4, u5 along direction `tdir` (0: x, 1: y) at cell i.
+// The viscous term's quotients of cell i (PDSide): u4, u5 and their limited slopes along
+// direction `tdir` (0: x, 1: y), each over u3 + H0, in the reference's operations.
 template <typename T>
 __device__ __forceinline__ PDSide<T> pd_side(const T* __restrict__ U, const Grid& g, long long i,
                                              int tdir) {
   const long long P = g.P, st = tdir == 0 ? 1 : g.nxd;
+  const T u3 = U[3 * P + i], u4 = U[4 * P + i], u5 = U[5 * P + i];
+  const T t4 = tvd_slope(U[4 * P + i - st], u4, U[4 * P + i + st]);
+  const T t5 = tvd_slope(U[5 * P + i - st], u5, U[5 * P + i + st]);
+  const T d = u3 + T(H0);
   PDSide<T> p;
-  p.u3 = U[3 * P + i];
-  p.u4 = U[4 * P + i];
-  p.u5 = U[5 * P + i];
-  p.t4 = tvd_slope(U[4 * P + i - st], p.u4, U[4 * P + i + st]);
-  p.t5 = tvd_slope(U[5 * P + i - st], p.u5, U[5 * P + i + st]);
+  p.q4 = qdiv(u4, d);
+  p.q5 = qdiv(u5, d);
+  p.s4 = qdiv(t4, d);
+  p.s5 = qdiv(t5, d);
   return p;
 }
 
@@ -518,19 +521,16 @@ __device__ __forceinline__ void load_states(T (*Tl)[TY + 1][64], int w, int lane
 }
 
 // Phase C: fluxes of one face from its final states, into the tile.
+// pl, pr: the viscous quotients of the face's left/lower and right/upper cells.
 template <typename T, int TY>
 __device__ __forceinline__ void flux_store(T (*Tl)[TY + 1][64], int w, int lane,
-                                           const T* __restrict__ U, const T* __restrict__ M,
-                                           const Grid& g, const Phys& ph, int dir, int r, int c,
-                                           T L[6], T R[6]) {
+                                           const T* __restrict__ M, const Grid& g,
+                                           const Phys& ph, int dir, int r, int c, T L[6], T R[6],
+                                           const PDSide<T>& pl, const PDSide<T>& pr) {
   FaceMet<T> f;
   face_metric<T, std::is_same<T, double>::value>(M, g, dir, r, c, f);
   FaceSpeed<T> sp;
   face_speed(dir, L, R, f, ph, sp);
-  const long long i = idx(g, r, c);
-  const long long j = i + (dir == 0 ? 1 : g.nxd);
-  const PDSide<T> pl = pd_side(U, g, i, 1 - dir);
-  const PDSide<T> pr = pd_side(U, g, j, 1 - dir);
   FaceFlux<T> o;
   face_flux(dir, L, R, f, sp, pl, pr, g, ph, o);
 #pragma unroll
@@ -577,7 +577,11 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
     } else if (has) {
       base_states(Uin, g, 0, r, c, L, R);
     }
-    if (has) flux_store<T, TY>(F, w, lane, Uin, M, g, ph, 0, r, c, L, R);
+    if (has) {
+      const long long i = idx(g, r, c);
+      flux_store<T, TY>(F, w, lane, M, g, ph, 0, r, c, L, R, pd_side(Uin, g, i, 1),
+                        pd_side(Uin, g, i + 1, 1));
+    }
   }
   __syncthreads();
 
@@ -600,6 +604,20 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
 #pragma unroll
     for (int q = 0; q < 2; q++) tD[q] = (dtdx) * (F[10 + q][w][lane + 1] - F[10 + q][w][lane]) * (fD);
   }
+  // The viscous quotients of cell rows r0-1 .. r0+TY (x-slopes), for the y-faces: face row
+  // r0-1+w takes row w as its lower cells and row w+1 as its upper ones, so each cell's
+  // quotients are formed once (wave 0 also forms the extra top row).
+  __shared__ PDSide<T> Q[TY + 2][64];
+  {
+    const int cq = c0 + lane;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const int row = w + k * (TY + 1);   // k = 1: the extra row, wave 0 only
+      if (row > TY + 1) break;
+      const int rq = r0 - 1 + row;
+      if (lane < ST_TX && rq <= ymax + 1 && cq <= xmax) Q[row][lane] = pd_side(Uin, g, idx(g, rq, cq), 0);
+    }
+  }
   __syncthreads();
 
   // y faces (r0-1+w, c0+lane), faces (r, c)-(r+1, c); the overridden face row 2 lies in
@@ -616,7 +634,7 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
     } else if (has) {
       base_states(Uin, g, 1, rf, cf, L, R);
     }
-    if (has) flux_store<T, TY>(F, w, lane, Uin, M, g, ph, 1, rf, cf, L, R);
+    if (has) flux_store<T, TY>(F, w, lane, M, g, ph, 1, rf, cf, L, R, Q[w][lane], Q[w + 1][lane]);
   }
   __syncthreads();
   if (!active) return 0.0;
