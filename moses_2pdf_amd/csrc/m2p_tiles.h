@@ -562,6 +562,20 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
   const int xmax = g.nx + MD - 1, ymax = g.ny + MD - 1;
   const long long P = g.P;
 
+  // The viscous quotients (y-slopes) of the cells the x-faces touch, formed once per cell
+  // and shared through LDS: wave w < TY forms row r0+w, columns c0-1..c0+62 (slot lane), and
+  // the last wave, idle in the x phase, forms column c0+63 of every row (slot 64); face
+  // c0-1+lane takes slots lane and lane+1.  The y phase reuses the array.
+  __shared__ PDSide<T> Q[(TY + 2) * 64];
+  if (w < TY) {
+    const int r = r0 + w, c = c0 - 1 + lane;
+    if (r <= ymax && c <= xmax + 1) Q[w * 65 + lane] = pd_side(Uin, g, idx(g, r, c), 1);
+  } else if (lane < TY) {
+    const int r = r0 + lane, c = c0 + 63;
+    if (r <= ymax && c <= xmax + 1) Q[lane * 65 + 64] = pd_side(Uin, g, idx(g, r, c), 1);
+  }
+  __syncthreads();
+
   // x faces (r0+w, c0-1+lane), faces (r, c)-(r, c+1).  Only tiles in the first tile column
   // hold the overridden face at column 2; the others form their face states in registers
   // and skip the LDS round trip (the branch is uniform over the block).
@@ -577,11 +591,7 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
     } else if (has) {
       base_states(Uin, g, 0, r, c, L, R);
     }
-    if (has) {
-      const long long i = idx(g, r, c);
-      flux_store<T, TY>(F, w, lane, M, g, ph, 0, r, c, L, R, pd_side(Uin, g, i, 1),
-                        pd_side(Uin, g, i + 1, 1));
-    }
+    if (has) flux_store<T, TY>(F, w, lane, M, g, ph, 0, r, c, L, R, Q[w * 65 + lane], Q[w * 65 + lane + 1]);
   }
   __syncthreads();
 
@@ -606,8 +616,8 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
   }
   // The viscous quotients of cell rows r0-1 .. r0+TY (x-slopes), for the y-faces: face row
   // r0-1+w takes row w as its lower cells and row w+1 as its upper ones, so each cell's
-  // quotients are formed once (wave 0 also forms the extra top row).
-  __shared__ PDSide<T> Q[TY + 2][64];
+  // quotients are formed once (wave 0 also forms the extra top row).  The x phase's reads
+  // of Q all precede the barrier after the x-face fluxes, so these stores may reuse it.
   {
     const int cq = c0 + lane;
 #pragma unroll
@@ -615,7 +625,7 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
       const int row = w + k * (TY + 1);   // k = 1: the extra row, wave 0 only
       if (row > TY + 1) break;
       const int rq = r0 - 1 + row;
-      if (lane < ST_TX && rq <= ymax + 1 && cq <= xmax) Q[row][lane] = pd_side(Uin, g, idx(g, rq, cq), 0);
+      if (lane < ST_TX && rq <= ymax + 1 && cq <= xmax) Q[row * 64 + lane] = pd_side(Uin, g, idx(g, rq, cq), 0);
     }
   }
   __syncthreads();
@@ -634,7 +644,7 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
     } else if (has) {
       base_states(Uin, g, 1, rf, cf, L, R);
     }
-    if (has) flux_store<T, TY>(F, w, lane, M, g, ph, 1, rf, cf, L, R, Q[w][lane], Q[w + 1][lane]);
+    if (has) flux_store<T, TY>(F, w, lane, M, g, ph, 1, rf, cf, L, R, Q[w * 64 + lane], Q[(w + 1) * 64 + lane]);
   }
   __syncthreads();
   if (!active) return 0.0;
