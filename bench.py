@@ -200,6 +200,8 @@ def main() -> int:
                     help="ensemble and serve: compute every tile (dry-tile skipping off) on both sides")
     ap.add_argument("--no-dry-skip", action="store_true",
                     help="leave out the second run of the landslide workload with dry-tile skipping")
+    ap.add_argument("--no-shard", action="store_true",
+                    help="leave out the run of one GPU's strip of the grid cut into 8")
     ap.add_argument("--copy-calib", action="store_true",
                     help="after the timed steps, copy the 7 state planes once device to device "
                          "(a plain copy kernel of known size, for HBM counter calibration)")
@@ -247,6 +249,35 @@ def dry_skip_run(sc, args, u_dense, dense_value) -> dict:
             "note": "same workload and steps with m2p_params.dry_skip = 1: tiles whose inputs "
                     "are all zero are left out (exact); not the headline, which computes every "
                     "tile as the reference does"}
+
+
+def shard_run(sc, args, dense_value, parts: int = 8) -> dict:
+    """One GPU's share of the same grid cut into ``parts`` row strips (nx x ny/parts plus the
+    3-cell ghost ring, which stands in for the halo a neighbour would send), every tile
+    computed.  The strip is the one holding the most released material.  The halo exchange
+    and the dt all-reduce of a multi-GPU run are not in this figure."""
+    import dataclasses
+    import numpy as np
+    from moses_2pdf_amd import Simulation
+    h = sc.ny // parts
+    rows = sc.depth.sum(axis=1)
+    r0 = int(np.argmax(np.convolve(rows, np.ones(h), mode="valid")))
+    sub = dataclasses.replace(sc, topo=sc.topo[r0:r0 + h].copy(), depth=sc.depth[r0:r0 + h].copy(),
+                              phi=sc.phi[r0:r0 + h].copy())
+    with Simulation(sub, precision=args.precision, dry_skip=False) as sim:
+        sim.step(args.warmup)
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.step(args.steps)
+        sim.sync()
+        el = time.perf_counter() - t0
+    v = sub.nx * sub.ny * args.steps / el
+    print(f"# shard run {sub.nx}x{sub.ny}: {args.steps} steps in {el:.3f} s", file=sys.stderr,
+          flush=True)
+    return {"grid": [sub.nx, sub.ny], "rows": [r0, r0 + h], "value": v, "unit": "cell-updates/s",
+            "ms_per_step": el / args.steps * 1e3, "per_cell_rate_vs_full_grid": v / dense_value,
+            "note": f"one GPU's share of the {sc.nx}x{sc.ny} grid cut into {parts} row strips, every "
+                    "tile computed; halo exchange and dt all-reduce not included"}
 
 
 def landslide(args) -> int:
@@ -324,6 +355,8 @@ def landslide(args) -> int:
     }
     if not args.no_dry_skip:
         out["dry_skip"] = dry_skip_run(sc, args, u, value)
+    if not args.no_shard:
+        out["shard"] = shard_run(sc, args, value)
     if not args.no_cpu_baseline:
         print("# cpu baseline (oracle) ...", file=sys.stderr, flush=True)
         del sim

@@ -206,10 +206,12 @@ int m2p_batch_step(m2p_batch* b, int nsteps);
 int m2p_batch_sync(m2p_batch* b);
 /* Autotuner: times every compiled stage-kernel variant (tile height x waves per SIMD)
  * under each block schedule (XCD banding on/off, member order as added / largest first)
- * for trial_steps steps on the batch itself, restores every member's state, and keeps
+ * for trial_steps steps on the batch itself, then, under the fastest, the CFL launch's
+ * block count (2048 .. 32768); restores every member's state after each trial and keeps
  * the fastest.  A text table of the timings goes to report (nullable). */
 int m2p_batch_tune(m2p_batch* b, int trial_steps, char* report, int report_len);
-/* Manual schedule: keys "ty", "waves", "remap", "order", "graph_steps" (0: no graph).
+/* Manual schedule: keys "ty", "waves", "remap", "order", "graph_steps" (0: no graph),
+ * "cfl_blocks" (>= 64; most blocks of the CFL launch, default 4096).
  * The (ty, waves) pair must be a compiled variant; m2p_batch_step reports M2P_EINVAL
  * otherwise. */
 int m2p_batch_set(m2p_batch* b, const char* key, int value);

@@ -45,6 +45,7 @@ struct BatchDev {
                   // corrector tiles [nBox + nRing, +nStage)
   int nmem, nGhost, nBox, nRing, nStage;
   int remap;   // stage blocks dealt to the 8 XCDs in contiguous bands
+  int cfl_blocks;   // most box blocks of the CFL launch; each strides over the boxes
   int vmax;    // some member has a stop speed: reduce corrector speeds after each step
   // dry-tile skipping (some member has it on): the stage plans (8 bands whatever remap
   // says) and their PLAN_COUNTERS counters, zeroed by block 0 of the dt kernel
@@ -60,6 +61,7 @@ struct StageVariant {
 int stage_variants(int prec, const StageVariant** out);
 
 constexpr int GHOST_LINES = 128;   // ghost-fill lines per block
+constexpr int BATCH_CFL_BLOCKS = 4096;   // default of BatchDev::cfl_blocks
 int ring_blocks(const Grid& g);
 int box_blocks_x(const Grid& g);
 int box_blocks_y(const Grid& g);

@@ -43,6 +43,7 @@ struct m2p_batch {
   std::vector<m2p_ctx*> mem;   // in order of addition
   // schedule
   int ty = 7, w = 4, remap = 1, order = 1, graph_steps = 16;
+  int cfl_blocks = m2p::BATCH_CFL_BLOCKS;
   bool dirty = true;
   // device tables
   m2p::Member* dMem = nullptr;
@@ -203,6 +204,7 @@ int rebuild(m2p_batch* b) {
   B.nRing = (int)mr.size();
   B.nStage = (int)ms.size();
   B.remap = b->remap;
+  B.cfl_blocks = b->cfl_blocks;
   B.vmax = any_vstop ? 1 : 0;
   B.plan_on = any_dry ? 1 : 0;
   B.plan_cnt = b->dPlan;
@@ -357,6 +359,8 @@ int m2p_batch_set(m2p_batch* b, const char* key, int value) {
     b->order = value;
   } else if (k == "graph_steps" && value >= 0) {
     b->graph_steps = value;
+  } else if (k == "cfl_blocks" && value >= 64) {
+    b->cfl_blocks = value;
   } else {
     return bfail(b, M2P_EINVAL, "unknown key or bad value: " + k);
   }
@@ -372,6 +376,7 @@ int m2p_batch_get(const m2p_batch* b, const char* key) {
   if (k == "remap") return b->remap;
   if (k == "order") return b->order;
   if (k == "graph_steps") return b->graph_steps;
+  if (k == "cfl_blocks") return b->cfl_blocks;
   if (k == "stage_blocks") return b->B.nStage;
   return M2P_EINVAL;
 }
@@ -383,6 +388,7 @@ int m2p_batch_tune(m2p_batch* b, int trial_steps, char* report, int report_len) 
   Snapshot sn;
   int rc = save(b, sn, false);
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  b->cfl_blocks = m2p::BATCH_CFL_BLOCKS;   // the stage schedules are timed at the default
   struct Cand {
     int ty, w, remap, order;
     float ms;
@@ -425,9 +431,40 @@ int m2p_batch_tune(m2p_batch* b, int trial_steps, char* report, int report_len) 
     b->remap = best->remap;
     b->order = best->order;
     b->dirty = true;
+  }
+  // then the CFL launch's block count, under the chosen stage schedule
+  float best_ms = 0.f;
+  for (auto& cd : cands)
+    if (best_ms == 0.f || cd.ms < best_ms) best_ms = cd.ms;
+  int best_cb = b->cfl_blocks;
+  for (int cb : {2048, 8192, 16384, 32768}) {
+    if (rc) break;
+    b->cfl_blocks = cb;
+    b->dirty = true;
+    float ms = 0.f;
+    if ((rc = step_batch(b, 1, false))) break;
+    if (hipEventRecord(e0, b->s) != hipSuccess) rc = bfail(b, M2P_EHIP, "hipEventRecord");
+    if (!rc) rc = step_batch(b, trial_steps, false);
+    if (!rc && hipEventRecord(e1, b->s) != hipSuccess) rc = bfail(b, M2P_EHIP, "hipEventRecord");
+    if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = bfail(b, M2P_EHIP, "hipEventSynchronize");
+    if (!rc) (void)hipEventElapsedTime(&ms, e0, e1);
+    ms /= trial_steps;
+    if (!rc) rc = save(b, sn, true);
     char line[160];
-    std::snprintf(line, sizeof line, "chosen: ty=%d waves=%d remap=%d order=%d (%.4f ms/step)\n",
-                  best->ty, best->w, best->remap, best->order, best->ms);
+    std::snprintf(line, sizeof line, "cfl_blocks=%d: %.4f ms/step\n", cb, ms);
+    rep += line;
+    if (!rc && ms < best_ms) {
+      best_ms = ms;
+      best_cb = cb;
+    }
+  }
+  b->cfl_blocks = best_cb;
+  b->dirty = true;
+  if (!rc) {
+    char line[160];
+    std::snprintf(line, sizeof line,
+                  "chosen: ty=%d waves=%d remap=%d order=%d cfl_blocks=%d (%.4f ms/step)\n", b->ty, b->w,
+                  b->remap, b->order, b->cfl_blocks, best_ms);
     rep += line;
   }
   for (void* p : sn.buf) (void)hipFree(p);

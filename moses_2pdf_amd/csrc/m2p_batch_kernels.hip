@@ -41,8 +41,9 @@ __global__ void __launch_bounds__(GHOST_LINES) kb_ghost(BatchDev B, int post) {
 // box's maximum goes to its own partial slot (a dry box's is the wave-speed floor).
 // Halted members' slots are left alone: their dt kernel does not read them.  The blocks
 // after the first nbb take the ring faces in the same launch: each of their 4 waves is one
-// ring block of the block map (one member's 32 ring faces), with its own slot.
-constexpr int BATCH_CFL_BLOCKS = 4096;
+// ring block of the block map (one member's 32 ring faces), with its own slot.  nbb is
+// B.cfl_blocks or fewer (a tuned parameter): with fewer blocks each one does more of the
+// dry tests and skips, with more the wet boxes spread over more blocks.
 
 template <typename T>
 __global__ void __launch_bounds__(256) kb_cfl(BatchDev B, int nbb) {
@@ -133,7 +134,7 @@ hipError_t launch_batch_steps(const BatchDev& B, int ty, int w, bool any_inflow,
     if (any_inflow) kb_inflow<T><<<B.nmem, 64, 0, s>>>(B);
     kb_ghost<T><<<dim3(B.nGhost, MN), GHOST_LINES, 0, s>>>(B, 0);
     {
-      const int nbb = B.nBox < BATCH_CFL_BLOCKS ? B.nBox : BATCH_CFL_BLOCKS;
+      const int nbb = B.nBox < B.cfl_blocks ? B.nBox : B.cfl_blocks;
       kb_cfl<T><<<nbb + (B.nRing + 3) / 4, dim3(64, 4), 0, s>>>(B, nbb);
     }
     kb_dt<T><<<B.nmem, 256, 0, s>>>(B);

@@ -95,6 +95,21 @@ def test_every_variant_and_schedule_gives_the_same_result(precision):
     b0.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_cfl_block_count_gives_the_same_result(precision):
+    # each box keeps its own partial slot, so the number of blocks striding over the boxes
+    # (from 64, many boxes per block, to more blocks than boxes) cannot change a result
+    scs = population()
+    ref = alone(scs, 15, precision)
+    for cb in (64, 1000, 32768):
+        def setup(b, cb=cb):
+            b.set("cfl_blocks", cb)
+            assert b.get("cfl_blocks") == cb
+        got = batched(scs, 15, precision, chunks=[3, 12], setup=setup)
+        assert same(got, ref), cb
+    say(precision, "cfl_blocks 64 / 1000 / 32768 equal to alone")
+
+
 def test_tune_restores_state_and_keeps_results():
     scs = population()
     ref = alone(scs, 10, "fp32")
