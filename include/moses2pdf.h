@@ -207,8 +207,9 @@ int m2p_batch_sync(m2p_batch* b);
 /* Autotuner: times every compiled stage-kernel variant (tile height x waves per SIMD)
  * under each block schedule (XCD banding on/off, member order as added / largest first)
  * for trial_steps steps on the batch itself, then, under the fastest, the CFL launch's
- * block count (2048 .. 32768); restores every member's state after each trial and keeps
- * the fastest.  A text table of the timings goes to report (nullable). */
+ * block count (2048 .. 32768), and keeps the fastest.  The trials start from the state
+ * one step on, restored after each; every member's state, time and step count are put
+ * back at the end.  A text table of the timings goes to report (nullable). */
 int m2p_batch_tune(m2p_batch* b, int trial_steps, char* report, int report_len);
 /* Manual schedule: keys "ty", "waves", "remap", "order", "graph_steps" (0: no graph),
  * "cfl_blocks" (>= 64; most blocks of the CFL launch, default 4096).

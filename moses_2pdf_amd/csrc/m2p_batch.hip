@@ -44,7 +44,8 @@ struct m2p_batch {
   // schedule
   int ty = 7, w = 4, remap = 1, order = 1, graph_steps = 16;
   int cfl_blocks = m2p::BATCH_CFL_BLOCKS;
-  bool dirty = true;
+  bool dirty = true;   // members changed: the block maps must be rebuilt
+  int built_ty = 0, built_order = -1, built_w = 0;   // schedule the device tables are for
   // device tables
   m2p::Member* dMem = nullptr;
   size_t capMem = 0;
@@ -214,6 +215,27 @@ int rebuild(m2p_batch* b) {
     B.plan[io].per = per8;
   }
   b->dirty = false;
+  b->built_ty = b->ty;
+  b->built_order = b->order;
+  b->built_w = b->w;
+  return 0;
+}
+
+// Brings the device tables in line with the members and the schedule.  The block maps are
+// rebuilt when the members, the tile height or the member order changed; the occupancy
+// cap, XCD banding and CFL block count only change kernel arguments (and drop a captured
+// graph, which holds them).
+int refresh(m2p_batch* b) {
+  if (b->dirty || b->ty != b->built_ty || b->order != b->built_order) return rebuild(b);
+  if (b->w != b->built_w || b->remap != b->B.remap || b->cfl_blocks != b->B.cfl_blocks) {
+    if (!variant_ok(b->prec, b->ty, b->w))
+      return bfail(b, M2P_EINVAL, "no stage kernel compiled for ty=" + std::to_string(b->ty) +
+                                      " waves=" + std::to_string(b->w));
+    drop_graph(b);
+    b->B.remap = b->remap;
+    b->B.cfl_blocks = b->cfl_blocks;
+    b->built_w = b->w;
+  }
   return 0;
 }
 
@@ -242,8 +264,7 @@ int ensure_graph(m2p_batch* b) {
 }
 
 int step_batch(m2p_batch* b, int nsteps, bool use_graph) {
-  if (b->dirty)
-    if (int rc = rebuild(b)) return rc;
+  if (int rc = refresh(b)) return rc;
   if (b->mem.empty() || nsteps <= 0) return 0;
   int left = nsteps;
   if (use_graph && b->graph_steps > 0 && left >= b->graph_steps) {
@@ -254,42 +275,51 @@ int step_batch(m2p_batch* b, int nsteps, bool use_graph) {
   return 0;
 }
 
-// Saved member state for the autotuner's trial steps.
+// Saved member state for the autotuner's trial steps: every member's state buffers in one
+// allocation (each part 256-byte aligned), and the copy lists that save and restore them
+// (one launch each).
 struct Snapshot {
-  std::vector<void*> buf;
+  void* buf = nullptr;
+  m2p::CopyItem* items = nullptr;   // device: [n] save items, then [n] restore items
+  int n = 0;
+  ~Snapshot() {
+    if (buf) (void)hipFree(buf);
+    if (items) (void)hipFree(items);
+  }
 };
 
-size_t snap_bytes(const m2p_ctx* c) {
-  return (16 * (size_t)c->P) * c->es + (c->dLIM ? (size_t)c->P : 0) + sizeof(DevState) +
-         2 * (size_t)m2p::dry_flag_count(c->g);
-}
-
 int save(m2p_batch* b, Snapshot& sn, bool restore) {
-  const hipMemcpyKind k = hipMemcpyDeviceToDevice;
-  for (size_t i = 0; i < b->mem.size(); i++) {
-    m2p_ctx* c = b->mem[i];
-    if (!restore) {
-      void* p = nullptr;
-      BCHK(b, hipMalloc(&p, snap_bytes(c)));
-      sn.buf.push_back(p);
-    }
-    char* p = (char*)sn.buf[i];
-    const size_t P = (size_t)c->P, es = c->es;
+  if (!restore) {
     struct Part {
       void* dev;
-      size_t n;
-    } parts[] = {{c->dU0, 7 * P * es}, {c->dU1, 7 * P * es}, {c->dONE, 2 * P * es},
-                 {c->dLIM, c->dLIM ? P : 0}, {c->dSt, sizeof(DevState)},
-                 {c->dZ, 2 * (size_t)m2p::dry_flag_count(c->g)}};
-    for (auto& q : parts) {
-      if (q.n == 0) continue;
-      if (restore)
-        BCHK(b, hipMemcpyAsync(q.dev, p, q.n, k, b->s));
-      else
-        BCHK(b, hipMemcpyAsync(p, q.dev, q.n, k, b->s));
-      p += q.n;
+      size_t n, off;
+    };
+    std::vector<Part> parts;
+    size_t total = 0;
+    for (m2p_ctx* c : b->mem) {
+      const size_t P = (size_t)c->P, es = c->es;
+      const Part q[] = {{c->dU0, 7 * P * es, 0}, {c->dU1, 7 * P * es, 0}, {c->dONE, 2 * P * es, 0},
+                        {c->dLIM, c->dLIM ? P : 0, 0}, {c->dSt, sizeof(DevState), 0},
+                        {c->dZ, c->dZ ? 2 * (size_t)m2p::dry_flag_count(c->g) : 0, 0}};
+      for (const Part& x : q) {
+        if (x.n == 0 || !x.dev) continue;
+        parts.push_back({x.dev, x.n, total});
+        total += (x.n + 255) / 256 * 256;
+      }
     }
+    sn.n = (int)parts.size();
+    BCHK(b, hipMalloc(&sn.buf, total ? total : 256));
+    std::vector<m2p::CopyItem> it(2 * parts.size());
+    unsigned char* base = (unsigned char*)sn.buf;
+    for (size_t i = 0; i < parts.size(); i++) {
+      it[i] = {(const unsigned char*)parts[i].dev, base + parts[i].off, parts[i].n};
+      it[parts.size() + i] = {base + parts[i].off, (unsigned char*)parts[i].dev, parts[i].n};
+    }
+    BCHK(b, hipMalloc((void**)&sn.items, it.size() * sizeof(m2p::CopyItem) + 1));
+    BCHK(b, hipMemcpy(sn.items, it.data(), it.size() * sizeof(m2p::CopyItem), hipMemcpyHostToDevice));
   }
+  m2p::launch_copy_list(sn.items + (restore ? sn.n : 0), sn.n, b->s);
+  BCHK(b, hipGetLastError());
   BCHK(b, hipStreamSynchronize(b->s));
   return 0;
 }
@@ -364,8 +394,7 @@ int m2p_batch_set(m2p_batch* b, const char* key, int value) {
   } else {
     return bfail(b, M2P_EINVAL, "unknown key or bad value: " + k);
   }
-  b->dirty = true;
-  return 0;
+  return 0;   // the next step brings the device tables in line (refresh)
 }
 
 int m2p_batch_get(const m2p_batch* b, const char* key) {
@@ -385,43 +414,59 @@ int m2p_batch_tune(m2p_batch* b, int trial_steps, char* report, int report_len) 
   if (!b || !b->s) return M2P_ESTATE;
   if (trial_steps < 1) return bfail(b, M2P_EINVAL, "trial_steps must be >= 1");
   if (b->mem.empty()) return bfail(b, M2P_ESTATE, "empty batch");
+  Snapshot sn0;   // the caller's state, put back at the end
+  int rc = save(b, sn0, false);
+  // The trials start from the state one step on.  Before its first step a landslide batch
+  // has no dry segment (phi starts at phiS0 on every cell), so each trial's warm-up step
+  // would compute every tile; one step clears phi on the dry cells and sets exact flags.
   Snapshot sn;
-  int rc = save(b, sn, false);
+  if (!rc) rc = step_batch(b, 1, false);
+  if (!rc) rc = save(b, sn, false);
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  b->cfl_blocks = m2p::BATCH_CFL_BLOCKS;   // the stage schedules are timed at the default
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = bfail(b, M2P_EHIP, "hipEventCreate");
+  // one step to warm up, trial_steps timed, then every member's state restored
+  auto trial = [&](float* ms) {
+    *ms = 0.f;
+    if ((rc = step_batch(b, 1, false))) return;
+    if (hipEventRecord(e0, b->s) != hipSuccess) rc = bfail(b, M2P_EHIP, "hipEventRecord");
+    if (!rc) rc = step_batch(b, trial_steps, false);
+    if (!rc && hipEventRecord(e1, b->s) != hipSuccess) rc = bfail(b, M2P_EHIP, "hipEventRecord");
+    if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = bfail(b, M2P_EHIP, "hipEventSynchronize");
+    if (!rc) (void)hipEventElapsedTime(ms, e0, e1);
+    *ms /= trial_steps;
+    if (!rc) rc = save(b, sn, true);
+  };
   struct Cand {
     int ty, w, remap, order;
     float ms;
   };
+  // candidates in an order that changes the tile height or the member order (the two that
+  // need new block maps) as seldom as possible
   std::vector<Cand> cands;
   const m2p::StageVariant* v;
   const int nv = m2p::stage_variants(b->prec, &v);
-  for (int i = 0; i < nv; i++)
-    for (int remap = 1; remap >= 0; remap--)
-      for (int order = 1; order >= 0; order--) cands.push_back({v[i].ty, v[i].w, remap, order, 0.f});
-  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
-    rc = bfail(b, M2P_EHIP, "hipEventCreate");
+  std::vector<int> vi(nv);
+  std::iota(vi.begin(), vi.end(), 0);
+  std::stable_sort(vi.begin(), vi.end(), [&](int x, int y) { return v[x].ty < v[y].ty; });
+  for (int order = 1; order >= 0; order--)
+    for (int i : vi)
+      for (int remap = 1; remap >= 0; remap--) cands.push_back({v[i].ty, v[i].w, remap, order, 0.f});
+  b->cfl_blocks = m2p::BATCH_CFL_BLOCKS;   // the stage schedules are timed at the default
   std::string rep;
+  char line[160];
   for (auto& cd : cands) {
     if (rc) break;
     b->ty = cd.ty;
     b->w = cd.w;
     b->remap = cd.remap;
     b->order = cd.order;
-    b->dirty = true;
-    if ((rc = step_batch(b, 1, false))) break;  // warm-up
-    if (hipEventRecord(e0, b->s) != hipSuccess) rc = bfail(b, M2P_EHIP, "hipEventRecord");
-    if (!rc) rc = step_batch(b, trial_steps, false);
-    if (!rc && hipEventRecord(e1, b->s) != hipSuccess) rc = bfail(b, M2P_EHIP, "hipEventRecord");
-    if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = bfail(b, M2P_EHIP, "hipEventSynchronize");
-    if (!rc) (void)hipEventElapsedTime(&cd.ms, e0, e1);
-    cd.ms /= trial_steps;
-    if (!rc) rc = save(b, sn, true);
-    char line[160];
+    trial(&cd.ms);
     std::snprintf(line, sizeof line, "ty=%d waves=%d remap=%d order=%d: %.4f ms/step\n", cd.ty, cd.w,
                   cd.remap, cd.order, cd.ms);
     rep += line;
   }
+  float best_ms = 0.f;
   if (!rc) {
     const Cand* best = &cands[0];
     for (auto& cd : cands)
@@ -430,27 +475,15 @@ int m2p_batch_tune(m2p_batch* b, int trial_steps, char* report, int report_len) 
     b->w = best->w;
     b->remap = best->remap;
     b->order = best->order;
-    b->dirty = true;
+    best_ms = best->ms;
   }
   // then the CFL launch's block count, under the chosen stage schedule
-  float best_ms = 0.f;
-  for (auto& cd : cands)
-    if (best_ms == 0.f || cd.ms < best_ms) best_ms = cd.ms;
-  int best_cb = b->cfl_blocks;
+  int best_cb = m2p::BATCH_CFL_BLOCKS;
   for (int cb : {2048, 8192, 16384, 32768}) {
     if (rc) break;
     b->cfl_blocks = cb;
-    b->dirty = true;
-    float ms = 0.f;
-    if ((rc = step_batch(b, 1, false))) break;
-    if (hipEventRecord(e0, b->s) != hipSuccess) rc = bfail(b, M2P_EHIP, "hipEventRecord");
-    if (!rc) rc = step_batch(b, trial_steps, false);
-    if (!rc && hipEventRecord(e1, b->s) != hipSuccess) rc = bfail(b, M2P_EHIP, "hipEventRecord");
-    if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = bfail(b, M2P_EHIP, "hipEventSynchronize");
-    if (!rc) (void)hipEventElapsedTime(&ms, e0, e1);
-    ms /= trial_steps;
-    if (!rc) rc = save(b, sn, true);
-    char line[160];
+    float ms;
+    trial(&ms);
     std::snprintf(line, sizeof line, "cfl_blocks=%d: %.4f ms/step\n", cb, ms);
     rep += line;
     if (!rc && ms < best_ms) {
@@ -459,15 +492,16 @@ int m2p_batch_tune(m2p_batch* b, int trial_steps, char* report, int report_len) 
     }
   }
   b->cfl_blocks = best_cb;
-  b->dirty = true;
+  {
+    const int rc0 = save(b, sn0, true);
+    if (!rc) rc = rc0;
+  }
   if (!rc) {
-    char line[160];
     std::snprintf(line, sizeof line,
                   "chosen: ty=%d waves=%d remap=%d order=%d cfl_blocks=%d (%.4f ms/step)\n", b->ty, b->w,
                   b->remap, b->order, b->cfl_blocks, best_ms);
     rep += line;
   }
-  for (void* p : sn.buf) (void)hipFree(p);
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   b->report = rep;

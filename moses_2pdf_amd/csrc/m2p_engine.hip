@@ -115,6 +115,15 @@ void release(m2p_ctx* c) {
   c->s = nullptr;
 }
 
+// Both dry-flag maps (dm.nseg segments per row) from the state buffers, after they were
+// replaced (a new state, a restore, skipping switched on).
+void rescan_dry(m2p_ctx* c, const m2p::DryMap& dm, hipStream_t s) {
+  if (c->prec == M2P_FP64)
+    m2p::launch_dry_scan<double>(c->g, dm, (const double*)c->dU0, (const double*)c->dU1, s);
+  else
+    m2p::launch_dry_scan<float>(c->g, dm, (const float*)c->dU0, (const float*)c->dU1, s);
+}
+
 // Device allocation, uploads, metric setup (L/R:395-642) and the initial state.
 int setup(m2p_ctx* c) {
   HIPCHK(c, hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
@@ -205,6 +214,8 @@ int setup(m2p_ctx* c) {
   HIPCHK(c, hipGetLastError());
   // the first predictor's corner ghosts come from the initial state's own ghost rows
   HIPCHK(c, hipMemcpyAsync(c->dU1, c->dU0, 7 * P * c->es, hipMemcpyDeviceToDevice, c->s));
+  rescan_dry(c, c->dry, c->s);
+  HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->s));
   if (c->prec == M2P_FP32) (void)hipFree(dU64);
   // the grids are only needed by the setup
@@ -233,7 +244,6 @@ int download_state(m2p_ctx* c, void* dsrc, size_t planes, double* u) {
 
 int upload_state(m2p_ctx* c, void* ddst, const double* u) {
   const size_t n = 7 * (size_t)c->P;
-  m2p::launch_dry_fill(c->g, c->dry, c->s);   // the flags no longer describe the state
   if (c->prec == M2P_FP64) {
     HIPCHK(c, hipMemcpyAsync(ddst, u, n * sizeof(double), hipMemcpyHostToDevice, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
@@ -701,6 +711,8 @@ int m2p_set_state(m2p_ctx* c, const double* u) {
   if (int rc = upload_state(c, c->dU0, u)) return rc;
   HIPCHK(c, hipMemcpyAsync(c->dU1, c->dU0, 7 * (size_t)c->P * c->es, hipMemcpyDeviceToDevice, c->s));
   if (c->dLIM) HIPCHK(c, hipMemsetAsync(c->dLIM, 0, (size_t)c->P, c->s));
+  rescan_dry(c, c->dry, c->s);   // the flags must describe the new state
+  HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->s));
   return 0;
 }
@@ -840,10 +852,10 @@ int m2p_set_dry_skip(m2p_ctx* c, int on) {
   hipStream_t s = strm(c);
   c->par.dry_skip = on;
   c->dry.nseg = on && m2p::dry_capable(c->g) ? m2p::dry_segments(c->g) : 0;
-  // while off, no kernel keeps the flags, so they start again from "maybe wet"
+  // while off, no kernel keeps the flags, so they are set again from the state
   m2p::DryMap all = c->dry;
   all.nseg = m2p::dry_segments(c->g);
-  m2p::launch_dry_fill(c->g, all, s);
+  rescan_dry(c, all, s);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(s));
   if (c->batch) m2p_batch_mark_dirty(c->batch);
@@ -986,7 +998,7 @@ int m2p_restore(m2p_ctx* c, const char* path) {
   HIPCHK(c, hipMemcpyAsync(c->dU0, u0.data(), nst, hipMemcpyHostToDevice, c->s));
   HIPCHK(c, hipMemcpyAsync(c->dU1, u1.data(), nst, hipMemcpyHostToDevice, c->s));
   if (c->dLIM) HIPCHK(c, hipMemcpyAsync(c->dLIM, lim.data(), P, hipMemcpyHostToDevice, c->s));
-  m2p::launch_dry_fill(c->g, c->dry, c->s);
+  rescan_dry(c, c->dry, c->s);
   HIPCHK(c, hipStreamSynchronize(c->s));
   c->outtime = ot;
   c->vstop_on = st.vstop2 > 0;

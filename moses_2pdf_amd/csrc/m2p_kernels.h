@@ -54,6 +54,25 @@ inline bool dry_capable(const Grid& g) { return g.nx >= 2 && g.ny >= 2; }
 inline long long dry_flag_count(const Grid& g) { return (long long)g.ny * dry_segments(g); }
 // Sets every flag of both maps to 1 (nothing may be skipped in the next step).
 void launch_dry_fill(const Grid& g, const DryMap& dm, hipStream_t s);
+// Sets both maps from the state itself, z0 from U0 and z1 from U1, for dm.nseg segments per
+// row: a segment's flag is 0 when all 7 components of its cells are +0 bit for bit, else 1
+// (a -0 or a NaN counts as wet, so such cells are computed as without skipping).  phi is
+// tested too: a skipped tile keeps its phi, where a computed dry cell gets +0.  So an
+// initial landslide state (phi = phiS0 on every cell) has no dry segment, while a state
+// that has been stepped has its exact flags again.
+template <typename T>
+void launch_dry_scan(const Grid& g, const DryMap& dm, const T* U0, const T* U1, hipStream_t s);
+
+// A list of device-to-device copies done by one launch (the batch autotuner's snapshots of
+// every member).  `items` is in device memory; copies whose ends are both 16-byte aligned
+// move 16 bytes per lane, the others one byte.
+struct CopyItem {
+  const unsigned char* src;
+  unsigned char* dst;
+  size_t n;
+};
+constexpr int COPY_BLOCKS_PER_ITEM = 64;
+void launch_copy_list(const CopyItem* items, int nitems, hipStream_t s);
 
 // Tile plan of one stage launch under dry-tile skipping.  A plan kernel tests every tile
 // (8 lanes per tile) and lists, per XCD band (band x holds tiles [x * per, (x + 1) * per)),

@@ -391,6 +391,60 @@ void launch_inflow(const Grid& g, T* U, const double* M, const double* table, co
   k_inflow<T><<<1, 64, 0, s>>>(g, U, M, table, loc, dire, nloc, nsamp, phiS0, st, LIM, dm);
 }
 
+__device__ __forceinline__ bool nonzero_bits(double v) { return __double_as_longlong(v) != 0; }
+__device__ __forceinline__ bool nonzero_bits(float v) { return __float_as_uint(v) != 0u; }
+
+// One wave per row segment (63 cells, one per lane); blocks of 4 waves.
+template <typename T>
+__global__ void __launch_bounds__(256) k_dry_scan(Grid g, const T* __restrict__ U,
+                                                  unsigned char* __restrict__ z, int nseg) {
+  const long long seg = (long long)blockIdx.x * 4 + threadIdx.y;
+  if (seg >= (long long)g.ny * nseg) return;   // the whole wave
+  const int r = (int)(seg / nseg), c = (int)(seg % nseg) * 63 + (int)threadIdx.x;
+  bool wet = false;
+  if (threadIdx.x < 63 && c < g.nx) {
+    const long long i = idx(g, r + MD, c + MD);
+#pragma unroll
+    for (int k = 0; k < 7; k++) wet = wet || nonzero_bits(U[k * g.P + i]);
+  }
+  const unsigned long long b = __ballot(wet);
+  if (threadIdx.x == 0) z[seg] = b != 0ull;
+}
+
+template <typename T>
+void launch_dry_scan(const Grid& g, const DryMap& dm, const T* U0, const T* U1, hipStream_t s) {
+  if (dm.nseg <= 0) return;
+  const long long n = (long long)g.ny * dm.nseg;
+  const unsigned nb = (unsigned)((n + 3) / 4);
+  k_dry_scan<T><<<nb, dim3(64, 4), 0, s>>>(g, U0, dm.z0, dm.nseg);
+  k_dry_scan<T><<<nb, dim3(64, 4), 0, s>>>(g, U1, dm.z1, dm.nseg);
+}
+template void launch_dry_scan<double>(const Grid&, const DryMap&, const double*, const double*,
+                                      hipStream_t);
+template void launch_dry_scan<float>(const Grid&, const DryMap&, const float*, const float*,
+                                     hipStream_t);
+
+// COPY_BLOCKS_PER_ITEM blocks per item, each striding over the item.
+__global__ void __launch_bounds__(256) k_copy_list(const CopyItem* items) {
+  const CopyItem c = items[blockIdx.x / COPY_BLOCKS_PER_ITEM];
+  const size_t lane = (size_t)(blockIdx.x % COPY_BLOCKS_PER_ITEM) * 256 + threadIdx.x;
+  const size_t stride = (size_t)COPY_BLOCKS_PER_ITEM * 256;
+  size_t done = 0;
+  if ((((uintptr_t)c.src | (uintptr_t)c.dst) & 15) == 0) {
+    const uint4* s = (const uint4*)c.src;
+    uint4* d = (uint4*)c.dst;
+    const size_t n16 = c.n / 16;
+    for (size_t i = lane; i < n16; i += stride) d[i] = s[i];
+    done = n16 * 16;
+  }
+  for (size_t i = done + lane; i < c.n; i += stride) c.dst[i] = c.src[i];
+}
+
+void launch_copy_list(const CopyItem* items, int nitems, hipStream_t s) {
+  if (nitems <= 0) return;
+  k_copy_list<<<(unsigned)nitems * COPY_BLOCKS_PER_ITEM, 256, 0, s>>>(items);
+}
+
 void launch_dry_fill(const Grid& g, const DryMap& dm, hipStream_t s) {
   if (dm.nseg <= 0) return;
   const long long n = dry_flag_count(g);
