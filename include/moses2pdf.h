@@ -9,9 +9,12 @@
  * replaces.
  *
  * Conventions: every function returns 0 on success and a negative M2P_E* code on error
- * (the library never exits); m2p_last_error() gives the message.  One context owns one
- * HIP stream and all of its device memory.  A context is not thread-safe; distinct
- * contexts may be driven from different host threads.  Buffers passed in are owned by
+ * (the library never exits); m2p_last_error() gives the message.  A context owns all of
+ * its device memory and orders all of its work on one HIP stream, taken round robin from
+ * a pool of 8 per device (creating a stream costs about a millisecond, and the device
+ * serves streams from 4 hardware queues): contexts sharing a stream also wait for each
+ * other's queued work.  A context is not thread-safe; distinct contexts may be driven
+ * from different host threads.  Buffers passed in are owned by
  * the caller.  State planes are laid out as in the reference: 7 planes of nxd*nyd
  * doubles, plane k holding u[k] at index row*nxd+col, with nxd = nx+6, nyd = ny+6
  * (three ghost cells per side), k = 0 hs, 1 hs*us, 2 hs*vs, 3 hf, 4 hf*uf, 5 hf*vf,
@@ -137,7 +140,8 @@ int m2p_scalars(m2p_ctx* ctx, double* dt, double* dmax, int* iter);
 int m2p_get_field(m2p_ctx* ctx, const char* name, double* out);
 
 /* Mean device time per launch of each kernel family since the last reset, measured with
- * HIP events on the context's stream when profiling is enabled. */
+ * HIP events on the context's stream when profiling is enabled (work of another context
+ * queued on the same pool stream in between would be counted too). */
 int m2p_profile(m2p_ctx* ctx, int enable);
 int m2p_profile_get(m2p_ctx* ctx, const char* kernel, double* ms_per_launch, int* launches);
 

@@ -11,6 +11,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -30,6 +31,32 @@ int fail(m2p_ctx* c, int code, const std::string& msg) {
 
 // The stream a scenario's work is ordered on: its own, or its batch's while in a batch.
 hipStream_t strm(m2p_ctx* c) { return c->batch ? m2p_batch_stream(c->batch) : c->s; }
+
+// Scenarios take their stream from a small per-device pool, round robin, instead of each
+// creating one: creating a HIP stream costs about a millisecond (a quarter of opening a
+// 128^2 scenario), and the device serves all streams from a few hardware queues (4 by
+// default).  Every operation of a scenario stays on its one stream, so its work keeps its
+// order; scenarios sharing a stream also wait for each other's queued work.  The pool's
+// streams live until the process ends.
+constexpr int STREAM_POOL = 8;
+
+hipError_t pool_stream(hipStream_t* out) {
+  static std::mutex mu;
+  static std::map<int, std::vector<hipStream_t>> pools;
+  static std::map<int, unsigned> next;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lock(mu);
+  std::vector<hipStream_t>& v = pools[dev];
+  if (v.size() < (size_t)STREAM_POOL) {
+    hipStream_t s = nullptr;
+    if ((e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) != hipSuccess) return e;
+    v.push_back(s);
+  }
+  *out = v[next[dev]++ % v.size()];
+  return hipSuccess;
+}
 
 #define IN_BATCH(c)                                                                        \
   if ((c)->batch) return fail((c), M2P_ESTATE, "scenario is stepped by a batch")
@@ -93,6 +120,7 @@ int upload(m2p_ctx* c, double* d, const std::vector<double>& h) {
 }
 
 void release(m2p_ctx* c) {
+  if (c->s) (void)hipStreamSynchronize(c->s);   // this scenario's queued work is done
   for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
   c->evpool.clear();
   c->pending.clear();
@@ -111,8 +139,7 @@ void release(m2p_ctx* c) {
   c->dry = m2p::DryMap{};
   c->dDire = nullptr;
   c->dSt = nullptr;
-  if (c->s) (void)hipStreamDestroy(c->s);
-  c->s = nullptr;
+  c->s = nullptr;   // a pool stream: not destroyed
 }
 
 // Both dry-flag maps (dm.nseg segments per row) from the state buffers, after they were
@@ -126,7 +153,7 @@ void rescan_dry(m2p_ctx* c, const m2p::DryMap& dm, hipStream_t s) {
 
 // Device allocation, uploads, metric setup (L/R:395-642) and the initial state.
 int setup(m2p_ctx* c) {
-  HIPCHK(c, hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+  HIPCHK(c, pool_stream(&c->s));
   HIPCHK(c, m2p::probe_kernels());
   const size_t P = (size_t)c->P;
   int rc;
