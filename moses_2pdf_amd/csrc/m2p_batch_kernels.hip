@@ -36,18 +36,20 @@ __global__ void __launch_bounds__(GHOST_LINES) kb_ghost(BatchDev B, int post) {
   ghost_line(m.g, m.mode, U, prev, line, (int)blockIdx.y);
 }
 
-// Blocks stride over the batch's CFL boxes, 8 at a time: one round of loads tests all 8
-// (32 threads per box; members without dry-tile skipping are always computed), then each
-// box's maximum goes to its own partial slot (a dry box's is the wave-speed floor).
-// Halted members' slots are left alone: their dt kernel does not read them.  The blocks
-// after the first nbb take the ring faces in the same launch: each of their 4 waves is one
-// ring block of the block map (one member's 32 ring faces), with its own slot.  nbb is
-// B.cfl_blocks or fewer (a tuned parameter): with fewer blocks each one does more of the
-// dry tests and skips, with more the wet boxes spread over more blocks.
+// Blocks stride over the batch's CFL boxes, CFL_GROUP at a time: one round of loads tests
+// all of them (CFL_LANES lanes per box; members without dry-tile skipping are always
+// computed) and writes the dry boxes' slots (the wave-speed floor), then the block computes
+// the group's wet boxes one after another, each maximum into its box's slot.  Halted
+// members' slots are left alone: their dt kernel does not read them.  The blocks after the
+// first nbb take the ring faces in the same launch: each of their 4 waves is one ring block
+// of the block map (one member's 32 ring faces), with its own slot.  nbb is B.cfl_blocks
+// or fewer (a tuned parameter).
+constexpr int CFL_GROUP = 64;
+constexpr int CFL_LANES = 256 / CFL_GROUP;
 
 template <typename T>
 __global__ void __launch_bounds__(256) kb_cfl(BatchDev B, int nbb) {
-  __shared__ int state[8];   // 0: dry, 1: compute, 2: halted
+  __shared__ int state[CFL_GROUP];   // 0: dry, 1: compute, 2: halted or past the end
   const int tid = threadIdx.y * 64 + threadIdx.x;
   if ((int)blockIdx.x >= nbb) {   // ring faces, one ring block per wave
     const int rb = ((int)blockIdx.x - nbb) * 4 + threadIdx.y;
@@ -60,39 +62,43 @@ __global__ void __launch_bounds__(256) kb_cfl(BatchDev B, int nbb) {
     if (threadIdx.x == 0) B.part[B.nBox + rb] = v;
     return;
   }
-  for (int t0 = blockIdx.x; t0 < B.nBox; t0 += 8 * nbb) {
+  for (int t0 = blockIdx.x; t0 < B.nBox; t0 += CFL_GROUP * nbb) {
     __syncthreads();   // every thread is done with the previous group's state
-    if (tid < 8) state[tid] = 0;
-    __syncthreads();
     {
-      const int k = tid >> 5, t = t0 + k * nbb;
+      const int k = tid / CFL_LANES, q = tid % CFL_LANES, t = t0 + k * nbb;
+      int wet = 0, st = 2;
+      const Member* mp = nullptr;
       if (t < B.nBox) {
-        const Member& m = B.mem[B.mapBox[t]];
-        if (halted_pre(m.st)) {
-          if ((tid & 31) == 0) state[k] = 2;
-        } else if (m.dry.nseg == 0) {
-          if ((tid & 31) == 0) state[k] = 1;
+        mp = &B.mem[B.mapBox[t]];
+        if (halted_pre(mp->st)) {
+          st = 2;
+        } else if (mp->dry.nseg == 0) {
+          st = 1;
         } else {
-          const int lb = t - m.box0;
-          const DryRegion d = box_region(m.g, lb % m.nbx, lb / m.nbx);
-          int wet = 0;
-          for (int e = tid & 31; e < d.n; e += 32)
-            wet |= dry_item_wet(d, (const T*)m.U0, m.g, m.dry.z0, m.dry.nseg, e);
-          if (wet) atomicOr(&state[k], 1);
+          st = -1;   // from the test
+          const int lb = t - mp->box0;
+          const DryRegion d = box_region(mp->g, lb % mp->nbx, lb / mp->nbx);
+          for (int e = q; e < d.n; e += CFL_LANES)
+            wet |= dry_item_wet(d, (const T*)mp->U0, mp->g, mp->dry.z0, mp->dry.nseg, e);
         }
+      }
+      // the box's CFL_LANES lanes are adjacent lanes of one wave
+      const unsigned long long bal = __ballot(wet != 0);
+      const int l0 = (int)threadIdx.x & ~(CFL_LANES - 1);
+      const bool any = ((bal >> l0) & ((1ull << CFL_LANES) - 1ull)) != 0ull;
+      if (q == 0) {
+        if (st < 0) {
+          st = any ? 1 : 0;
+          if (!any) B.part[t] = cfl_floor<T>(mp->g);
+        }
+        state[k] = st;
       }
     }
     __syncthreads();
-    for (int k = 0; k < 8; k++) {
+    for (int k = 0; k < CFL_GROUP; k++) {
+      if (state[k] != 1) continue;
       const int t = t0 + k * nbb;
-      if (t >= B.nBox) break;
-      const int s = state[k];
-      if (s == 2) continue;
       const Member& m = B.mem[B.mapBox[t]];
-      if (s == 0) {
-        if (tid == 0) B.part[t] = cfl_floor<T>(m.g);
-        continue;
-      }
       const int lb = t - m.box0;
       block_max_store(cfl_box((const T*)m.U0, (const T*)m.MT, m.g, m.ph, lb % m.nbx, lb / m.nbx),
                       B.part + t);
