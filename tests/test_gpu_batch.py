@@ -132,6 +132,40 @@ def test_tune_restores_state_and_keeps_results():
     b.close()
 
 
+def test_tune_with_halted_and_inflow_members():
+    # the tuner steps the batch once before its trials and puts every member back after:
+    # a member halted before the tune stays halted at its step, an inflow member keeps its
+    # hydrograph position, and the others continue as if the tune had not run
+    scs = population()
+    sims = [Simulation(sc) for sc in scs]
+    b = Batch("fp64")
+    for s in sims:
+        b.add(s)
+    sims[0].set_stop(max_steps=3)
+    b.step(5)
+    b.sync()
+    before = [(s.state(), s.status()) for s in sims]
+    assert before[0][1][1] == 3 and before[0][1][2]
+    b.tune(trial_steps=1)
+    after = [(s.state(), s.status()) for s in sims]
+    for (u0, st0), (u1, st1) in zip(before, after):
+        assert st0 == st1
+        assert np.array_equal(u0.view(np.uint64), u1.view(np.uint64))
+    b.step(7)
+    b.sync()
+    got = [(s.state(), s.time_steps()) for s in sims]
+    b.close()
+    for s in sims:
+        s.close()
+    ref = alone(scs[1:], 12, "fp64")
+    for k, (x, y) in enumerate(zip(got[1:], ref)):
+        assert x[1] == y[1], k
+        assert np.array_equal(x[0], y[0]), k
+    with Simulation(scs[0]) as s:
+        s.step(3)
+        assert np.array_equal(got[0][0], s.state())
+
+
 def test_members_halt_on_their_own_rules():
     scs = population()[:4]
     sims = [Simulation(sc) for sc in scs]
