@@ -142,6 +142,19 @@ void release(m2p_ctx* c) {
   c->s = nullptr;   // a pool stream: not destroyed
 }
 
+// A landslide's momenta ghost cells in both state buffers set to 0 after a host-side write
+// of the state, which the ghost fills rely on (launch_zero_ghost_momenta).
+void zero_ghost_momenta(m2p_ctx* c, hipStream_t s) {
+  if (c->mode != 0) return;
+  if (c->prec == M2P_FP64) {
+    m2p::launch_zero_ghost_momenta<double>(c->g, (double*)c->dU0, s);
+    m2p::launch_zero_ghost_momenta<double>(c->g, (double*)c->dU1, s);
+  } else {
+    m2p::launch_zero_ghost_momenta<float>(c->g, (float*)c->dU0, s);
+    m2p::launch_zero_ghost_momenta<float>(c->g, (float*)c->dU1, s);
+  }
+}
+
 // Both dry-flag maps (dm.nseg segments per row) from the state buffers, after they were
 // replaced (a new state, a restore, skipping switched on).
 void rescan_dry(m2p_ctx* c, const m2p::DryMap& dm, hipStream_t s) {
@@ -738,6 +751,7 @@ int m2p_set_state(m2p_ctx* c, const double* u) {
   if (int rc = upload_state(c, c->dU0, u)) return rc;
   HIPCHK(c, hipMemcpyAsync(c->dU1, c->dU0, 7 * (size_t)c->P * c->es, hipMemcpyDeviceToDevice, c->s));
   if (c->dLIM) HIPCHK(c, hipMemsetAsync(c->dLIM, 0, (size_t)c->P, c->s));
+  zero_ghost_momenta(c, c->s);
   rescan_dry(c, c->dry, c->s);   // the flags must describe the new state
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->s));
@@ -1025,6 +1039,7 @@ int m2p_restore(m2p_ctx* c, const char* path) {
   HIPCHK(c, hipMemcpyAsync(c->dU0, u0.data(), nst, hipMemcpyHostToDevice, c->s));
   HIPCHK(c, hipMemcpyAsync(c->dU1, u1.data(), nst, hipMemcpyHostToDevice, c->s));
   if (c->dLIM) HIPCHK(c, hipMemcpyAsync(c->dLIM, lim.data(), P, hipMemcpyHostToDevice, c->s));
+  zero_ghost_momenta(c, c->s);
   rescan_dry(c, c->dry, c->s);
   HIPCHK(c, hipStreamSynchronize(c->s));
   c->outtime = ot;

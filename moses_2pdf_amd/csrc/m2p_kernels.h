@@ -120,6 +120,13 @@ void launch_inflow(const Grid& g, T* U, const double* M, const double* table, co
 template <typename T>
 void launch_ghost(const Grid& g, int mode, T* Tg, const T* prev, const DevState* st, int post,
                   hipStream_t s);
+// A landslide's ghost cells of the momenta (planes 1, 2, 4, 5) hold 0 at all times: every
+// ghost fill would write 0 there (Boundary7/8), nothing else writes a landslide's ghost
+// cells, and a host-side write of the state clears them with this kernel.  So the ghost
+// fills leave those planes alone.  (Inflow strips write momenta into ghost cells, so an
+// inflow fill writes all 7 planes.)
+template <typename T>
+void launch_zero_ghost_momenta(const Grid& g, T* U, hipStream_t s);
 // Per-block partial maxima of one step (CFL box blocks, CFL ring blocks, corrector tiles),
 // reduced by the dt kernel: partial_count() doubles per scenario.
 size_t partial_count(const Grid& g, int prec);

@@ -196,7 +196,19 @@ __global__ void k_ghost(Grid g, int mode, T* __restrict__ Tg, const T* __restric
                         const DevState* __restrict__ st, int post) {
   const int line = blockIdx.x * blockDim.x + threadIdx.x;
   if (line >= g.nyd + g.nxd || (post ? halted_post(st) : halted_pre(st))) return;
-  ghost_line(g, mode, Tg, prev, line, (int)blockIdx.y);
+  // a landslide fills planes 0, 3, 6 only: its momenta ghosts hold 0 (ghost_momenta_kept)
+  const int k = mode == 0 ? 3 * (int)blockIdx.y : (int)blockIdx.y;
+  ghost_line(g, mode, Tg, prev, line, k);
+}
+
+// The ghost ring of the momenta planes (1, 2, 4, 5) set to 0, as a landslide ghost fill
+// would (after a host-side write of the state).
+template <typename T>
+__global__ void k_zero_ghost_momenta(Grid g, T* __restrict__ U) {
+  const int line = blockIdx.x * blockDim.x + threadIdx.x;
+  if (line >= g.nyd + g.nxd) return;
+  const int k = blockIdx.y < 2 ? 1 + (int)blockIdx.y : 2 + (int)blockIdx.y;
+  ghost_line(g, 0, U, U, line, k);
 }
 
 // Blocks [0, nbb) stride over the 64x4 face boxes (at most CFL_BLOCKS blocks, so the dt
@@ -457,7 +469,13 @@ template <typename T>
 void launch_ghost(const Grid& g, int mode, T* Tg, const T* prev, const DevState* st, int post,
                   hipStream_t s) {
   const int lines = g.nyd + g.nxd;
-  k_ghost<T><<<dim3((lines + 127) / 128, MN), 128, 0, s>>>(g, mode, Tg, prev, st, post);
+  k_ghost<T><<<dim3((lines + 127) / 128, mode == 0 ? 3 : MN), 128, 0, s>>>(g, mode, Tg, prev, st,
+                                                                          post);
+}
+
+template <typename T>
+void launch_zero_ghost_momenta(const Grid& g, T* U, hipStream_t s) {
+  k_zero_ghost_momenta<T><<<dim3((g.nyd + g.nxd + 127) / 128, 4), 128, 0, s>>>(g, U);
 }
 
 static int cfl_boxes(const Grid& g) { return ((g.nx + 63) / 64) * ((g.ny + 3) / 4); }
@@ -522,6 +540,7 @@ void launch_mass(const Grid& g, const T* U, const double* M, double* out, hipStr
                                  unsigned char*, const DryMap&, hipStream_t);                  \
   template void launch_ghost<T>(const Grid&, int, T*, const T*, const DevState*, int,         \
                                 hipStream_t);                                                 \
+  template void launch_zero_ghost_momenta<T>(const Grid&, T*, hipStream_t);                     \
   template void launch_cfl<T>(const Grid&, const Phys&, const T*, const T*, DevState*, double*, \
                               const DryMap&, hipStream_t);                                     \
   template void launch_plan<T>(int, const Grid&, const T*, const DryMap&, const TilePlan&,     \
