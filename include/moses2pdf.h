@@ -120,7 +120,9 @@ int m2p_sync(m2p_ctx* ctx);
  * step count out.  Any pointer may be NULL. */
 int m2p_get_state(m2p_ctx* ctx, double* u, double* t, int* nt);
 
-/* Overwrites the state (7*nxd*nyd doubles, all cells including ghosts). */
+/* Overwrites the state (7*nxd*nyd doubles, all cells including ghosts).  In landslide
+ * mode the ghost cells of the momenta (planes 1, 2, 4, 5) are set to 0, the value every
+ * step's ghost fill gives them before anything reads them. */
 int m2p_set_state(m2p_ctx* ctx, const double* u);
 
 int m2p_dims(const m2p_ctx* ctx, int* nx, int* ny, int* nxd, int* nyd);
