@@ -25,10 +25,6 @@ def population(count: int = 256, seed: int = 4):
     return ms, [S.ensemble_scenario(m) for m in ms]
 
 
-def _full(h):
-    return h
-
-
 def run_plain(scs, steps, precision="fp64", keep=True, dry_skip=True):
     """Each scenario alone, one after another, `steps` m2p_step steps each."""
     t0 = time.perf_counter()
