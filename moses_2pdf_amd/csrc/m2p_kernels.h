@@ -75,7 +75,7 @@ constexpr int COPY_BLOCKS_PER_ITEM = 64;
 void launch_copy_list(const CopyItem* items, int nitems, hipStream_t s);
 
 // Tile plan of one stage launch under dry-tile skipping.  A plan kernel tests every tile
-// (PLAN_LANES = 4 lanes per tile) and lists, per XCD band (band x holds tiles [x * per, (x + 1) * per)),
+// (PLAN_LANES = 2 lanes per tile) and lists, per XCD band (band x holds tiles [x * per, (x + 1) * per)),
 // the tiles the stage must visit: those to compute and, for the predictor, dry tiles whose
 // U1 and ONE must be cleared (entry | PLAN_ZERO).  Block b of the stage runs on XCD b % 8
 // and takes entry b / 8 of list b % 8; blocks past the list's end return at once, so a dry

@@ -765,7 +765,7 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
 // entry: t to compute it, t | PLAN_ZERO (predictor) to clear its U1 and ONE because its U1
 // flags say they may hold values of an earlier step, or -1 for nothing to do.  A group
 // whose tile does not exist passes t < 0 (it still takes part in the wave's ballots).
-constexpr int PLAN_LANES = 4;
+constexpr int PLAN_LANES = 2;
 
 template <int IO, typename T, int TY>
 __device__ __forceinline__ int plan_tile(const Grid& g, const T* __restrict__ Uin,
