@@ -1,7 +1,7 @@
 """Parity of the HIP path (through the C ABI) with the CPU oracle on the same inputs.
 
-The bar (north star): <= 1e-10 relative on depths and momenta, total solid and fluid
-mass equal to 1e-12 relative.  The implementation keeps the reference's operation order
+The bar: <= 1e-10 relative on depths and momenta, total solid and fluid mass equal to
+1e-12 relative.  The implementation keeps the reference's operation order
 with FMA contraction off, so the tests also report how many values differ at all; the
 expected count is 0.
 """
