@@ -218,7 +218,7 @@ def main() -> int:
     ap.add_argument("--hbm-budget", type=float, default=200.0, help="GB")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (one MI355X per box); "
+        print("bench.py: measures one GPU only (the multi-GPU path is a design, not built); "
               "run with --gpus 1", file=sys.stderr)
         return 2
     if args.workload == "ensemble":
