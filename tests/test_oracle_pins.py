@@ -289,3 +289,22 @@ def test_oracle_reproduces_golden_states():
         h = digest(u)
         assert h == str(g[f"{name}/sha256"]), name
         assert t == float(g[f"{name}/time"]) and dt == float(g[f"{name}/dt"])
+
+
+def test_landslide_ghost_momenta_stay_zero():
+    # Boundary7/8 (L/K:905-956) give a landslide's momenta ghost cells 0 in every stage and
+    # nothing else writes a landslide's ghost cells, so after any step they hold 0; the
+    # HIP ghost fills rely on this and leave those planes alone (an inflow run's strips
+    # write momenta into ghost cells, so inflow fills all planes)
+    sc = S.landslide_fractal(n=48, seed=3)
+    o = Oracle(sc)
+    ring = np.ones((o.nyd, o.nxd), dtype=bool)
+    ring[3:-3, 3:-3] = False
+    moved = False
+    for _ in range(4):
+        o.step(5)
+        u = o.get("u")
+        for k in (1, 2, 4, 5):
+            assert not np.any(u[k][ring]), k
+        moved = moved or bool(np.any(u[1][~ring] != 0))
+    assert moved
