@@ -308,3 +308,19 @@ def test_landslide_ghost_momenta_stay_zero():
             assert not np.any(u[k][ring]), k
         moved = moved or bool(np.any(u[1][~ring] != 0))
     assert moved
+
+
+def test_dry_cells_hold_positive_zero_in_every_component():
+    # Flux20 (L/K:2682-2709) sets a dry phase's depth and momenta to +0 and Flux21 gives
+    # phi = 0/(0+0+H0) = +0, so after a step every interior cell without material holds +0
+    # bit for bit in all 7 components: a tile that skips such cells leaves exactly what
+    # computing them writes
+    sc = S.landslide_fractal(n=48, seed=3)
+    o = Oracle(sc)
+    o.step(12)
+    u = o.get("u")[:, 3:-3, 3:-3]
+    dry = (u[0] == 0) & (u[3] == 0)
+    assert dry.sum() > 100 and (~dry).sum() > 10
+    bits = u.view(np.uint64)
+    for k in range(7):
+        assert not np.any(bits[k][dry]), k
