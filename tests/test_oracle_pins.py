@@ -324,3 +324,39 @@ def test_dry_cells_hold_positive_zero_in_every_component():
     bits = u.view(np.uint64)
     for k in range(7):
         assert not np.any(bits[k][dry]), k
+
+
+@pytest.mark.parametrize("slope,v0,rel", [(0.5, 0.0, 1e-7), (0.1, 3.0, 5e-5)])
+def test_sliding_layer_on_an_incline_closed_form(slope, v0, rel):
+    # A uniform solid layer (phiS0 = 1, no fluid) on a plane of slope tan(theta): every
+    # flux difference vanishes in the interior, so the horizontal velocity us = u1/u0
+    # follows the sources alone, and a block sliding with Coulomb friction on an incline
+    # accelerates along the slope at g (sin(theta) - mu cos(theta)), horizontally
+    # c^2 (tan(theta) - mu) with c = cos(theta), g = 1 in the scaled units and
+    # mu = (1 - rho_r) tan(delta) (the basal load is buoyancy-reduced, Flux16/17
+    # L/K:2530-2621; tan(delta) with the reference's 3.14159, L/KH:17).  The RK2 steps
+    # integrate a constant acceleration exactly once the layer moves.  Sliding down a
+    # 0.5 slope from rest, and braking on a 0.1 slope from us = 3, pin the sign and size
+    # of gravity and friction.  Tolerance: the drag term with no fluid is
+    # Cd*phi*(1-phi)*h*us ~ Cd*H0*us, about 1e-5 of the acceleration at us = 3.
+    n, dx = 48, 20.0
+    x = np.arange(n) * dx
+    topo = np.tile((n * dx - x) * slope, (n, 1))   # exact in float after the x10 scaling
+    sc = S.Scenario(mode=0, topo=topo, depth=np.full((n, n), 2.0), cellsize=dx, phiS0=1.0,
+                    delta0=20.0)
+    o = Oracle(sc)
+    if v0:
+        u = o.get("u")
+        u[1] = u[0] * v0
+        o.set_u(u)
+    o.step(2)   # the first step switches the friction on
+    c = 3 + n // 2
+    u_a, t_a = o.get("u"), o.scalar("time")
+    o.step(6)
+    u_b, t_b = o.get("u"), o.scalar("time")
+    acc = (u_b[1][c, c] / u_b[0][c, c] - u_a[1][c, c] / u_a[0][c, c]) / (t_b - t_a)
+    mu = (1.0 - sc.rho_ratio) * math.tan(20.0 * 3.14159 / 180.0)
+    expect = (slope - mu) / (1.0 + slope * slope)
+    assert acc == pytest.approx(expect, rel=rel)
+    assert u_b[0][c, c] == u_a[0][c, c]   # the depth does not change
+    assert u_b[2][c, c] == 0.0            # nor does the cross-slope momentum
