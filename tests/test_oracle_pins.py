@@ -360,3 +360,43 @@ def test_sliding_layer_on_an_incline_closed_form(slope, v0, rel):
     assert acc == pytest.approx(expect, rel=rel)
     assert u_b[0][c, c] == u_a[0][c, c]   # the depth does not change
     assert u_b[2][c, c] == 0.0            # nor does the cross-slope momentum
+
+
+@pytest.mark.parametrize("slope,v0", [(0.3, 0.0), (0.0, 5.0)])
+def test_fluid_layer_on_an_incline_follows_the_midpoint_rule(slope, v0):
+    # A uniform fluid layer (phiS0 = 0, no solid) on a plane of slope tan(theta): the flux
+    # differences vanish in the interior and the fluid feels gravity on its load and the
+    # linear resistance of Flux17 (L/K:2562-2621), so its horizontal velocity obeys
+    #   dv/dt = r (c^2 tan(theta) - (varTheta / N_R) v),   r = hf / (hf + H0),
+    # with v = u4 / (u3 + H0) as the scheme forms it (r is that regularisation).  The
+    # predictor (dt/2) and corrector (dt) are the explicit midpoint rule, so v follows the
+    # midpoint recurrence of this ODE step by step, with each step's dt read back, to
+    # round-off.  Accelerating from rest on a 0.3 slope and relaxing from v = 5 on flat
+    # ground pin the gravity term and the resistance (its sign and varTheta / N_R).
+    n, dx = 48, 20.0
+    x = np.arange(n) * dx
+    topo = np.tile((n * dx - x) * slope, (n, 1))
+    sc = S.Scenario(mode=0, topo=topo, depth=np.full((n, n), 2.0), cellsize=dx, phiS0=0.0,
+                    delta0=20.0)
+    o = Oracle(sc)
+    c = 3 + n // 2
+    if v0:
+        u = o.get("u")
+        u[4] = u[3] * v0
+        o.set_u(u)
+    H0 = 1.0e-6
+    A, k = slope / (1.0 + slope * slope), sc.varTheta / sc.N_R
+    u = o.get("u")
+    r = u[3][c, c] / (u[3][c, c] + H0)
+    v, t = u[4][c, c] / (u[3][c, c] + H0), o.scalar("time")
+    f = lambda w: r * (A - k * w)
+    for _ in range(15):
+        o.step(1)
+        t_next = o.scalar("time")
+        dt = t_next - t
+        v = v + dt * f(v + 0.5 * dt * f(v))
+        t = t_next
+        u = o.get("u")
+        assert u[4][c, c] / (u[3][c, c] + H0) == pytest.approx(v, rel=1e-12)
+        assert u[3][c, c] / (u[3][c, c] + H0) == r   # the depth does not change
+    assert abs(v - v0) > 0.5   # the velocity did move
