@@ -400,3 +400,31 @@ def test_fluid_layer_on_an_incline_follows_the_midpoint_rule(slope, v0):
         assert u[4][c, c] / (u[3][c, c] + H0) == pytest.approx(v, rel=1e-12)
         assert u[3][c, c] / (u[3][c, c] + H0) == r   # the depth does not change
     assert abs(v - v0) > 0.5   # the velocity did move
+
+
+def test_phase_drag_conserves_momentum_and_relaxes():
+    # On flat ground with no basal friction (delta0 = 0) and no fluid resistance
+    # (varTheta = 0), the only source in a uniform two-phase layer is the drag between the
+    # phases (Flux17 L/K:2562-2621), which only moves momentum from one phase to the
+    # other: the density-weighted total u1 + rho_r u4 (solid plus fluid momentum over the
+    # solid density) must stay constant step after step, and the relative velocity must
+    # decay without changing sign (Cd small enough that the explicit step does not
+    # overshoot, so the velocity-crossing limiter stays idle).
+    n, dx = 48, 20.0
+    sc = S.Scenario(mode=0, topo=np.zeros((n, n)), depth=np.full((n, n), 2.0), cellsize=dx,
+                    phiS0=0.5, delta0=0.0, Cd=0.5, varTheta=0.0)
+    o = Oracle(sc)
+    c = 3 + n // 2
+    u = o.get("u")
+    u[4] = u[3] * 2.0   # the fluid moves, the solid rests
+    o.set_u(u)
+    H0, rr = 1.0e-6, sc.rho_ratio
+    Q0 = u[1][c, c] + rr * u[4][c, c]
+    rel = [2.0]
+    for _ in range(20):
+        o.step(1)
+        u = o.get("u")
+        assert u[1][c, c] + rr * u[4][c, c] == pytest.approx(Q0, rel=1e-13)
+        rel.append(u[4][c, c] / (u[3][c, c] + H0) - u[1][c, c] / (u[0][c, c] + H0))
+    assert all(a > b > 0.0 for a, b in zip(rel, rel[1:]))
+    assert rel[-1] < 1e-4 * rel[0]
