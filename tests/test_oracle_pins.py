@@ -428,3 +428,35 @@ def test_phase_drag_conserves_momentum_and_relaxes():
         rel.append(u[4][c, c] / (u[3][c, c] + H0) - u[1][c, c] / (u[0][c, c] + H0))
     assert all(a > b > 0.0 for a, b in zip(rel, rel[1:]))
     assert rel[-1] < 1e-4 * rel[0]
+
+
+def test_depth_flux_is_the_central_upwind_combination():
+    # On flat ground the metric is the identity (Det = 1, invJ = I, no surface slope), so
+    # the physical flux of the solid depth through an x-face is h u with u = m / (h + H0),
+    # and the numerical flux (Flux8, L/K:2039-2211) must be the central-upwind / Rusanov
+    # combination of the two face states with the face's wave speed a:
+    #   H = 1/2 (F_E + F_W - a Det (h_E - h_W)),  E = right state, W = left state
+    # (likewise for y-faces with N/S).  Checked on every interior face of a random state
+    # after a step (the face arrays then all belong to the corrector stage).
+    n, dx = 32, 20.0
+    rng = np.random.default_rng(7)
+    sc = S.Scenario(mode=0, topo=np.zeros((n, n)), depth=1.0 + 0.5 * rng.random((n, n)),
+                    cellsize=dx, phiS0=1.0, delta0=20.0)
+    o = Oracle(sc)
+    u = o.get("u")
+    u[1] = u[0] * rng.uniform(-1, 1, u[0].shape)
+    u[2] = u[0] * rng.uniform(-1, 1, u[0].shape)
+    o.set_u(u)
+    o.step(1)
+    H0 = 1.0e-6
+    inner = (slice(4, -4), slice(4, -4))
+    for a, b, k, ap, det, H in (("uE", "uW", 1, "apEW", "Detmin_avgEW", "Hpx"),
+                                ("uN", "uS", 2, "apSN", "Detmin_avgSN", "Hpy")):
+        ua, ub, c, D = o.get(a), o.get(b), o.get(ap), o.get(det)
+        assert np.all(D[inner] == 1.0)
+        Fa = ua[0] * (ua[k] / (ua[0] + H0)) * D
+        Fb = ub[0] * (ub[k] / (ub[0] + H0)) * D
+        expect = 0.5 * (Fa + Fb - c * D * (ua[0] - ub[0]))
+        got = o.get(H)[0]
+        assert np.abs(c * D * (ua[0] - ub[0]))[inner].mean() > 0.1 * np.abs(got[inner]).mean()
+        np.testing.assert_allclose(got[inner], expect[inner], rtol=1e-13, atol=1e-13)
