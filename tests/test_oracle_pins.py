@@ -430,14 +430,9 @@ def test_phase_drag_conserves_momentum_and_relaxes():
     assert rel[-1] < 1e-4 * rel[0]
 
 
-def test_depth_flux_is_the_central_upwind_combination():
-    # On flat ground the metric is the identity (Det = 1, invJ = I, no surface slope), so
-    # the physical flux of the solid depth through an x-face is h u with u = m / (h + H0),
-    # and the numerical flux (Flux8, L/K:2039-2211) must be the central-upwind / Rusanov
-    # combination of the two face states with the face's wave speed a:
-    #   H = 1/2 (F_E + F_W - a Det (h_E - h_W)),  E = right state, W = left state
-    # (likewise for y-faces with N/S).  Checked on every interior face of a random state
-    # after a step (the face arrays then all belong to the corrector stage).
+def _flat_solid_layer_after_a_step():
+    # a random solid-only state (phiS0 = 1) on flat ground, one step on: the face arrays
+    # then all belong to the corrector stage
     n, dx = 32, 20.0
     rng = np.random.default_rng(7)
     sc = S.Scenario(mode=0, topo=np.zeros((n, n)), depth=1.0 + 0.5 * rng.random((n, n)),
@@ -448,6 +443,17 @@ def test_depth_flux_is_the_central_upwind_combination():
     u[2] = u[0] * rng.uniform(-1, 1, u[0].shape)
     o.set_u(u)
     o.step(1)
+    return sc, o
+
+
+def test_depth_flux_is_the_central_upwind_combination():
+    # On flat ground the metric is the identity (Det = 1, invJ = I, no surface slope), so
+    # the physical flux of the solid depth through an x-face is h u with u = m / (h + H0),
+    # and the numerical flux (Flux8, L/K:2039-2211) must be the central-upwind / Rusanov
+    # combination of the two face states with the face's wave speed a:
+    #   H = 1/2 (F_E + F_W - a Det (h_E - h_W)),  E = right state, W = left state
+    # (likewise for y-faces with N/S), on every interior face.
+    sc, o = _flat_solid_layer_after_a_step()
     H0 = 1.0e-6
     inner = (slice(4, -4), slice(4, -4))
     for a, b, k, ap, det, H in (("uE", "uW", 1, "apEW", "Detmin_avgEW", "Hpx"),
@@ -460,3 +466,23 @@ def test_depth_flux_is_the_central_upwind_combination():
         got = o.get(H)[0]
         assert np.abs(c * D * (ua[0] - ub[0]))[inner].mean() > 0.1 * np.abs(got[inner]).mean()
         np.testing.assert_allclose(got[inner], expect[inner], rtol=1e-13, atol=1e-13)
+
+
+def test_momentum_flux_and_wave_speed_are_shallow_water():
+    # On flat ground a solid-only layer is a shallow-water layer under the buoyancy-reduced
+    # gravity g' = 1 - rho_r (g = 1): the flux of the normal momentum m through a face is
+    # m u + g' h^2 / 2 (Flux6/7 L/K:1773-1936), combined central-upwind as the depth flux,
+    # and the face's wave speed is the larger of |u| + sqrt(g' h) over its two states
+    # (Flux4/5 L/K:1685-1770).
+    sc, o = _flat_solid_layer_after_a_step()
+    H0, g = 1.0e-6, 1.0 - sc.rho_ratio
+    inner = (slice(4, -4), slice(4, -4))
+    for a, b, k, ap, H in (("uE", "uW", 1, "apEW", "Hpx"), ("uN", "uS", 2, "apSN", "Hpy")):
+        ua, ub, c = o.get(a), o.get(b), o.get(ap)
+        va, vb = ua[k] / (ua[0] + H0), ub[k] / (ub[0] + H0)
+        speed = np.maximum(np.abs(va) + np.sqrt(g * ua[0]), np.abs(vb) + np.sqrt(g * ub[0]))
+        np.testing.assert_allclose(c[inner], speed[inner], rtol=1e-14)
+        Fa = ua[k] * va + 0.5 * g * ua[0] ** 2
+        Fb = ub[k] * vb + 0.5 * g * ub[0] ** 2
+        expect = 0.5 * (Fa + Fb - c * (ua[k] - ub[k]))
+        np.testing.assert_allclose(o.get(H)[k][inner], expect[inner], rtol=1e-13, atol=1e-13)
