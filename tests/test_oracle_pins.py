@@ -486,3 +486,34 @@ def test_momentum_flux_and_wave_speed_are_shallow_water():
         Fb = ub[k] * vb + 0.5 * g * ub[0] ** 2
         expect = 0.5 * (Fa + Fb - c * (ua[k] - ub[k]))
         np.testing.assert_allclose(o.get(H)[k][inner], expect[inner], rtol=1e-13, atol=1e-13)
+
+
+def test_fluid_fluxes_are_shallow_water():
+    # The fluid branch of the same fluxes: a fluid-only layer (phiS0 = 0) on flat ground is
+    # a shallow-water layer under the full gravity g = 1, whose fluid depth and normal
+    # momentum fluxes are the central-upwind combinations of h u and m u + h^2 / 2 with the
+    # fluid wave speed |u| + sqrt(h / (1 - phi + H0)) (phi = 0 here) of each face.
+    n, dx = 32, 20.0
+    rng = np.random.default_rng(8)
+    sc = S.Scenario(mode=0, topo=np.zeros((n, n)), depth=1.0 + 0.5 * rng.random((n, n)),
+                    cellsize=dx, phiS0=0.0, delta0=20.0)
+    o = Oracle(sc)
+    u = o.get("u")
+    u[4] = u[3] * rng.uniform(-1, 1, u[3].shape)
+    u[5] = u[3] * rng.uniform(-1, 1, u[3].shape)
+    o.set_u(u)
+    o.step(1)
+    H0 = 1.0e-6
+    inner = (slice(4, -4), slice(4, -4))
+    for a, b, k, H in (("uE", "uW", 4, "Hpx"), ("uN", "uS", 5, "Hpy")):
+        ua, ub = o.get(a), o.get(b)
+        va, vb = ua[k] / (ua[3] + H0), ub[k] / (ub[3] + H0)
+        pa, pb = ua[0] / (ua[0] + ua[3] + H0), ub[0] / (ub[0] + ub[3] + H0)
+        speed = np.maximum(np.abs(va) + np.sqrt((ua[0] + ua[3]) / (1 - pa + H0)),
+                           np.abs(vb) + np.sqrt((ub[0] + ub[3]) / (1 - pb + H0)))
+        got = o.get(H)
+        depth = 0.5 * (ua[3] * va + ub[3] * vb - speed * (ua[3] - ub[3]))
+        mom = 0.5 * (ua[k] * va + 0.5 * ua[3] ** 2 + ub[k] * vb + 0.5 * ub[3] ** 2
+                     - speed * (ua[k] - ub[k]))
+        np.testing.assert_allclose(got[3][inner], depth[inner], rtol=1e-13, atol=1e-13)
+        np.testing.assert_allclose(got[k][inner], mom[inner], rtol=1e-13, atol=1e-13)
