@@ -517,3 +517,34 @@ def test_fluid_fluxes_are_shallow_water():
                      - speed * (ua[k] - ub[k]))
         np.testing.assert_allclose(got[3][inner], depth[inner], rtol=1e-13, atol=1e-13)
         np.testing.assert_allclose(got[k][inner], mom[inner], rtol=1e-13, atol=1e-13)
+
+
+def test_fluid_shear_viscosity_closed_form():
+    # On flat ground the fluid viscous fluxes (Flux8-10, L/K:2039-2347) are the
+    # depth-integrated Newtonian stresses h tau with tau_xy = mu (du/dy + dv/dx),
+    # mu = (1 - phi) / N_R.  A fluid-only layer of uniform depth with the shear profile
+    # u = beta y^2, v = 0 (y counted from 100 cells below the grid, so the profile is
+    # monotone and the minmod face-state jumps are all equal, which makes the upwind
+    # dissipation cancel between faces) and no resistance (varTheta = 0) then gains
+    # du/dt = mu d2u/dy2 = 2 beta / N_R everywhere (times h / (h + H0), the scheme's
+    # regularisation): after time t every interior row has gained r 2 beta t / (dy^2 N_R).
+    n, dx = 40, 1.0
+    sc = S.Scenario(mode=0, topo=np.zeros((n, n)), depth=np.full((n, n), 2.0), cellsize=dx,
+                    phiS0=0.0, delta0=20.0, varTheta=0.0, N_R=1.0)
+    o = Oracle(sc)
+    H0, beta = 1.0e-6, 1.0e-3
+    u = o.get("u")
+    q0 = beta * (np.arange(o.nyd)[:, None] + 100.0) ** 2 * np.ones((1, o.nxd))
+    u[4] = (u[3] + H0) * q0
+    o.set_u(u)
+    c = o.nyd // 2
+    r = u[3][c, c] / (u[3][c, c] + H0)
+    dy = 10.0 * dx   # the reference's lengths are scaled by 10
+    t0 = o.scalar("time")
+    o.step(10)
+    t = o.scalar("time") - t0
+    u = o.get("u")
+    band = slice(c - 4, c + 5)
+    gained = u[4][band, c] / (u[3][band, c] + H0) - q0[band, c]
+    np.testing.assert_allclose(gained, r * 2.0 * beta * t / (dy * dy * sc.N_R), rtol=1e-7)
+    assert np.all(np.abs(u[5][band, c]) < 1e-30)   # no cross-stream velocity appears
