@@ -20,6 +20,18 @@
 //     InterfacesKernel wrote, then all y-ghost overrides, reading the x-overridden values;
 //   * the inflow Boundary3Kernel (I/K:704-815): x copies first, then y copies;
 //   * CFL4Kernel (L/K:2023-2036): TotalTime advances by dt once per step.
+//
+// How it is pinned (tests/test_oracle_pins.py): closed forms of the metric on planes and
+// valleys, textbook minmod, exact interface states of linear data, the lake at rest, mass
+// conservation, mirror symmetry, the closed-form dt, the sliding-block acceleration
+// (gravity, Coulomb friction, buoyancy), the midpoint rule of a fluid layer's
+// gravity-resistance ODE, momentum exchange by the phase drag, the shallow-water fluxes
+// and wave speeds of both phases, the central-upwind combination, the fluid shear
+// viscosity, and the DF004 injection strips.  Parity unpinned (only the self-generated
+// golden states and the symmetry pin reach them): the factor 2 of the normal viscous
+// stress in Flux10, Flux9's transform of the velocity gradients off flat ground, and the
+// curvature terms of Flux15; also what only the CUDA reference on NVIDIA hardware could
+// show (the racy orders above, uninitialised memory, CUDA's device tan).
 
 #include <algorithm>
 #include <cmath>
