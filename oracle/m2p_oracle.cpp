@@ -27,11 +27,11 @@
 // (gravity, Coulomb friction, buoyancy), the midpoint rule of a fluid layer's
 // gravity-resistance ODE, momentum exchange by the phase drag, the shallow-water fluxes
 // and wave speeds of both phases, the central-upwind combination, the fluid shear
-// viscosity, and the DF004 injection strips.  Parity unpinned (only the self-generated
-// golden states and the symmetry pin reach them): the factor 2 of the normal viscous
-// stress in Flux10, Flux9's transform of the velocity gradients off flat ground, and the
-// curvature terms of Flux15; also what only the CUDA reference on NVIDIA hardware could
-// show (the racy orders above, uninitialised memory, CUDA's device tan).
+// viscosity on flat and tilted ground, and the DF004 injection strips.  Parity unpinned
+// (only the self-generated golden states and the symmetry pin reach them): the factor 2
+// of the normal viscous stress in Flux10, Flux9's transform for a tilt along the shear
+// direction, and the curvature terms of Flux15; also what only the CUDA reference on
+// NVIDIA hardware could show (the racy orders above, uninitialised memory, device tan).
 
 #include <algorithm>
 #include <cmath>

@@ -519,7 +519,8 @@ def test_fluid_fluxes_are_shallow_water():
         np.testing.assert_allclose(got[k][inner], mom[inner], rtol=1e-13, atol=1e-13)
 
 
-def test_fluid_shear_viscosity_closed_form():
+@pytest.mark.parametrize("slope", [0.0, 0.5])
+def test_fluid_shear_viscosity_closed_form(slope):
     # On flat ground the fluid viscous fluxes (Flux8-10, L/K:2039-2347) are the
     # depth-integrated Newtonian stresses h tau with tau_xy = mu (du/dy + dv/dx),
     # mu = (1 - phi) / N_R.  A fluid-only layer of uniform depth with the shear profile
@@ -528,9 +529,14 @@ def test_fluid_shear_viscosity_closed_form():
     # dissipation cancel between faces) and no resistance (varTheta = 0) then gains
     # du/dt = mu d2u/dy2 = 2 beta / N_R everywhere (times h / (h + H0), the scheme's
     # regularisation): after time t every interior row has gained r 2 beta t / (dy^2 N_R).
+    # On a plane tilted along x (slope 0.5) the y direction is untouched by the tilt, so
+    # the transform of the gradients to surface coordinates (Flux9) must cancel its
+    # tilt terms and leave the same viscous gain, on top of gravity's r c^2 tan(theta) t.
     n, dx = 40, 1.0
-    sc = S.Scenario(mode=0, topo=np.zeros((n, n)), depth=np.full((n, n), 2.0), cellsize=dx,
-                    phiS0=0.0, delta0=20.0, varTheta=0.0, N_R=1.0)
+    x = np.arange(n) * dx
+    sc = S.Scenario(mode=0, topo=np.tile((n * dx - x) * slope, (n, 1)),
+                    depth=np.full((n, n), 2.0), cellsize=dx, phiS0=0.0, delta0=20.0,
+                    varTheta=0.0, N_R=1.0)
     o = Oracle(sc)
     H0, beta = 1.0e-6, 1.0e-3
     u = o.get("u")
@@ -546,5 +552,7 @@ def test_fluid_shear_viscosity_closed_form():
     u = o.get("u")
     band = slice(c - 4, c + 5)
     gained = u[4][band, c] / (u[3][band, c] + H0) - q0[band, c]
-    np.testing.assert_allclose(gained, r * 2.0 * beta * t / (dy * dy * sc.N_R), rtol=1e-7)
+    visc = r * 2.0 * beta * t / (dy * dy * sc.N_R)
+    gravity = r * slope / (1.0 + slope * slope) * t
+    np.testing.assert_allclose(gained - gravity, visc, rtol=1e-7)
     assert np.all(np.abs(u[5][band, c]) < 1e-30)   # no cross-stream velocity appears
