@@ -27,11 +27,12 @@
 // (gravity, Coulomb friction, buoyancy), the midpoint rule of a fluid layer's
 // gravity-resistance ODE, momentum exchange by the phase drag, the shallow-water fluxes
 // and wave speeds of both phases, the central-upwind combination, the fluid shear
-// viscosity on flat and tilted ground, and the DF004 injection strips.  Parity unpinned
-// (only the self-generated golden states and the symmetry pin reach them): the factor 2
-// of the normal viscous stress in Flux10, Flux9's transform for a tilt along the shear
-// direction, and the curvature terms of Flux15; also what only the CUDA reference on
-// NVIDIA hardware could show (the racy orders above, uninitialised memory, device tan).
+// viscosity on flat and tilted ground, the viscous fluxes as the depth-integrated
+// Newtonian film stress on planes of any tilt (vector geometry: the factor 2 of the normal
+// stresses and every term of Flux9's transform), the centripetal term of the basal loads
+// as the second fundamental form of the bed on quadratic beds (Flux15-16), and the DF004
+// injection strips.  Parity unpinned: only what the CUDA reference on NVIDIA hardware
+// could show (the racy orders above, uninitialised memory, device tan).
 
 #include <algorithm>
 #include <cmath>
@@ -1179,6 +1180,17 @@ void orc_faces(void* h) {
   o->cflReduce();
 }
 
+// orc_faces, then the numerical and viscous fluxes (Flux8-10, L/K:2039-2347) and the cell
+// sources (Flux11-17, L/K:2349-2621) of that predictor, still without updating u, so
+// PDx, PDy, the transformed gradients, Ac, AcF and the basal loads hold that stage's
+// values.  Test hook.
+void orc_stage_terms(void* h) {
+  Orc* o = (Orc*)h;
+  orc_faces(h);
+  o->numFluxes();
+  o->sources();
+}
+
 // Applies only the inflow injection of the next step (I/K:818-887).  Test hook.
 void orc_inject(void* h) {
   Orc* o = (Orc*)h;
@@ -1264,7 +1276,20 @@ int orc_get(void* h, const char* name, double* out) {
                  {"invJ23_avgSN", &o->invJ23_avgSN},
                  {"Detmin_avgSN", &o->Detmin_avgSN},
                  {"dxdxi12_avgSN", &o->dxdxi12_avgSN},
-                 {"dxdxi22_avgSN", &o->dxdxi22_avgSN}};
+                 {"dxdxi22_avgSN", &o->dxdxi22_avgSN},
+                 {"dudxE", &o->dudxE},
+                 {"dvdxE", &o->dvdxE},
+                 {"dudyE", &o->dudyE},
+                 {"dvdyE", &o->dvdyE},
+                 {"dudxN", &o->dudxN},
+                 {"dvdxN", &o->dvdxN},
+                 {"dudyN", &o->dudyN},
+                 {"dvdyN", &o->dvdyN},
+                 {"Ac", &o->Ac},
+                 {"AcF", &o->AcF},
+                 {"Npress1", &o->Npress1},
+                 {"Npress2", &o->Npress2},
+                 {"NpressF", &o->NpressF}};
   for (auto& t : tab)
     if (std::strcmp(t.n, name) == 0) {
       std::memcpy(out, t.v->data(), t.v->size() * sizeof(double));

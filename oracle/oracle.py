@@ -48,6 +48,7 @@ def lib():
         L.orc_scalar.restype = ctypes.c_double
         L.orc_faces.argtypes = [ctypes.c_void_p]
         L.orc_inject.argtypes = [ctypes.c_void_p]
+        L.orc_stage_terms.argtypes = [ctypes.c_void_p]
         L.orc_nxd.argtypes = [ctypes.c_void_p]
         L.orc_nyd.argtypes = [ctypes.c_void_p]
         L.orc_threads.argtypes = [ctypes.c_int]
@@ -170,6 +171,11 @@ class Oracle:
     def faces(self) -> None:
         """First half of a predictor (no update): fills ux, uy, uE.., Val, dmax."""
         lib().orc_faces(self.h)
+
+    def stage_terms(self) -> None:
+        """faces(), then the numerical and viscous fluxes and the cell sources of that
+        predictor (no update): fills PDx, PDy, the face gradients, Ac, AcF and the loads."""
+        lib().orc_stage_terms(self.h)
 
     def inject(self) -> None:
         lib().orc_inject(self.h)

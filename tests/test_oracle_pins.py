@@ -556,3 +556,124 @@ def test_fluid_shear_viscosity_closed_form(slope):
     gravity = r * slope / (1.0 + slope * slope) * t
     np.testing.assert_allclose(gained - gravity, visc, rtol=1e-7)
     assert np.all(np.abs(u[5][band, c]) < 1e-30)   # no cross-stream velocity appears
+
+
+# ---------------------------------------------------------------------------------------
+# viscous stress through the surface transform (Flux8-10) and the centripetal load
+# (Flux15-16), read directly from one predictor's terms
+# ---------------------------------------------------------------------------------------
+
+def _film_stress_on_a_plane(tx, ty, a, b, c, d, h):
+    """Depth-integrated Newtonian stress of a thin film on the plane z = tx x + ty y, from
+    vector geometry alone: the film's 3-D velocity is V = (u, v, tx u + ty v) with the
+    horizontal velocity u = a x + b y, v = c x + d y; its surface strain rate is the
+    symmetric part of the tangential velocity gradient; the stress (per unit viscosity) is
+    twice that; the force on a face is the depth h times the stress applied to the face's
+    unit normal within the surface, times the face's length per unit grid coordinate.
+    Returns the horizontal (x, y) components of the force on an x-face (a line x = const)
+    and on a y-face."""
+    ex, ey = np.array([1.0, 0.0, tx]), np.array([0.0, 1.0, ty])
+    n = np.array([-tx, -ty, 1.0])
+    n /= np.linalg.norm(n)
+    dVx = np.array([a, c, tx * a + ty * c])          # dV/dx along the surface
+    dVy = np.array([b, d, tx * b + ty * d])          # dV/dy along the surface
+    E = np.stack([ex, ey])
+    G = E @ E.T                                       # metric of the (x, y) parametrisation
+    Lg = np.stack([dVx, dVy], axis=1) @ np.linalg.inv(G) @ E   # tangential gradient
+    Pn = np.eye(3) - np.outer(n, n)
+    tau = Pn @ (Lg + Lg.T) @ Pn                       # 2 x strain rate
+
+    def force(t, toward):
+        m = np.cross(t, n)
+        m /= np.linalg.norm(m)
+        if m @ toward < 0:
+            m = -m
+        return h * (tau @ m) * np.linalg.norm(t)
+    fx, fy = force(ey, ex), force(ex, ey)
+    return fx[0], fx[1], fy[0], fy[1]
+
+
+@pytest.mark.parametrize("tx,ty", [(0.0, 0.0), (0.5, 0.0), (0.0, 0.5), (0.5, 0.25),
+                                   (-0.25, 0.5)])
+def test_viscous_fluxes_are_the_film_stress_on_a_plane(tx, ty):
+    # The viscous fluxes of the fluid momenta (PDx4,5 through x-faces, PDy4,5 through
+    # y-faces; Flux8-10, L/K:2039-2347) are the depth-integrated Newtonian stress of the
+    # film, h tau with tau = 2 D (the viscosity (1 - phi)/N_R multiplies them in the
+    # update), projected on the horizontal axes.  On a plane of any tilt a fluid layer of
+    # uniform depth with a linear horizontal velocity field has a uniform strain rate, so
+    # every interior face must carry the force computed from vector geometry above.  This
+    # pins the factor 2 of the normal stresses (flat case), and every correction term of
+    # the transform to surface coordinates (Flux9), each of which the tilted cases weigh
+    # with a nonzero gradient.
+    n, dx = 40, 1.0
+    x = np.arange(n) * dx
+    topo = 1000.0 + tx * x[None, :] + ty * x[:, None]      # exact in float
+    sc = S.Scenario(mode=0, topo=topo, depth=np.full((n, n), 2.0), cellsize=dx, phiS0=0.0,
+                    delta0=20.0, varTheta=0.0, N_R=1.0)
+    o = Oracle(sc)
+    u = o.get("u")
+    u[3] = 20.0                    # uniform fluid depth (the edge columns' c differs)
+    A, B, C, D = 1.0e-3, 2.0e-3, -3.0e-3, 5.0e-3            # increments per cell
+    rows, cols = np.arange(o.nyd)[:, None], np.arange(o.nxd)[None, :]
+    u[4] = (u[3] + H0) * (A * cols + B * rows + 0.1)
+    u[5] = (u[3] + H0) * (C * cols + D * rows - 0.2)
+    o.set_u(u)
+    o.stage_terms()
+    ds = 10.0 * dx                                          # lengths are scaled by 10
+    want = _film_stress_on_a_plane(tx, ty, A / ds, B / ds, C / ds, D / ds, 20.0)
+    PDx, PDy = o.get("PDx"), o.get("PDy")
+    inner = (slice(5, -5), slice(5, -5))
+    for got, w in ((PDx[4], want[0]), (PDx[5], want[1]), (PDy[4], want[2]), (PDy[5], want[3])):
+        assert abs(w) > 1e-4
+        np.testing.assert_allclose(got[inner], w, rtol=1e-11)
+    for k in range(4):                                      # no viscous flux of depth or solid
+        assert np.all(PDx[k] == 0.0) and np.all(PDy[k] == 0.0)
+
+
+@pytest.mark.parametrize("k1,k12,k2", [(1 / 512, 0.0, 0.0), (0.0, 0.0, 1 / 512),
+                                       (1 / 512, 1 / 1024, -1 / 2048),
+                                       (1 / 1024, -1 / 512, 1 / 512)])
+def test_basal_load_carries_the_centripetal_term(k1, k12, k2):
+    # A layer moving over a curved bed presses on it with gravity's normal component plus
+    # the centripetal force of following the bed: the normal acceleration of a path on the
+    # surface is the second fundamental form of its velocity, II(V, V) = c (v . Hess(b) v)
+    # for the bed z = b(x, y), c = 1/sqrt(1 + |grad b|^2) and horizontal velocity v.  The
+    # scheme's curvature term (Flux15, L/K:2495-2527) is Ac = V . (dn/dt along the path)
+    # = -II(V, V), so on the quadratic bed b = (k1 x^2 + 2 k12 x y + k2 y^2)/2 (exact in
+    # float; its central differences are exact) with uniform velocities, Ac = -c v'Hv for
+    # each phase, to the second-order error of the differentiated normal, and the basal
+    # loads of Flux16 (L/K:2530-2559) follow with Det = 1/c:
+    #   Npress2 = Det hs (c - Ac), Npress1 = Det hs ((1 - rr) c - Ac + rr AcF),
+    #   NpressF = Det hf (c - AcF).
+    n, dx = 48, 4.0
+    i = np.arange(n) - n // 2
+    X, Y = i[None, :] * dx, i[:, None] * dx
+    topo = 100.0 + 0.5 * (k1 * X * X + 2 * k12 * X * Y + k2 * Y * Y)
+    assert np.array_equal(f32(topo), topo)
+    sc = S.Scenario(mode=0, topo=topo, depth=np.full((n, n), 2.0), cellsize=dx, phiS0=0.6,
+                    delta0=20.0)
+    o = Oracle(sc)
+    u = o.get("u")
+    us, vs, uf, vf = 30.0, -20.0, -25.0, 15.0               # ~30 m/s, scaled units
+    u[1], u[2] = (u[0] + H0) * us, (u[0] + H0) * vs
+    u[4], u[5] = (u[3] + H0) * uf, (u[3] + H0) * vf
+    o.set_u(u)
+    o.stage_terms()
+    Hs = np.array([[k1, k12], [k12, k2]]) / 10.0            # curvature per scaled length
+    Xs = (np.arange(o.nxd) - 3 - n // 2)[None, :] * dx * 10.0
+    Ys = (np.arange(o.nyd) - 3 - n // 2)[:, None] * dx * 10.0
+    bx, by = Hs[0, 0] * Xs + Hs[0, 1] * Ys, Hs[1, 0] * Xs + Hs[1, 1] * Ys
+    c = 1.0 / np.sqrt(1.0 + bx * bx + by * by)
+
+    def vHv(a, b):
+        return Hs[0, 0] * a * a + 2 * Hs[0, 1] * a * b + Hs[1, 1] * b * b
+    Ac, AcF = -c * vHv(us, vs), -c * vHv(uf, vf)
+    inner = (slice(8, -8), slice(8, -8))
+    assert np.abs(Ac[inner] / c[inner]).max() > 0.03        # the term matters
+    np.testing.assert_allclose(o.get("Ac")[inner], Ac[inner], rtol=2e-4)
+    np.testing.assert_allclose(o.get("AcF")[inner], AcF[inner], rtol=2e-4)
+    rr, hs, hf = sc.rho_ratio, u[0], u[3]
+    np.testing.assert_allclose(o.get("Npress2")[inner], (hs * (c - Ac) / c)[inner], rtol=1e-4)
+    np.testing.assert_allclose(o.get("Npress1")[inner],
+                               (hs * ((1 - rr) * c - Ac + rr * AcF) / c)[inner], rtol=1e-4)
+    np.testing.assert_allclose(o.get("NpressF")[inner], (hf * (c - AcF) / c)[inner], rtol=1e-4)
