@@ -85,7 +85,7 @@ def ensemble(args) -> int:
     steps = args.ens_steps
     # plain FP64 path: each scenario alone, one after another
     dry = not args.dense
-    t_plain, ref = E.run_plain(scs, steps, "fp64", dry_skip=dry)
+    t_plain, ref, _ = E.run_plain(scs, steps, "fp64", dry_skip=dry)
     # batched FP64 and FP32 (autotuned, graph-captured)
     b64 = E.run_batch(scs, steps, "fp64", keep=False, dry_skip=dry)
     b32 = E.run_batch(scs, steps, "fp32", keep=False, dry_skip=dry)

@@ -47,7 +47,12 @@ enum {
 
 /* Run parameters.  The first seven are par_List lines 1-7 (adNOC_2Pb_runKernels.cu:21-40);
  * the last four are constants the reference hard-codes (adNOC_2Pb_kernels.cuh:16,
- * adNOC_2Pb_kernels.cu:1710 and 2014-2017; inflow 2060) and default to those values. */
+ * adNOC_2Pb_kernels.cu:1710 and 2014-2017; inflow 2060) and default to those values.
+ * The reference reads the first seven with sscanf and checks nothing; m2p_open and
+ * m2p_open_grid return M2P_EINVAL with the reason in m2p_last_error unless every value is
+ * finite and total_sim >= 0, each_out > 0, 0 <= delta0 < 90, Cd >= 0, N_R > 0,
+ * varTheta >= 0, 0 <= phiS0 <= 1 (inflow: 0 < phiS0 <= 1, its injection divides by it),
+ * 0 < cfl <= 1, 0 <= rho_ratio < 1 and 0 < dt_min <= dt_max. */
 typedef struct m2p_params {
   double total_sim;   /* seconds */
   double each_out;    /* seconds */
@@ -75,7 +80,7 @@ void m2p_default_params(int mode, m2p_params* p);
  * the files it names (relative paths resolve against the current directory, as in the
  * reference), parses them with the reference's precision rules and uploads them.
  * `overlay` is NULL or "key=value;key=value" over cfl, rho_ratio, dt_min, dt_max,
- * precision (fp64 | fp32) and dry_skip (0 | 1). */
+ * precision (fp64 | fp32) and dry_skip (0 | 1); a number must be the whole value. */
 int m2p_open(const char* par_list, int mode, const char* overlay, m2p_ctx** out);
 
 /* In-memory constructor (no files).  topo[NY*NX] and depth[NY*NX] hold the grid values
@@ -110,7 +115,9 @@ int m2p_step(m2p_ctx* ctx, int nsteps);
  * the last step shortened to end on it, or until max_steps steps are done; *steps_done
  * (nullable) receives the count.  No reference counterpart: it replaces the output-time
  * clipping of the loop (adNOC_2Pb_runKernels.cu:983-988) for a single end time, so that
- * runs of different precision or dt can be compared at the same time. */
+ * runs of different precision or dt can be compared at the same time.  The halting rules
+ * of m2p_set_stop stay in force during the call (it also stops where they halt the
+ * scenario) and are left unchanged by it. */
 int m2p_run_to(m2p_ctx* ctx, double t_end, int max_steps, int* steps_done);
 
 /* Waits for all work queued on the context's stream. */
@@ -155,17 +162,18 @@ void m2p_close(m2p_ctx* ctx);
 /* ---- Checkpoint / resume (no reference counterpart: its text outputs are lossy) ---- */
 
 /* Writes the scenario's resumable state to `path` (binary, written to path.tmp and then
- * renamed): a header (magic "M2PCKPT1", version 1, mode, NX, NY, precision, output-time
- * count, P), the device loop state (time, dt, output and inflow counters, step count,
- * halting rules), the Time.dat history of m2p_run, the 7 state planes of u^n and of the
+ * renamed): a header (magic "M2PCKPT1", version 2, mode, NX, NY, precision, output-time
+ * count, P, and an FNV-1a hash of the padded topography, the run parameters and the
+ * hydrograph tables), the device loop state (time, dt, output and inflow counters, step
+ * count, halting rules, non-finite flag), the Time.dat history of m2p_run, the 7 state planes of u^n and of the
  * half-step buffer (its ghost rows feed the next step's landslide corners), the FP32
  * limiter flags, and an FNV-1a checksum.  Stepping a context restored from it gives
  * bit-identical results to never having stopped. */
 int m2p_checkpoint(m2p_ctx* ctx, const char* path);
 
-/* Loads a checkpoint into a context opened on the same scenario (mode, grid, precision
- * and output times must match; M2P_EINVAL otherwise, with the reason in
- * m2p_last_error).  m2p_run then continues the output sequence where it stopped. */
+/* Loads a checkpoint into a context opened on the same scenario (mode, grid, precision,
+ * output times, topography, parameters and hydrograph tables must match; M2P_EINVAL
+ * otherwise, with the reason in m2p_last_error).  m2p_run then continues the output sequence where it stopped. */
 int m2p_restore(m2p_ctx* ctx, const char* path);
 
 /* ---- Halting rules and status (no reference counterpart) -------------------------- */
@@ -185,6 +193,18 @@ int m2p_set_dry_skip(m2p_ctx* ctx, int on);
 
 /* Simulated time (scaled), steps done and whether the scenario has halted. */
 int m2p_status(m2p_ctx* ctx, double* t, int* nt, int* halted);
+
+/* Whether the state has gone non-finite.  The reference's dt reduction drops a NaN wave
+ * speed through fmaxf (adNOC_2Pb_kernels.cu:1988, 1996), and its speed floor and face
+ * maxima drop it before that (Flux5/6Kernel, adNOC_2Pb_kernels.cu:1762-1807), so a run
+ * that blows up keeps stepping NaNs with no sign of it.  Here the CFL pass of every step
+ * also marks a face whose one-sided wave speeds are not all finite, in the same
+ * reduction as dt (which still ignores it, so results are unchanged): *flag is 1 once
+ * that has happened, and *first_nt the step count of the first state it was seen in (the
+ * state after that many steps held a NaN or an Inf; -1 while *flag is 0).  The CFL pass
+ * reads u^n, so a state is seen by the step after the one that made it.  m2p_set_state
+ * clears the flag; a checkpoint carries it. */
+int m2p_nonfinite(m2p_ctx* ctx, int* flag, int* first_nt);
 
 /* Device bytes a scenario of this shape holds once open (*steady) and at most while it
  * is being set up (*peak), for admission by HBM footprint. */

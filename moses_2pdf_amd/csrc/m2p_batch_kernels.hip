@@ -60,7 +60,7 @@ __global__ void __launch_bounds__(256) kb_cfl(BatchDev B, int nbb) {
     if (halted_pre(m.st)) return;
     const int t = (rb - m.ring0) * 64 + threadIdx.x;
     double v = cfl_ring_face((const T*)m.U0, (const T*)m.MT, m.g, m.ph, t, m.dry);
-    v = wave_max(v);
+    v = wave_cfl(v);
     if (threadIdx.x == 0) B.part[B.nBox + rb] = v;
     return;
   }
@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(256) kb_cfl(BatchDev B, int nbb) {
       const int t = t0 + k * nbb;
       const Member& m = B.mem[B.mapBox[t]];
       const int lb = t - m.box0;
-      block_max_store(cfl_box((const T*)m.U0, (const T*)m.MT, m.g, m.ph, lb % m.nbx, lb / m.nbx),
+      block_cfl_store(cfl_box((const T*)m.U0, (const T*)m.MT, m.g, m.ph, lb % m.nbx, lb / m.nbx),
                       B.part + t);
     }
   }
@@ -117,9 +117,9 @@ __global__ void __launch_bounds__(256) kb_dt(BatchDev B) {
   if (threadIdx.x == 0) halted = dt_halted(m.st) ? 1 : 0;
   __syncthreads();
   if (halted) return;
-  const double mb = block_max_of(B.part + m.box0, m.box_n);
-  const double mr = block_max_of(B.part + B.nBox + m.ring0, m.ring_n);
-  if (threadIdx.x == 0) dt_body(m.dp, m.st, m.tf, m.itime, fmax(mb, mr));
+  const double mb = block_cfl_of(B.part + m.box0, m.box_n);
+  const double mr = block_cfl_of(B.part + B.nBox + m.ring0, m.ring_n);
+  if (threadIdx.x == 0) dt_body(m.dp, m.st, m.tf, m.itime, cfl_join(mb, mr));
 }
 
 // One block per member after the corrector: its squared max flow speed into DevState.

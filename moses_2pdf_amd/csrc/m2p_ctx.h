@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
 #include <map>
 #include <string>
 #include <utility>
@@ -59,4 +60,7 @@ struct m2p_ctx {
   std::vector<std::pair<const char*, std::pair<hipEvent_t, hipEvent_t>>> pending;
   m2p_batch* batch = nullptr;  // the batch that steps this scenario, if any
   bool vstop_on = false;       // a stop speed is set (the step reduces corrector speeds)
+  // FNV-1a hash of what a checkpoint's state is only valid with: the padded topography,
+  // the run parameters and the hydrograph tables (set when the context is opened)
+  uint64_t scen_hash = 0;
 };

@@ -241,6 +241,9 @@ int setup(m2p_ctx* c) {
   st.t_end = -1.0;
   st.halt_at = INT_MAX;
   st.v2_nt = -1;
+  st.call_t_end = -1.0;
+  st.call_halt = INT_MAX;
+  st.nonfinite_nt = -1;
   HIPCHK(c, hipMemcpyAsync(c->dSt, &st, sizeof st, hipMemcpyHostToDevice, c->s));
   m2p::launch_metric_setup(c->g, c->mode, c->dTopo, c->dDepth, c->dPhi, c->par.phiS0, c->dM,
                            dU64, c->s);
@@ -433,7 +436,9 @@ int parse_overlay(const char* ov, m2p_params& p) {
       else return -1;
       continue;
     }
-    double v = std::strtod(val.c_str(), nullptr);
+    char* endp = nullptr;
+    const double v = std::strtod(val.c_str(), &endp);
+    if (val.empty() || endp != val.c_str() + val.size() || !std::isfinite(v)) return -1;
     if (k == "cfl") p.cfl = v;
     else if (k == "rho_ratio") p.rho_ratio = v;
     else if (k == "dt_min") p.dt_min = v;
@@ -441,6 +446,56 @@ int parse_overlay(const char* ov, m2p_params& p) {
     else return -1;
   }
   return 0;
+}
+
+// The reason a parameter set cannot run, or "" (the reference reads par_List with sscanf
+// and checks nothing, adNOC_2Pb_runKernels.cu:11-48).
+std::string check_params(int mode, const m2p_params& p) {
+  auto fin = [](double x) { return std::isfinite(x); };
+  if (!fin(p.total_sim) || p.total_sim < 0) return "total_sim must be finite and >= 0";
+  if (!fin(p.each_out) || !(p.each_out > 0)) return "each_out must be finite and > 0";
+  if (!fin(p.delta0) || p.delta0 < 0 || !(p.delta0 < 90)) return "delta0 must be in [0, 90) degrees";
+  if (!fin(p.Cd) || p.Cd < 0) return "Cd must be finite and >= 0";
+  if (!fin(p.N_R) || !(p.N_R > 0)) return "N_R must be finite and > 0";
+  if (!fin(p.varTheta) || p.varTheta < 0) return "varTheta must be finite and >= 0";
+  if (!fin(p.phiS0) || p.phiS0 < 0 || p.phiS0 > 1) return "phiS0 must be in [0, 1]";
+  if (mode == 1 && !(p.phiS0 > 0)) return "phiS0 must be > 0 in inflow mode (the injection divides by it)";
+  if (!fin(p.cfl) || !(p.cfl > 0) || p.cfl > 1) return "cfl must be in (0, 1]";
+  if (!fin(p.rho_ratio) || p.rho_ratio < 0 || !(p.rho_ratio < 1)) return "rho_ratio must be in [0, 1)";
+  if (!fin(p.dt_min) || !(p.dt_min > 0)) return "dt_min must be finite and > 0";
+  if (!fin(p.dt_max) || p.dt_max < p.dt_min) return "dt_max must be finite and >= dt_min";
+  if (p.precision != M2P_FP64 && p.precision != M2P_FP32) return "precision must be M2P_FP64 or M2P_FP32";
+  if (p.dry_skip != 0 && p.dry_skip != 1) return "dry_skip must be 0 or 1";
+  return "";
+}
+
+// FNV-1a over the scenario's topography, parameters and hydrograph tables.
+uint64_t scenario_hash(const m2p_ctx* c) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](const void* p, size_t n) {
+    const unsigned char* b = (const unsigned char*)p;
+    for (size_t i = 0; i < n; i++) {
+      h ^= b[i];
+      h *= 1099511628211ull;
+    }
+  };
+  const double pv[] = {c->par.total_sim, c->par.each_out, c->par.delta0, c->par.Cd, c->par.N_R,
+                       c->par.varTheta,  c->par.phiS0,    c->par.cfl,    c->par.rho_ratio,
+                       c->par.dt_min,    c->par.dt_max,   c->dx};
+  mix(pv, sizeof pv);
+  mix(c->topo.data(), c->topo.size() * sizeof(double));
+  mix(c->table.data(), c->table.size() * sizeof(double));
+  mix(c->loc.data(), c->loc.size() * sizeof(double));
+  mix(c->itime.data(), c->itime.size() * sizeof(double));
+  return h;
+}
+
+// A context that holds only an error message, for the caller to read and close.
+int reject(m2p_ctx** out, int code, const std::string& msg) {
+  m2p_ctx* c = new m2p_ctx();
+  c->err = msg;
+  *out = c;
+  return code;
 }
 
 }  // namespace
@@ -475,8 +530,10 @@ int m2p_open_grid(int mode, int NX, int NY, double cellsize, const double* topo,
     return M2P_EINVAL;
   if (mode == 0 && !depth) return M2P_EINVAL;
   if (mode == 1 && (nsamp < 1 || nloc < 1 || !t || !h || !u || !v || !loc)) return M2P_EINVAL;
-  if (par->precision != M2P_FP64 && par->precision != M2P_FP32) return M2P_EINVAL;
-  if (par->dry_skip != 0 && par->dry_skip != 1) return M2P_EINVAL;
+  {
+    const std::string why = check_params(mode, *par);
+    if (!why.empty()) return reject(out, M2P_EINVAL, "invalid parameters: " + why);
+  }
   m2p_ctx* c = new m2p_ctx();
   c->mode = mode;
   c->prec = par->precision;
@@ -535,6 +592,7 @@ int m2p_open_grid(int mode, int NX, int NY, double cellsize, const double* topo,
       }
     }
   }
+  c->scen_hash = scenario_hash(c);
   int rc = setup(c);
   if (rc) {
     // keep the context so the caller can read the message
@@ -571,7 +629,8 @@ int m2p_open(const char* par_list, int mode, const char* overlay, m2p_ctx** out)
   p.N_R = pl.N_R;
   p.varTheta = pl.varTheta;
   p.phiS0 = pl.phiS0;
-  if (parse_overlay(overlay, p)) return bad(M2P_EINVAL, "bad overlay string");
+  if (parse_overlay(overlay, p))
+    return bad(M2P_EINVAL, std::string("bad overlay string: ") + overlay);
   int rc;
   m2p_ctx* c = nullptr;
   if (mode == 0) {
@@ -589,6 +648,10 @@ int m2p_open(const char* par_list, int mode, const char* overlay, m2p_ctx** out)
   }
   (void)errctx;
   if (!c) return bad(rc ? rc : M2P_EINVAL, "invalid grid or inflow table");
+  if (rc && !c->s) {   // an error-only context from m2p_open_grid: hand its message on
+    *out = c;
+    return rc;
+  }
   c->xll = dem.xll;
   c->yll = dem.yll;
   *out = c;
@@ -610,15 +673,17 @@ int m2p_step(m2p_ctx* c, int nsteps) {
 int m2p_run_to(m2p_ctx* c, double t_end, int max_steps, int* steps_done) {
   if (!c || !c->s) return M2P_ESTATE;
   IN_BATCH(c);
-  if (!(t_end > 0) || max_steps < 0) return fail(c, M2P_EINVAL, "t_end must be > 0, max_steps >= 0");
+  if (!(t_end > 0) || !std::isfinite(t_end) || max_steps < 0)
+    return fail(c, M2P_EINVAL, "t_end must be finite and > 0, max_steps >= 0");
   DevState st{};
   HIPCHK(c, hipMemcpyAsync(&st, c->dSt, sizeof st, hipMemcpyDeviceToHost, c->s));
   HIPCHK(c, hipStreamSynchronize(c->s));
   const int nt0 = st.nt;
   if (st.time < t_end && max_steps > 0) {
-    st.t_end = t_end;
-    st.halt_at = (max_steps > INT_MAX - nt0) ? INT_MAX : nt0 + max_steps;
-    st.skip = 0;
+    // the call's own end time and budget; the halting rules of m2p_set_stop stay in force
+    // beside them and are left as they are
+    st.call_t_end = t_end;
+    st.call_halt = (max_steps > INT_MAX - nt0) ? INT_MAX : nt0 + max_steps;
     HIPCHK(c, hipMemcpyAsync(c->dSt, &st, sizeof st, hipMemcpyHostToDevice, c->s));
     // chunks of steps; the kernels of a halted scenario return at once, so a chunk may run
     // past the end
@@ -630,11 +695,12 @@ int m2p_run_to(m2p_ctx* c, double t_end, int max_steps, int* steps_done) {
       HIPCHK(c, hipGetLastError());
       HIPCHK(c, hipMemcpyAsync(&st, c->dSt, sizeof st, hipMemcpyDeviceToHost, c->s));
       HIPCHK(c, hipStreamSynchronize(c->s));
-      if (st.skip || st.nt >= st.halt_at) break;
+      if (st.skip || st.nt >= st.halt_at || st.nt >= st.call_halt) break;
     }
-    st.t_end = -1.0;
-    st.halt_at = INT_MAX;
-    st.skip = 0;
+    st.call_t_end = -1.0;
+    st.call_halt = INT_MAX;
+    // halted by the call's rules only: running again is allowed
+    if (st.skip && st.nt < st.halt_at) st.skip = 0;
     HIPCHK(c, hipMemcpyAsync(c->dSt, &st, sizeof st, hipMemcpyHostToDevice, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
   }
@@ -749,6 +815,12 @@ int m2p_set_state(m2p_ctx* c, const double* u) {
   IN_BATCH(c);
   if (!u) return fail(c, M2P_EINVAL, "null state");
   if (int rc = upload_state(c, c->dU0, u)) return rc;
+  {
+    // a new state: no non-finite value has been met in it yet
+    const int nf[2] = {0, -1};
+    HIPCHK(c, hipMemcpyAsync((char*)c->dSt + offsetof(DevState, nonfinite), nf, sizeof nf,
+                             hipMemcpyHostToDevice, c->s));
+  }
   HIPCHK(c, hipMemcpyAsync(c->dU1, c->dU0, 7 * (size_t)c->P * c->es, hipMemcpyDeviceToDevice, c->s));
   if (c->dLIM) HIPCHK(c, hipMemsetAsync(c->dLIM, 0, (size_t)c->P, c->s));
   zero_ghost_momenta(c, c->s);
@@ -915,6 +987,17 @@ int m2p_status(m2p_ctx* c, double* t, int* nt, int* halted) {
   return 0;
 }
 
+int m2p_nonfinite(m2p_ctx* c, int* flag, int* first_nt) {
+  if (!c || !c->s) return M2P_ESTATE;
+  hipStream_t s = strm(c);
+  DevState st{};
+  HIPCHK(c, hipMemcpyAsync(&st, c->dSt, sizeof st, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  if (flag) *flag = st.nonfinite;
+  if (first_nt) *first_nt = st.nonfinite ? st.nonfinite_nt : -1;
+  return 0;
+}
+
 int m2p_footprint(int mode, int NX, int NY, int precision, int nsamp, int nloc,
                   long long* steady, long long* peak) {
   if (NX < 2 || NY < 2 || (precision != M2P_FP64 && precision != M2P_FP32)) return M2P_EINVAL;
@@ -955,7 +1038,9 @@ struct CkptHeader {
   int32_t version, mode, NX, NY, prec, ntf;
   int64_t P;
   uint32_t devstate_bytes, lim_bytes_per_cell;
+  uint64_t scen_hash;   // scenario_hash of the context that wrote it
 };
+constexpr int32_t kCkptVersion = 2;
 
 }  // namespace
 
@@ -973,7 +1058,7 @@ int m2p_checkpoint(m2p_ctx* c, const char* path) {
   HIPCHK(c, hipStreamSynchronize(s));
   CkptHeader h{};
   std::memcpy(h.magic, kMagic, 8);
-  h.version = 1;
+  h.version = kCkptVersion;
   h.mode = c->mode;
   h.NX = c->NX;
   h.NY = c->NY;
@@ -982,6 +1067,7 @@ int m2p_checkpoint(m2p_ctx* c, const char* path) {
   h.P = c->P;
   h.devstate_bytes = sizeof(DevState);
   h.lim_bytes_per_cell = c->dLIM ? 1 : 0;
+  h.scen_hash = c->scen_hash;
   std::vector<double> ot(c->outtime);
   ot.resize(c->tf.size() + 1, 0.0);
   const std::string tmp = std::string(path) + ".tmp";
@@ -1019,12 +1105,14 @@ int m2p_restore(m2p_ctx* c, const char* path) {
   DevState st{};
   const size_t P = (size_t)c->P, nst = 7 * P * c->es;
   std::string why;
-  if (!get(&h, sizeof h) || std::memcmp(h.magic, kMagic, 8) != 0 || h.version != 1)
+  if (!get(&h, sizeof h) || std::memcmp(h.magic, kMagic, 8) != 0 || h.version != kCkptVersion)
     why = "not a checkpoint of this format";
   else if (h.mode != c->mode || h.NX != c->NX || h.NY != c->NY || h.prec != c->prec ||
            h.P != c->P || h.ntf != (int32_t)c->tf.size() || h.devstate_bytes != sizeof(DevState) ||
            h.lim_bytes_per_cell != (c->dLIM ? 1u : 0u))
     why = "checkpoint does not match this scenario (mode, grid, precision or output times)";
+  else if (h.scen_hash != c->scen_hash)
+    why = "checkpoint does not match this scenario (topography, parameters or inflow tables)";
   std::vector<double> ot(c->tf.size() + 1);
   std::vector<char> u0(nst), u1(nst), lim(c->dLIM ? P : 0);
   if (why.empty() && !(get(&st, sizeof st) && get(ot.data(), ot.size() * sizeof(double)) &&

@@ -30,6 +30,14 @@ struct DevState {
   double v2;          // squared max flow speed of the corrector of step v2_nt
   int moved;          // the flow speed has reached the stop threshold at least once
   int v2_nt;          // step whose corrector v2 comes from (-1: none)
+  // Halting rules of one m2p_run_to call, kept apart from the caller's (t_end, halt_at),
+  // which stay as they were when the call returns.
+  double call_t_end;  // > 0: end time of the call in progress
+  int call_halt;      // step count at which the call's budget is spent (INT_MAX: none)
+  // A CFL pass met a non-finite wave speed on a face (the max for dt ignores it, as the
+  // reference's fmaxf does): the state after nonfinite_nt steps holds a NaN or an Inf.
+  int nonfinite;
+  int nonfinite_nt;   // -1: never
 };
 
 // Dry-tile skipping.  A stage or CFL tile whose every input cell is zero in every
@@ -93,9 +101,11 @@ struct TilePlan {
 inline int plan_blocks(int tiles) { return 8 * ((tiles + 7) / 8 > 0 ? (tiles + 7) / 8 : 1); }
 
 // A scenario is halted for the kernels that run before the dt kernel of a step
-// (inflow, ghost fill, CFL) once it has done halt_at steps, and for those after it once
-// the dt kernel has set `skip`.
-__device__ __forceinline__ bool halted_pre(const DevState* st) { return st->nt >= st->halt_at; }
+// (inflow, ghost fill, CFL) once it has done halt_at (or call_halt) steps, and for those
+// after it once the dt kernel has set `skip`.
+__device__ __forceinline__ bool halted_pre(const DevState* st) {
+  return st->nt >= st->halt_at || st->nt >= st->call_halt;
+}
 __device__ __forceinline__ bool halted_post(const DevState* st) { return st->skip != 0; }
 
 struct DevParams {

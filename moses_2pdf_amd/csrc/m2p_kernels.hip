@@ -225,12 +225,12 @@ __global__ void __launch_bounds__(256) k_cfl(Grid g, Phys ph, const T* __restric
   double m = 0.0;
   if ((int)blockIdx.x >= nbb) {   // ring faces
     const int t = ((int)blockIdx.x - nbb) * 256 + threadIdx.y * 64 + threadIdx.x;
-    block_max_store(cfl_ring_face(U, M, g, ph, t, dm), part + blockIdx.x);
+    block_cfl_store(cfl_ring_face(U, M, g, ph, t, dm), part + blockIdx.x);
     return;
   }
   if (dm.nseg == 0) {
-    for (int t = blockIdx.x; t < nbox; t += nbb) m = fmax(m, cfl_box(U, M, g, ph, t % nbx, t / nbx));
-    block_max_store(m, part + blockIdx.x);
+    for (int t = blockIdx.x; t < nbox; t += nbb) m = cfl_join(m, cfl_box(U, M, g, ph, t % nbx, t / nbx));
+    block_cfl_store(m, part + blockIdx.x);
     return;
   }
   // Dry-tile skipping: the dry tests of up to 8 boxes of this block in one round of loads
@@ -255,10 +255,10 @@ __global__ void __launch_bounds__(256) k_cfl(Grid g, Phys ph, const T* __restric
     for (int k = 0; k < 8; k++) {
       const int t = t0 + k * nbb;
       if (t >= nbox) break;
-      m = fmax(m, wetk[k] ? cfl_box(U, M, g, ph, t % nbx, t / nbx) : fl);
+      m = cfl_join(m, wetk[k] ? cfl_box(U, M, g, ph, t % nbx, t / nbx) : fl);
     }
   }
-  block_max_store(m, part + blockIdx.x);
+  block_cfl_store(m, part + blockIdx.x);
 }
 
 // One block: the max over the CFL partials [0, ncfl), then the dt control in thread 0.
@@ -272,7 +272,7 @@ __global__ void __launch_bounds__(256) k_dt(DevParams dp, DevState* __restrict__
   __syncthreads();
   if (halted) return;
   if (plan_cnt && threadIdx.x < PLAN_COUNTERS) plan_cnt[threadIdx.x] = 0;
-  const double mx = block_max_of(part, ncfl);
+  const double mx = block_cfl_of(part, ncfl);
   if (threadIdx.x == 0) dt_body(dp, st, tf, itime, mx);
 }
 

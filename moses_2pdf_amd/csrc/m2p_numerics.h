@@ -229,6 +229,7 @@ struct FaceSpeed {
   T qL, qR, qFL, qFR;     // normal contravariant speeds (q_xi for x, q_et for y)
   T ap, apF;              // apEW/apSN, apFEW/apFSN
   T emS, emF;
+  bool finite;            // all four one-sided speeds finite (the maxima above drop a NaN)
 };
 
 template <typename T>
@@ -266,6 +267,7 @@ __device__ __forceinline__ void face_speed(int dir, T L[6], T R[6], const FaceMe
   const T apL = fabs(o.qL) + an * qsqrt(NpL * Drtio1);
   const T apFR = fabs(o.qFR) + an * qsqrt(qdiv(NpR, T(1.0) - o.phiR + h0));
   const T apFL = fabs(o.qFL) + an * qsqrt(qdiv(NpL, T(1.0) - o.phiL + h0));
+  o.finite = isfinite(apR) && isfinite(apL) && isfinite(apFR) && isfinite(apFL);
   // Flux5Kernel L/K:1762-1766 (max(E, W) / max(N, S))
   o.ap = fmax(apR, apL);
   o.apF = fmax(apFR, apFL);

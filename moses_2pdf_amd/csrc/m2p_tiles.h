@@ -166,6 +166,14 @@ __device__ __forceinline__ double wave_max(double v) {
 // which the dt kernel reduces.  k_cfl covers the faces of the interior box [3, ny+2] x [3, nx+2]
 // (InterfacesKernel states); its ring blocks the rest of [1, N-3]^2, where Interfaces2Kernel
 // overrides the states.  Speeds are formed in T; the quotient by dx and the max in FP64.
+//
+// CFL values carry a non-finite mark in their sign.  A face value is at least the floor
+// 1e-10/dx > 0; it is negated when one of the face's four one-sided speeds (apE, apW, apFE,
+// apFW) is not finite.  The reference's speed floor and maxima drop a NaN there without a
+// trace (fmax, fmaxf), so this is the only place it can still be seen.  cfl_join keeps
+// the larger magnitude, negative if either value is, so the magnitude of a reduction is
+// the same fmax the dt control has always used, and its sign says whether any face met a
+// non-finite state.  Slots with no faces hold +0.
 template <typename T>
 __device__ __forceinline__ double face_cfl(const T* __restrict__ U, const T* __restrict__ M,
                                            const Grid& g, const Phys& ph, int dir, int r, int c,
@@ -180,15 +188,29 @@ __device__ __forceinline__ double face_cfl(const T* __restrict__ U, const T* __r
   FaceSpeed<T> s;
   face_speed(dir, L, R, f, ph, s);
   const double d = dir == 0 ? g.dx : g.dy;
-  return fmax((double)s.emS / d, (double)s.emF / d);
+  const double v = fmax((double)s.emS / d, (double)s.emF / d);
+  return s.finite ? v : -v;
+}
+
+__device__ __forceinline__ double cfl_join(double a, double b) {
+  const double m = fmax(fabs(a), fabs(b));
+  return (a < 0.0 || b < 0.0) ? -m : m;
+}
+
+__device__ __forceinline__ double wave_cfl(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = cfl_join(v, __shfl_xor(v, off, 64));
+  return v;
 }
 
 // Max over the block of a per-thread value (fmax: NaN ignored unless all are NaN), written
 // by thread 0 to *out.  Every thread of the block must call it; it may be called again
 // right away (it ends with a barrier).  Each block writes its own slot and the dt kernel
-// reduces the slots, so no two blocks contend for one address.
+// reduces the slots, so no two blocks contend for one address.  JOIN: combine with
+// cfl_join (CFL values) instead of fmax.
+template <bool JOIN = false>
 __device__ __forceinline__ void block_max_store(double m, double* out) {
-  m = wave_max(m);
+  m = JOIN ? wave_cfl(m) : wave_max(m);
   __shared__ double part[16];
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
   const int nw = (blockDim.x * blockDim.y + 63) / 64;
@@ -196,35 +218,45 @@ __device__ __forceinline__ void block_max_store(double m, double* out) {
   __syncthreads();
   if (tid == 0) {
     double b = part[0];
-    for (int w = 1; w < nw; w++) b = fmax(b, part[w]);
+    for (int w = 1; w < nw; w++) b = JOIN ? cfl_join(b, part[w]) : fmax(b, part[w]);
     *out = b;
   }
   __syncthreads();
 }
 
+__device__ __forceinline__ void block_cfl_store(double m, double* out) {
+  block_max_store<true>(m, out);
+}
+
 // fmax of p[0..n) by all threads of a 1-D block; every thread gets the result.  The max
-// of a set does not depend on the order it is formed in.
+// of a set does not depend on the order it is formed in.  JOIN: cfl_join instead.
+template <bool JOIN = false>
 __device__ __forceinline__ double block_max_of(const double* __restrict__ p, int n) {
+  auto op = [](double a, double b) { return JOIN ? cfl_join(a, b) : fmax(a, b); };
   double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;   // four independent chains of loads
   const int s = blockDim.x;
   int i = threadIdx.x;
   for (; i + 3 * s < n; i += 4 * s) {
-    m0 = fmax(m0, p[i]);
-    m1 = fmax(m1, p[i + s]);
-    m2 = fmax(m2, p[i + 2 * s]);
-    m3 = fmax(m3, p[i + 3 * s]);
+    m0 = op(m0, p[i]);
+    m1 = op(m1, p[i + s]);
+    m2 = op(m2, p[i + 2 * s]);
+    m3 = op(m3, p[i + 3 * s]);
   }
-  for (; i < n; i += s) m0 = fmax(m0, p[i]);
-  double m = fmax(fmax(m0, m1), fmax(m2, m3));
-  m = wave_max(m);
+  for (; i < n; i += s) m0 = op(m0, p[i]);
+  double m = op(op(m0, m1), op(m2, m3));
+  m = JOIN ? wave_cfl(m) : wave_max(m);
   __shared__ double part[16];
   __syncthreads();
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
   __syncthreads();
   double b = part[0];
-  for (int w = 1; w < (int)((blockDim.x + 63) / 64); w++) b = fmax(b, part[w]);
+  for (int w = 1; w < (int)((blockDim.x + 63) / 64); w++) b = op(b, part[w]);
   __syncthreads();
   return b;
+}
+
+__device__ __forceinline__ double block_cfl_of(const double* __restrict__ p, int n) {
+  return block_max_of<true>(p, n);
 }
 
 // The dry test of a region [ra, rb] x [ca, cb] of U (clipped to the padded grid) as a list
@@ -332,7 +364,7 @@ __device__ __forceinline__ double cfl_box(const T* __restrict__ U, const T* __re
   double m = 0.0;
   if (r <= g.ny + MD - 1 && c <= g.nx + MD - 1) {
     m = face_cfl(U, M, g, ph, 0, r, c, false);
-    m = fmax(m, face_cfl(U, M, g, ph, 1, r, c, false));
+    m = cfl_join(m, face_cfl(U, M, g, ph, 1, r, c, false));
   }
   return m;
 }
@@ -404,14 +436,16 @@ __device__ __forceinline__ double cfl_ring_face(const T* __restrict__ U, const T
 
 // CFL3Kernel + the host's dt control + CFL4Kernel (L/K:2006-2036, L/R:978-991; inflow
 // sample advance I/R:1078-1081), on the device.  mx is the max wave speed over the CFL
-// faces; it is rounded to float as the reference's fmaxf reduction does (L/K:1988, 1996).
+// faces, negative when some face met a non-finite state (face_cfl); its magnitude is
+// rounded to float as the reference's fmaxf reduction does (L/K:1988, 1996).
 // Also the halting rules, which have no reference counterpart: a step budget (halt_at),
 // an end time (t_end: the step that reaches it is shortened to end on it) and a stop speed
 // (vstop2: once the squared max flow speed v2 of a corrector has reached it and a later one
-// falls below it, the next step is the last).
+// falls below it, the next step is the last); and the same budget and end time for the
+// duration of one m2p_run_to call (call_halt, call_t_end).
 __device__ __forceinline__ bool dt_halted(DevState* __restrict__ st) {
   if (st->skip) return true;
-  if (st->nt >= st->halt_at) {
+  if (st->nt >= st->halt_at || st->nt >= st->call_halt) {
     st->skip = 1;
     return true;
   }
@@ -421,6 +455,11 @@ __device__ __forceinline__ bool dt_halted(DevState* __restrict__ st) {
 __device__ __forceinline__ void dt_body(const DevParams& dp, DevState* __restrict__ st,
                                         const double* __restrict__ tf,
                                         const double* __restrict__ itime, double mx) {
+  if (mx < 0.0) {
+    if (!st->nonfinite) st->nonfinite_nt = st->nt;
+    st->nonfinite = 1;
+    mx = -mx;
+  }
   if (st->vstop2 > 0.0 && st->v2_nt == st->nt) {
     const double v2 = st->v2;
     if (v2 >= st->vstop2)
@@ -442,9 +481,16 @@ __device__ __forceinline__ void dt_body(const DevParams& dp, DevState* __restric
   }
   if (dp.nsamp > 0 && T >= itime[st->inflowCount] && st->inflowCount < (dp.nsamp - 1))
     st->inflowCount = st->inflowCount + 1;
-  if (st->t_end > 0.0 && (T + dtval) >= st->t_end) {
-    dtval = st->t_end - T;
-    if (st->halt_at > st->nt + 1) st->halt_at = st->nt + 1;
+  // the nearer of the two end times shortens the step; each rule whose end is reached
+  // halts the scenario after it
+  const bool own = st->t_end > 0.0, call = st->call_t_end > 0.0;
+  if (own || call) {
+    const double te = (own && call) ? fmin(st->t_end, st->call_t_end) : own ? st->t_end : st->call_t_end;
+    if ((T + dtval) >= te) {
+      dtval = te - T;
+      if (own && st->t_end == te && st->halt_at > st->nt + 1) st->halt_at = st->nt + 1;
+      if (call && st->call_t_end == te && st->call_halt > st->nt + 1) st->call_halt = st->nt + 1;
+    }
   }
   st->dtval = dtval;
   st->dt_pred = dtval / (2.0 - 0);

@@ -1,8 +1,10 @@
 """Batched-ensemble runs: the plain one-at-a-time path, batched runs in either precision,
 and the FP32-versus-FP64 final-depth error at equal simulated time.
 
-Every run returns its wall time and, per member, the final flow depth and simulated time,
-so the caller can price scenarios/hour and compare precisions.
+Every run returns its wall time and, per member, the final state and simulated time, so
+the caller can price scenarios/hour and compare precisions, and the member's non-finite
+flag (m2p_nonfinite), so that runs which blew up are reported as failed rather than
+compared.
 """
 from __future__ import annotations
 
@@ -26,9 +28,11 @@ def population(count: int = 256, seed: int = 4):
 
 
 def run_plain(scs, steps, precision="fp64", keep=True, dry_skip=True):
-    """Each scenario alone, one after another, `steps` m2p_step steps each."""
+    """Each scenario alone, one after another, `steps` m2p_step steps each.  Returns the
+    wall time, the (state, time) of each member (if keep) and each member's non-finite
+    flag."""
     t0 = time.perf_counter()
-    out = []
+    out, bad = [], []
     for k, sc in enumerate(scs):
         with Simulation(sc, precision=precision, dry_skip=dry_skip) as s:
             s.step(steps)
@@ -37,9 +41,10 @@ def run_plain(scs, steps, precision="fp64", keep=True, dry_skip=True):
                 out.append((s.state(), s.time_steps()[0]))
             else:
                 s.time_steps()
+            bad.append(s.nonfinite()[0])
         if (k + 1) % 32 == 0:
             log(f"plain {precision}: {k + 1}/{len(scs)} scenarios, {time.perf_counter() - t0:.1f} s")
-    return time.perf_counter() - t0, out
+    return time.perf_counter() - t0, out, bad
 
 
 def run_batch(scs, steps, precision, tune_steps=3, t_end=None, keep=True, graph_steps=None,
@@ -86,11 +91,12 @@ def run_batch(scs, steps, precision, tune_steps=3, t_end=None, keep=True, graph_
     t_step = time.perf_counter() - t1
     out = [(s.state(), s.time_steps()[0]) for s in sims] if keep else []
     nts = [s.status()[1] for s in sims]
+    bad = [s.nonfinite()[0] for s in sims]
     b.close()
     for s in sims:
         s.close()
     return {"wall_s": t_step + t_open + t_tune, "step_s": t_step, "open_s": t_open, "tune_s": t_tune,
-            "tune_report": report, "out": out, "steps": nts}
+            "tune_report": report, "out": out, "steps": nts, "nonfinite": bad}
 
 
 def errors(u32s, u64s):

@@ -5,8 +5,10 @@ keeps one batch of a fixed precision on the device.  Between ticks of `tick_step
 steps it admits waiting requests in arrival order while the device footprint of the
 scenarios it holds, plus the newcomer's set-up peak (m2p_footprint), stays within
 `hbm_budget`.  It retires scenarios whose device-side stop rule (a max flow speed below
-`vstop` after motion, or `max_steps` steps) has halted them.  A request's latency runs
-from its arrival to the moment its result (final state) is on the host.
+`vstop` after motion, or `max_steps` steps) has halted them, and, as failed, scenarios
+whose state has gone non-finite (m2p_nonfinite; no result state is handed back for them).
+A request's latency runs from its arrival to the moment its result (final state) is on
+the host.
 
 Each scenario's halting rule is evaluated on the device from its own state, and the
 batched kernels compute every member exactly as if it ran alone, so a result does not
@@ -43,6 +45,8 @@ class Result:
     steps: int
     time: float                # simulated time reached (scaled)
     state: Optional[np.ndarray] = None
+    failed: bool = False       # the state went non-finite (no result)
+    nonfinite_nt: int = -1     # step count of the first non-finite state seen
 
     @property
     def latency(self) -> float:
@@ -138,9 +142,10 @@ class Server:
             for rid in list(active):
                 sim, req, start, steady = active[rid]
                 t, nt, halted = sim.status()
-                if halted:
-                    st = sim.state() if self.keep_state else None
-                    results.append(Result(rid, req.arrival, start, now(), nt, t, st))
+                bad, bad_nt = sim.nonfinite()
+                if halted or bad:
+                    st = sim.state() if (self.keep_state and not bad) else None
+                    results.append(Result(rid, req.arrival, start, now(), nt, t, st, bad, bad_nt))
                     b.remove(sim)
                     sim.close()
                     held -= steady
@@ -169,10 +174,11 @@ def serve_alone(requests: List[Request], precision: str = "fp64", vstop: float =
             while True:
                 sim.step(tick_steps)
                 t, nt, halted = sim.status()
-                if halted:
+                bad, bad_nt = sim.nonfinite()
+                if halted or bad:
                     break
-            st = sim.state() if keep_state else None
-        results.append(Result(req.rid, req.arrival, start, now(), nt, t, st))
+            st = sim.state() if (keep_state and not bad) else None
+        results.append(Result(req.rid, req.arrival, start, now(), nt, t, st, bad, bad_nt))
     return results
 
 
@@ -182,4 +188,5 @@ def summary(results: List[Result], wall: float, max_steps: int = 8000) -> dict:
             "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
             "latency_mean_s": float(lat.mean()),
             "steps_mean": float(np.mean([r.steps for r in results])),
-            "stopped_by_speed": int(sum(1 for r in results if r.steps < max_steps))}
+            "stopped_by_speed": int(sum(1 for r in results if r.steps < max_steps and not r.failed)),
+            "failed_nonfinite": int(sum(1 for r in results if r.failed))}

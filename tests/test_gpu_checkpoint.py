@@ -77,6 +77,20 @@ def test_restore_rejects_mismatch_and_corruption(tmp_path):
     with Simulation(S.landslide_plane(n=40), precision="fp32") as b:
         with pytest.raises(M2PError, match="does not match"):
             b.restore(ck)
+    # same grid size, but another DEM, another friction angle, another CFL number
+    other_dem = S.landslide_plane(n=40)
+    other_dem.topo = other_dem.topo + 1.0
+    other_par = S.landslide_plane(n=40)
+    other_par.delta0 += 1.0
+    other_cfl = S.landslide_plane(n=40)
+    other_cfl.cfl = 0.2
+    for sc in (other_dem, other_par, other_cfl):
+        with Simulation(sc) as b:
+            with pytest.raises(M2PError, match="topography, parameters or inflow"):
+                b.restore(ck)
+    with Simulation(S.landslide_plane(n=40)) as b:   # the same scenario restores
+        b.restore(ck)
+        assert b.time_steps()[1] == 3
     data = bytearray(open(ck, "rb").read())
     data[len(data) // 2] ^= 0x40
     bad = str(tmp_path / "bad.m2p")
