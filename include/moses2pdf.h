@@ -154,10 +154,30 @@ int m2p_get_field(m2p_ctx* ctx, const char* name, double* out);
 int m2p_profile(m2p_ctx* ctx, int enable);
 int m2p_profile_get(m2p_ctx* ctx, const char* kernel, double* ms_per_launch, int* launches);
 
+/* The context's last error message; with ctx == NULL, the calling thread's last error of
+ * a call that takes no context (m2p_mass_noalloc). */
 const char* m2p_last_error(const m2p_ctx* ctx);
 
 /* Closes the context (removing it from its batch first) and frees its device memory. */
 void m2p_close(m2p_ctx* ctx);
+
+/* ---- Allocation-free variant (no reference counterpart) ----------------------------- */
+
+/* m2p_mass for a state held in the caller's own device memory, allocating nothing: no
+ * counterpart in the reference; it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * u: device pointer to a state (7 planes of (nx+6)*(ny+6) values, layout above), double
+ * for precision M2P_FP64 and float for M2P_FP32; detmin: device pointer to the Detmin
+ * plane ((nx+6)*(ny+6) doubles, as m2p_get_field "Detmin" gives it); out: device pointer
+ * to 2 doubles that receive the solid and fluid totals, equal bit for bit to what
+ * m2p_mass returns for that state; stream: a hipStream_t, or NULL for the null stream.
+ * The call enqueues its two kernels on `stream` and returns without waiting (so it can be
+ * captured into a graph).  Its block partials go to one fixed 16 KiB device array per
+ * GPU (held until the process ends), whatever the grid size: calls that overlap on one
+ * GPU race.  Rejects null pointers, a bad precision and grids outside 1 <= nx, ny,
+ * nx*ny <= INT_MAX with M2P_EINVAL (message: m2p_last_error(NULL)). */
+int m2p_mass_noalloc(const void* u, int precision, const double* detmin, int nx, int ny,
+                     double* out, void* stream);
 
 /* ---- Checkpoint / resume (no reference counterpart: its text outputs are lossy) ---- */
 

@@ -18,6 +18,12 @@ hipStream_t m2p_batch_stream(m2p_batch* b);
 // the batch must re-read its members' settings before its next step
 void m2p_batch_mark_dirty(m2p_batch* b);
 
+namespace m2p {
+// The message of the calling thread's last failed call that takes no context
+// (m2p_mass_noalloc), or "".
+const char* context_free_error();
+}  // namespace m2p
+
 using m2p::DevParams;
 using m2p::DevState;
 

@@ -845,9 +845,9 @@ int m2p_mass(m2p_ctx* c, double* solid, double* fluid) {
   std::vector<double> part(4 * (size_t)nb);
   hipStream_t s = strm(c);
   if (c->prec == M2P_FP32)
-    m2p::launch_mass(c->g, (const float*)c->dU0, c->dM, c->dMass, s);
+    m2p::launch_mass(c->g, (const float*)c->dU0, c->dM + m2p::MC_DET * c->P, c->dMass, s);
   else
-    m2p::launch_mass(c->g, (const double*)c->dU0, c->dM, c->dMass, s);
+    m2p::launch_mass(c->g, (const double*)c->dU0, c->dM + m2p::MC_DET * c->P, c->dMass, s);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(part.data(), c->dMass, part.size() * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipStreamSynchronize(s));
@@ -1135,7 +1135,11 @@ int m2p_restore(m2p_ctx* c, const char* path) {
   return 0;
 }
 
-const char* m2p_last_error(const m2p_ctx* c) { return c ? c->err.c_str() : "null context"; }
+const char* m2p_last_error(const m2p_ctx* c) {
+  if (c) return c->err.c_str();
+  const char* e = m2p::context_free_error();
+  return *e ? e : "null context";
+}
 
 void m2p_close(m2p_ctx* c) {
   if (!c) return;

@@ -159,9 +159,15 @@ template <typename T>
 void launch_stage(int io, const Grid& g, const Phys& ph, const T* Uin, const T* Uzero, T* Uout,
                   T* ONE, const T* M, unsigned char* LIM, DevState* st, double* part,
                   const DryMap& dm, const TilePlan& pl, hipStream_t s);
+// Compensated mass totals: each of mass_blocks() blocks writes 4 partials (solid hi, lo,
+// fluid hi, lo) of sum u0*Det and u3*Det over its share of the interior cells into
+// out[4 b .. 4 b + 3], every slot every call.  det is the Detmin plane.
 int mass_blocks();
 template <typename T>
-void launch_mass(const Grid& g, const T* U, const double* M, double* out, hipStream_t s);
+void launch_mass(const Grid& g, const T* U, const double* det, double* out, hipStream_t s);
+// The double-double sum of the nb blocks' partials in block order, out[0] = solid,
+// out[1] = fluid: the same operations, in the same order, as m2p_mass's host sum.
+void launch_mass_sum(const double* part, int nb, double* out, hipStream_t s);
 hipError_t probe_kernels();
 
 }  // namespace m2p

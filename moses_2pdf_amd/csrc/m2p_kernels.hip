@@ -319,7 +319,7 @@ __device__ __forceinline__ void dd_add(double& hi, double& lo, double x) {
 }
 
 template <typename T>
-__global__ void k_mass(Grid g, const T* __restrict__ U, const double* __restrict__ M,
+__global__ void k_mass(Grid g, const T* __restrict__ U, const double* __restrict__ det,
                        double* __restrict__ out) {
   const int tid = threadIdx.x;
   double sh = 0, sl = 0, fh = 0, fl = 0;
@@ -328,7 +328,7 @@ __global__ void k_mass(Grid g, const T* __restrict__ U, const double* __restrict
        q += (long long)gridDim.x * blockDim.x) {
     const int r = (int)(q / g.nx) + MD, c = (int)(q % g.nx) + MD;
     const long long i = idx(g, r, c);
-    const double d = M[MC_DET * g.P + i];
+    const double d = det[i];
     dd_add(sh, sl, (double)U[i] * d);
     dd_add(fh, fl, (double)U[3 * g.P + i] * d);
   }
@@ -530,8 +530,33 @@ size_t stage_partials_offset(const Grid& g) {
 int mass_blocks() { return 512; }
 
 template <typename T>
-void launch_mass(const Grid& g, const T* U, const double* M, double* out, hipStream_t s) {
-  k_mass<T><<<mass_blocks(), 256, 0, s>>>(g, U, M, out);
+void launch_mass(const Grid& g, const T* U, const double* det, double* out, hipStream_t s) {
+  k_mass<T><<<mass_blocks(), 256, 0, s>>>(g, U, det, out);
+}
+
+// One thread: hi + lo accumulation of the partials in block order (a two-sum of the
+// running high part and the partial, the low parts added, then renormalised).
+__global__ void k_mass_sum(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  auto add = [](double& hi, double& lo, double x) {
+    double s = hi + x, bb = s - hi, e = (hi - (s - bb)) + (x - bb);
+    e += lo;
+    hi = s + e;
+    lo = e - (hi - s);
+  };
+  double a = 0, al = 0, b = 0, bl = 0;
+  for (int i = 0; i < nb; i++) {
+    add(a, al, part[4 * i + 0]);
+    add(a, al, part[4 * i + 1]);
+    add(b, bl, part[4 * i + 2]);
+    add(b, bl, part[4 * i + 3]);
+  }
+  out[0] = a + al;
+  out[1] = b + bl;
+}
+
+void launch_mass_sum(const double* part, int nb, double* out, hipStream_t s) {
+  k_mass_sum<<<1, 64, 0, s>>>(part, nb, out);
 }
 
 #define M2P_INSTANTIATE(T)                                                                    \

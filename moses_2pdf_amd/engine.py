@@ -28,7 +28,7 @@ EXPORTS = ["m2p_default_params", "m2p_open", "m2p_open_grid", "m2p_run", "m2p_st
            "m2p_get_state", "m2p_set_state", "m2p_dims", "m2p_mass", "m2p_scalars",
            "m2p_get_field", "m2p_profile", "m2p_profile_get", "m2p_last_error", "m2p_close",
            "m2p_set_stop", "m2p_status", "m2p_footprint", "m2p_checkpoint", "m2p_restore",
-           "m2p_run_steps", "m2p_set_dry_skip", "m2p_nonfinite",
+           "m2p_run_steps", "m2p_set_dry_skip", "m2p_nonfinite", "m2p_mass_noalloc",
            "m2p_batch_create", "m2p_batch_add", "m2p_batch_remove", "m2p_batch_size",
            "m2p_batch_step", "m2p_batch_sync", "m2p_batch_tune", "m2p_batch_set", "m2p_batch_get",
            "m2p_batch_last_error", "m2p_batch_destroy"]
@@ -87,6 +87,7 @@ def lib():
     L.m2p_run_steps.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, ip, dp]
     L.m2p_status.argtypes = [vp, dp, ip, ip]
     L.m2p_nonfinite.argtypes = [vp, ip, ip]
+    L.m2p_mass_noalloc.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, vp, vp]
     L.m2p_footprint.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 ctypes.c_int, llp, llp]
     L.m2p_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
