@@ -1,13 +1,18 @@
 #!/usr/bin/env python3
 """Throughput benchmark of the adNOC_2Pb time-step loop on one MI355X.
 
-Workload: landslide mode, plain FP64, 2048x2048 cells at 20 m on a seeded diamond-square
-DEM (Hurst 0.8, 1500 m relief, 3-cell box filter) with three seeded paraboloid releases
-(1e5..5e6 m^3, solid fraction 0.4..0.7).  One step = one full time step (CFL max
-reduction, predictor, corrector) over every interior cell; the state stays resident in
-HBM.  Prints one JSON line.
+Headline workload: landslide mode, plain FP64, 2048x2048 cells at 20 m on a seeded
+diamond-square DEM (Hurst 0.8, 1500 m relief, 3-cell box filter) with three seeded
+paraboloid releases (1e5..5e6 m^3, solid fraction 0.4..0.7).  One step = one full time
+step (CFL max reduction, predictor, corrector) over every interior cell; the state stays
+resident in HBM.  The same JSON line carries, beside it, the batched-ensemble workload
+(256 scenarios, 3000 steps: batched FP32 and FP64 against the plain FP64 path, and the
+FP32 error), the serving workload (64 Poisson requests, continuous batching against the
+plain path, p50/p99 latency) and the CPU baseline (the FP64 oracle in fixed windows).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 2048] [--no-cpu-baseline]
+                    [--no-ensemble] [--no-serving]
+    python bench.py --workload ensemble|serve|accuracy     (one workload on its own line)
 """
 from __future__ import annotations
 
@@ -49,33 +54,127 @@ def traffic_from_profiles(kernel: str, n: int, precision: str = "fp64"):
     return None
 
 
-def cpu_baseline(sc, steps: int):
-    """The FP64 oracle (C++ restatement of the reference kernels, OpenMP over the host's
-    threads; its results do not depend on the thread count) on the same scenario, timed
-    around `steps` steps only, with all threads and with one."""
+def valu_from_profiles(kernel: str, n: int, precision: str = "fp64"):
+    """(VALU issue share, wait share) of `kernel` at grid n from the committed rocprofv3 PMC
+    summary (profiles/valu_counters_<n>.txt lines '<precision> <kernel>: ... VALU issue share
+    A%, wait share B%'), or None."""
+    path = os.path.join(ROOT, "profiles", f"valu_counters_{n}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(f"{precision} {kernel}:") and "issue share" in line:
+            try:
+                a = float(line.split("VALU issue share")[1].split("%")[0])
+                b = float(line.split("wait share")[1].split("%")[0])
+                return a / 100.0, b / 100.0
+            except (IndexError, ValueError):
+                return None
+    return None
+
+
+def _cpu_info() -> dict:
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "cpus_available": avail}
+
+
+def _oracle_rate(sc, steps: int) -> tuple:
+    """Cell-updates/s of the FP64 oracle on `sc`, timed around `steps` steps after one
+    untimed step (first touch of its work arrays); (rate, seconds)."""
     from oracle import oracle as O
-    nthr = O.threads(int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
     o = O.Oracle(sc)
-    o.step(1)   # first touch of the work arrays
+    o.step(1)
     t0 = time.perf_counter()
     o.step(steps)
     dt = time.perf_counter() - t0
+    return sc.nx * sc.ny * steps / dt, dt
+
+
+def cpu_baseline(sc_c2, c2_steps: int = 20, c1_steps: int = 500, c3_steps: int = 20,
+                 serial_steps: int = 20) -> dict:
+    """The FP64 oracle (the C++ restatement of the reference kernels, -O2 -ffp-contract=off,
+    OpenMP over the host's threads; its results do not depend on the thread count) timed
+    around the step loop only, in fixed windows of the three plain configurations: the
+    2048^2 landslide (the headline workload, `value`), the 256^2 landslide on the 35-degree
+    plane (CFL 0.4) and the 1024 x 4096 inflow valley; and the 256^2 case on one thread."""
+    from moses_2pdf_amd import scenarios as S
+    from oracle import oracle as O
+    nthr = O.threads(int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
+    log = lambda *a: print("#", *a, file=sys.stderr, flush=True)
+    v2, d2 = _oracle_rate(sc_c2, c2_steps)
+    log(f"cpu baseline: 2048^2 landslide, {c2_steps} steps in {d2:.1f} s on {nthr} threads")
+    c1 = S.landslide_plane(n=256, phi=0.6)
+    c1.cfl, c1.rho_ratio = 0.4, 1000.0 / 2700.0
+    v1, d1 = _oracle_rate(c1, c1_steps)
+    log(f"cpu baseline: 256^2 plane, {c1_steps} steps in {d1:.1f} s")
+    c3 = S.inflow_valley()
+    v3, d3 = _oracle_rate(c3, c3_steps)
+    log(f"cpu baseline: 1024x4096 inflow valley, {c3_steps} steps in {d3:.1f} s")
     O.threads(1)
-    o1 = O.Oracle(sc)
-    o1.step(1)
-    t1 = time.perf_counter()
-    o1.step(1)
-    d1 = time.perf_counter() - t1
+    vs, ds = _oracle_rate(c1, serial_steps)
     O.threads(nthr)
-    return {"value": sc.nx * sc.ny * steps / dt, "unit": "cell-updates/s", "cores": nthr,
-            "kind": "port", "single_core_value": sc.nx * sc.ny / d1,
-            "sample": f"{steps} full step(s) of the same {sc.nx}x{sc.ny} landslide workload after one "
-                      f"untimed step, FP64 oracle (-O2 -ffp-contract=off, OpenMP) on {nthr} threads, "
-                      f"step loop only, {dt:.1f} s; one step on 1 thread took {d1:.1f} s"}
+    log(f"cpu baseline: 256^2 plane, {serial_steps} steps on 1 thread in {ds:.1f} s")
+    out = {"value": v2, "unit": "cell-updates/s", "cores": nthr, "kind": "port",
+           "sample": f"window of {c2_steps} full steps of the same {sc_c2.nx}x{sc_c2.ny} landslide "
+                     f"workload after one untimed step, FP64 oracle (-O2 -ffp-contract=off, "
+                     f"OpenMP) on {nthr} threads, step loop only, {d2:.1f} s"}
+    out.update(_cpu_info())
+    out["windows"] = {
+        "landslide_2048": {"steps": c2_steps, "value": v2, "seconds": d2, "threads": nthr},
+        "landslide_plane_256": {"steps": c1_steps, "of_steps": 2000, "value": v1, "seconds": d1,
+                                "threads": nthr},
+        "inflow_valley_1024x4096": {"steps": c3_steps, "of_steps": 10000, "value": v3, "seconds": d3,
+                                    "threads": nthr},
+        "landslide_plane_256_serial": {"steps": serial_steps, "value": vs, "seconds": ds, "threads": 1},
+    }
+    return out
 
 
-def ensemble(args) -> int:
-    """Batched FP32 ensemble against the plain FP64 path, 256 members of 128^2..1024^2."""
+def accuracy(args) -> int:
+    """The FP32-bulk variant against FP64 at equal simulated time on five scenarios (the
+    first 128^2, 256^2 and 1024^2 ensemble members and the reference's two datasets): L1
+    of the final flow depth after a short horizon (30 steps; 10 for the inflow case, whose
+    hydrograph sample switches on a time test) and after 1000 steps, the conditioning floor
+    (L1 of an FP64 run perturbed by 1e-7 relative) and the runout IoU at 1000 steps."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from util import df004, hsiaolin   # the reference's datasets, kept as test fixtures
+    from moses_2pdf_amd import scenarios as S
+    from moses_2pdf_amd.accuracy import fp32_vs_fp64
+    ms = S.ensemble_members(256)
+    first = lambda n: S.ensemble_scenario([m for m in ms if m.n == n][0])
+    cases = {"ensemble_member_128": lambda: first(128), "ensemble_member_256": lambda: first(256),
+             "ensemble_member_1024": lambda: first(1024), "hsiaolin": hsiaolin, "df004": df004}
+    rows = {}
+    for name, make in cases.items():
+        sc = make()
+        short = fp32_vs_fp64(sc, 10 if sc.mode == 1 else 30, floor=False)
+        long_ = fp32_vs_fp64(sc, 1000)
+        rows[name] = {"short_steps": short["steps"], "short_l1": short["l1"],
+                      "l1_1000": long_["l1"], "floor_l1_1000": long_["floor_l1"],
+                      "runout_iou_1000": long_["runout_iou"], "mass_rel_1000": long_["mass_rel"],
+                      "linf_1000": long_["linf"]}
+        print(f"# {name}: {rows[name]}", file=sys.stderr, flush=True)
+    print(json.dumps({"metric": "FP32 final-depth L1 vs FP64", "cases": rows}), flush=True)
+    return 0
+
+
+def ensemble_measure(args, dense_steps=None) -> dict:
+    """The 256-member batched ensemble: batched FP32 and FP64 (autotuned, graph-captured)
+    against the plain FP64 path (each member alone with m2p_step), all with dry-tile
+    skipping (unless --dense), and the FP32 final-depth error against FP64 at equal time.
+    dense_steps: also time every path that computes every tile, over the first dense_steps
+    of the steps (a dense step's cost does not depend on the state, so the full run's cost
+    is the per-step cost times the steps)."""
     from moses_2pdf_amd import ensemble as E
     import numpy as np
     t0 = time.perf_counter()
@@ -83,25 +182,25 @@ def ensemble(args) -> int:
     E.log(f"population of {len(scs)} built in {time.perf_counter() - t0:.1f} s")
     cells = sum(sc.nx * sc.ny for sc in scs)
     steps = args.ens_steps
-    # plain FP64 path: each scenario alone, one after another
     dry = not args.dense
-    t_plain, ref, _ = E.run_plain(scs, steps, "fp64", dry_skip=dry)
-    # batched FP64 and FP32 (autotuned, graph-captured)
+    n = len(scs)
+    per_h = lambda wall: n / wall * 3600.0
+    plain = E.run_plain(scs, steps, "fp64", dry_skip=dry)
+    ref = plain["out"]
     b64 = E.run_batch(scs, steps, "fp64", keep=False, dry_skip=dry)
     b32 = E.run_batch(scs, steps, "fp32", keep=False, dry_skip=dry)
     # FP32 error at equal time: each member stepped to its FP64 run's final time (members
     # whose FP64 run went non-finite are left out: the reference scheme itself fails there)
-    fin64 = [bool(np.isfinite(r[0]).all()) for r in ref]
-    keep = [k for k in range(len(scs)) if fin64[k]]
+    keep = [k for k in range(n) if not plain["nonfinite"][k] and np.isfinite(ref[k][0]).all()]
     # (a member whose FP32 dt collapses to its floor cannot reach the FP64 time within 4x the
     # steps: it is counted as stalled, not compared)
     acc = E.run_batch([scs[k] for k in keep], 4 * steps, "fp32", tune_steps=0,
                       t_end=[ref[k][1] for k in keep])
-    fin32 = [bool(np.isfinite(o[0]).all()) for o in acc["out"]]
+    fin32 = [bool(np.isfinite(o[0]).all()) and not f for o, f in zip(acc["out"], acc["nonfinite"])]
     reached = [acc["out"][j][1] == ref[keep[j]][1] for j in range(len(keep))]
     both = [j for j in range(len(keep)) if fin32[j] and reached[j]]
     err, l1 = E.errors([acc["out"][j][0] for j in both], [ref[keep[j]][0] for j in both])
-    err["members_nonfinite_fp64"] = len(scs) - len(keep)
+    err["members_nonfinite_fp64"] = n - len(keep)
     err["members_nonfinite_fp32"] = len(keep) - sum(fin32)
     err["members_stalled_fp32"] = sum(1 for j in range(len(keep)) if fin32[j] and not reached[j])
     ok = [keep[j] for j in both]
@@ -109,40 +208,62 @@ def ensemble(args) -> int:
     err["floor_l1_median_subset"] = float(np.median(fl1))
     err["fp32_over_floor_median_subset"] = float(np.median(l1[fidx] / np.maximum(fl1, 1e-12)))
     err["frac_l1_le_2floor_plus_1pct_subset"] = float((l1[fidx] <= 2 * fl1 + 1e-2).mean())
-    n = len(scs)
-    per_h = lambda wall: n / wall * 3600.0
+    del ref, plain["out"]
     out = {
-        "metric": "ensemble scenarios/hour",
-        "value": per_h(b32["wall_s"]),
-        "unit": "scenarios/h",
-        "n_gpus": 1,
-        "steps": steps,
-        "warmup": 0,
-        "ms_per_step": b32["step_s"] / steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 (f64 dt, CFL max, mass)",
-        "data": "synthetic",
-        "config": {"workload": f"batched ensemble, {n} landslides on seeded fractal DEMs "
-                               "(self-affine relief), sides 128/256/512/1024 with weights 4:3:2:1, "
-                               "LHS basal friction 15-30 deg, volume 1e5-5e6 m^3, phi 0.4-0.7, "
-                               f"{steps} steps each, per-scenario dt",
-                   "members": n, "cells": cells, "steps": steps, "dry_skip": dry},
-        "speedup_vs_plain_fp64": t_plain / b32["wall_s"],
-        "plain_fp64": {"scenarios_per_hour": per_h(t_plain), "wall_s": t_plain},
+        "scenarios_per_hour": per_h(b32["wall_s"]),
+        "members": n, "cells": cells, "steps": steps, "dry_skip": dry,
+        "speedup_vs_plain_fp64": plain["wall_s"] / b32["wall_s"],
+        "plain_fp64": {"scenarios_per_hour": per_h(plain["wall_s"]), "wall_s": plain["wall_s"],
+                       "members_nonfinite": int(sum(plain["nonfinite"]))},
         "batched_fp64": {"scenarios_per_hour": per_h(b64["wall_s"]), "wall_s": b64["wall_s"],
-                         "step_s": b64["step_s"], "tune": b64["tune_report"].strip().split("\n")[-1]},
-        "batched_fp32": {"wall_s": b32["wall_s"], "step_s": b32["step_s"], "open_s": b32["open_s"],
-                         "tune_s": b32["tune_s"], "tune": b32["tune_report"].strip().split("\n")[-1],
-                         "cell_updates_per_s": cells * steps / b32["step_s"]},
+                         "step_s": b64["step_s"], "tune": b64["tune_report"].strip().split("\n")[-1],
+                         "members_nonfinite": int(sum(b64["nonfinite"]))},
+        "batched_fp32": {"scenarios_per_hour": per_h(b32["wall_s"]), "wall_s": b32["wall_s"],
+                         "step_s": b32["step_s"], "open_s": b32["open_s"], "tune_s": b32["tune_s"],
+                         "tune": b32["tune_report"].strip().split("\n")[-1],
+                         "cell_updates_per_s": cells * steps / b32["step_s"],
+                         "members_nonfinite": int(sum(b32["nonfinite"]))},
         "fp32_error_vs_fp64": err,
     }
+    if dense_steps and dry:
+        scale = steps / dense_steps
+        pd = E.run_plain(scs, dense_steps, "fp64", keep=False, dry_skip=False)
+        bd = E.run_batch(scs, dense_steps, "fp32", keep=False, dry_skip=False)
+        wp = pd["open_s"] + pd["step_s"] * scale
+        wb = bd["open_s"] + bd["tune_s"] + bd["step_s"] * scale
+        out["dense"] = {"timed_steps": dense_steps, "plain_fp64_scenarios_per_hour": per_h(wp),
+                        "batched_fp32_scenarios_per_hour": per_h(wb), "speedup_vs_plain_fp64": wp / wb,
+                        "note": f"every tile computed; stepping timed over {dense_steps} of the "
+                                f"{steps} steps and scaled (a dense step's cost does not depend on "
+                                "the state), set-up and tuning as measured"}
+    return out
+
+
+def ensemble(args) -> int:
+    """The batched-ensemble workload on its own JSON line."""
+    r = ensemble_measure(args)
+    steps = r["steps"]
+    out = {
+        "metric": "ensemble scenarios/hour", "value": r["scenarios_per_hour"], "unit": "scenarios/h",
+        "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": r["batched_fp32"]["step_s"] / steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 dt, CFL max, mass)",
+        "data": "synthetic",
+        "config": {"workload": ENSEMBLE_WORKLOAD.format(n=r["members"], steps=steps),
+                   "members": r["members"], "cells": r["cells"], "steps": steps,
+                   "dry_skip": r["dry_skip"]},
+    }
+    out.update({k: v for k, v in r.items() if k not in ("scenarios_per_hour",)})
     print(json.dumps(out), flush=True)
     return 0
 
 
-def serve(args) -> int:
+ENSEMBLE_WORKLOAD = ("batched ensemble, {n} landslides on seeded fractal DEMs (self-affine relief), "
+                     "sides 128/256/512/1024 with weights 4:3:2:1, LHS basal friction 15-30 deg, "
+                     "volume 1e5-5e6 m^3, phi 0.4-0.7, {steps} steps each, per-scenario dt")
+
+
+def serve_measure(args) -> dict:
     """Continuous-batching FP32 serving against the plain FP64 one-at-a-time path on the
     same Poisson request stream."""
     from moses_2pdf_amd import serving as V
@@ -167,21 +288,31 @@ def serve(args) -> int:
     wall0 = time.perf_counter() - t1
     plain = V.summary(res0, wall0, args.max_steps)
     V_log("plain fp64 one at a time:", plain)
+    return {"runs_per_hour": cb["runs_per_hour"], "latency_p50_s": cb["latency_p50_s"],
+            "latency_p99_s": cb["latency_p99_s"], "requests": len(reqs), "cells": cells,
+            "dry_skip": dry, "continuous_batching_fp32": cb, "plain_fp64_one_at_a_time": plain,
+            "speedup_runs_per_hour": cb["runs_per_hour"] / plain["runs_per_hour"]}
+
+
+SERVE_WORKLOAD = ("serving: {k} hazard requests, Poisson {rate}/s, grid side lognormal (median 384, "
+                  "sigma 0.5) clipped to 128-2048, landslide/inflow 70/30, stop at max flow speed "
+                  "< 0.01 m/s after motion or {ms} steps, admission by HBM footprint ({gb} GB budget)")
+
+
+def serve(args) -> int:
+    """The serving workload on its own JSON line."""
+    r = serve_measure(args)
     out = {
-        "metric": "serving runs/hour", "value": cb["runs_per_hour"], "unit": "runs/h", "n_gpus": 1,
+        "metric": "serving runs/hour", "value": r["runs_per_hour"], "unit": "runs/h", "n_gpus": 1,
         "steps": args.max_steps, "warmup": 0, "ms_per_step": None, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 dt, CFL max, mass)",
         "data": "synthetic",
-        "config": {"workload": f"serving: {len(reqs)} hazard requests, Poisson {args.rate}/s, grid side "
-                               "lognormal (median 384, sigma 0.5) clipped to 128-2048, landslide/inflow "
-                               f"70/30, stop at max flow speed < 0.01 m/s after motion or {args.max_steps} "
-                               f"steps, admission by HBM footprint ({args.hbm_budget} GB budget)",
-                   "requests": len(reqs), "cells": cells, "tick_steps": args.tick,
-                   "dry_skip": dry},
-        "latency_p50_s": cb["latency_p50_s"], "latency_p99_s": cb["latency_p99_s"],
-        "continuous_batching_fp32": cb, "plain_fp64_one_at_a_time": plain,
-        "speedup_runs_per_hour": cb["runs_per_hour"] / plain["runs_per_hour"],
+        "config": {"workload": SERVE_WORKLOAD.format(k=r["requests"], rate=args.rate, ms=args.max_steps,
+                                                     gb=args.hbm_budget),
+                   "requests": r["requests"], "cells": r["cells"], "tick_steps": args.tick,
+                   "dry_skip": r["dry_skip"]},
     }
+    out.update({k: v for k, v in r.items() if k not in ("runs_per_hour",)})
     print(json.dumps(out), flush=True)
     return 0
 
@@ -193,7 +324,14 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=20,
+                    help="window of the 2048^2 workload the CPU baseline times")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="leave out the batched-ensemble summary of the default line")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="leave out the serving summary of the default line")
+    ap.add_argument("--dense-window", type=int, default=300,
+                    help="steps over which the default line times the dense ensemble paths")
     ap.add_argument("--precision", choices=sorted(ELEM), default="fp64",
                     help="fp64: the reference's arithmetic (the headline); fp32: FP32-bulk")
     ap.add_argument("--dense", action="store_true",
@@ -205,9 +343,11 @@ def main() -> int:
     ap.add_argument("--copy-calib", action="store_true",
                     help="after the timed steps, copy the 7 state planes once device to device "
                          "(a plain copy kernel of known size, for HBM counter calibration)")
-    ap.add_argument("--workload", choices=["landslide", "ensemble", "serve"], default="landslide",
-                    help="landslide: the 2048^2 FP64 throughput case (default); ensemble: the "
-                         "batched FP32 ensemble against the plain FP64 path")
+    ap.add_argument("--workload", choices=["landslide", "ensemble", "serve", "accuracy"],
+                    default="landslide",
+                    help="landslide: the 2048^2 FP64 throughput case with the ensemble, serving "
+                         "and CPU summaries (default); ensemble, serve: one of those on its own "
+                         "line; accuracy: the FP32-against-FP64 table")
     ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--ens-steps", type=int, default=3000)
     ap.add_argument("--floor-members", type=int, default=16)
@@ -225,6 +365,8 @@ def main() -> int:
         return ensemble(args)
     if args.workload == "serve":
         return serve(args)
+    if args.workload == "accuracy":
+        return accuracy(args)
     return landslide(args)
 
 
@@ -278,6 +420,18 @@ def shard_run(sc, args, dense_value, parts: int = 8) -> dict:
             "ms_per_step": el / args.steps * 1e3, "per_cell_rate_vs_full_grid": v / dense_value,
             "note": f"one GPU's share of the {sc.nx}x{sc.ny} grid cut into {parts} row strips, every "
                     "tile computed; halo exchange and dt all-reduce not included"}
+
+
+def limiter(kernel: str, args) -> dict:
+    """What holds `kernel` below its HBM roofline, from the committed PMC counters."""
+    v = valu_from_profiles(kernel, args.n, args.precision)
+    why = ("every FP64 divide and square root stays the IEEE sequence (bit parity with the "
+           "oracle), so VALU issue and the latency of dependent loads, not bytes, set the time"
+           if args.precision == "fp64" else
+           "FP32 arithmetic with hardware reciprocals; VALU issue sets the time")
+    return {"kind": "valu issue and latency", "valu_issue_share": v[0] if v else None,
+            "wait_share": v[1] if v else None,
+            "source": f"profiles/valu_counters_{args.n}.txt" if v else None, "why": why}
 
 
 def landslide(args) -> int:
@@ -343,11 +497,10 @@ def landslide(args) -> int:
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "alg_bytes_per_cell": alg_bytes[dom],
                      "measured_GBs": (traffic / (avg_ms * 1e-3) / 1e9) if traffic else None,
-                     "note": ("the step is bound by VALU issue and latency, not HBM: every FP64 "
-                              "divide and sqrt stays IEEE (bit parity)" if args.precision == "fp64" else
-                              "the step is bound by VALU issue (about 65% issue share), not HBM")
-                             + "; VALU issue share and wait share per kernel in "
-                               "profiles/valu_counters_2048.txt"},
+                     "limiter": limiter(dom, args),
+                     "note": "bound names the roofline the fraction is priced against (HBM: the "
+                             "path is a stencil, no MFMA); limiter names what the counters show "
+                             "holds the kernel below it"},
         "step_hbm_alg_GBs": step_alg * args.steps / elapsed / 1e9,
         "kernels_ms": {k: v[0] for k, v in kern.items()},
         "state_finite": ok,
@@ -357,9 +510,20 @@ def landslide(args) -> int:
         out["dry_skip"] = dry_skip_run(sc, args, u, value)
     if not args.no_shard:
         out["shard"] = shard_run(sc, args, value)
+    del sim
+    if args.precision == "fp64" and args.n == 2048 and not args.no_ensemble:
+        print("# ensemble ...", file=sys.stderr, flush=True)
+        r = ensemble_measure(args, dense_steps=args.dense_window)
+        out["ensemble"] = dict(workload=ENSEMBLE_WORKLOAD.format(n=r["members"], steps=r["steps"]),
+                               **r)
+    if args.precision == "fp64" and args.n == 2048 and not args.no_serving:
+        print("# serving ...", file=sys.stderr, flush=True)
+        r = serve_measure(args)
+        out["serving"] = dict(workload=SERVE_WORKLOAD.format(k=r["requests"], rate=args.rate,
+                                                             ms=args.max_steps, gb=args.hbm_budget),
+                              **r)
     if not args.no_cpu_baseline:
         print("# cpu baseline (oracle) ...", file=sys.stderr, flush=True)
-        del sim
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps)
     else:
         out["cpu_baseline"] = None

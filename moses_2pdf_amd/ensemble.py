@@ -29,14 +29,20 @@ def population(count: int = 256, seed: int = 4):
 
 def run_plain(scs, steps, precision="fp64", keep=True, dry_skip=True):
     """Each scenario alone, one after another, `steps` m2p_step steps each.  Returns the
-    wall time, the (state, time) of each member (if keep) and each member's non-finite
-    flag."""
+    wall time ("wall_s", of which "open_s" opening the scenarios and "step_s" stepping
+    them), each member's (state, time) ("out", if keep) and non-finite flag ("nonfinite")."""
     t0 = time.perf_counter()
     out, bad = [], []
+    t_open = t_step = 0.0
     for k, sc in enumerate(scs):
+        ta = time.perf_counter()
         with Simulation(sc, precision=precision, dry_skip=dry_skip) as s:
+            s.sync()
+            tb = time.perf_counter()
             s.step(steps)
             s.sync()
+            t_step += time.perf_counter() - tb
+            t_open += tb - ta
             if keep:
                 out.append((s.state(), s.time_steps()[0]))
             else:
@@ -44,7 +50,8 @@ def run_plain(scs, steps, precision="fp64", keep=True, dry_skip=True):
             bad.append(s.nonfinite()[0])
         if (k + 1) % 32 == 0:
             log(f"plain {precision}: {k + 1}/{len(scs)} scenarios, {time.perf_counter() - t0:.1f} s")
-    return time.perf_counter() - t0, out, bad
+    return {"wall_s": time.perf_counter() - t0, "open_s": t_open, "step_s": t_step, "out": out,
+            "nonfinite": bad}
 
 
 def run_batch(scs, steps, precision, tune_steps=3, t_end=None, keep=True, graph_steps=None,

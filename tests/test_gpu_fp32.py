@@ -16,7 +16,6 @@ m2p_run_to (its last step shortened to land on it).  Stated tolerances:
 Known limitation (not tested): on the 35-degree plane with CFL 0.4 the FP32 run develops a
 thin-layer front instability after ~200 steps that FP64 does not.
 """
-import dataclasses
 import sys
 
 import numpy as np
@@ -24,7 +23,7 @@ import pytest
 
 from moses_2pdf_amd import Simulation
 from moses_2pdf_amd import scenarios as S
-from moses_2pdf_amd.accuracy import depth_error
+from moses_2pdf_amd.accuracy import depth_error, mass_rel, mid_step_time, run_to_time
 from util import df004, hsiaolin
 
 pytestmark = pytest.mark.gpu
@@ -44,31 +43,12 @@ CASES = {"member128": lambda: member(128), "member256": lambda: member(256),
          "hsiaolin": hsiaolin, "df004": df004}
 
 
-def mid_step_time(sc, steps):
-    """A time half-way through the FP64 run's step number `steps`."""
-    with Simulation(sc) as a:
-        a.step(steps - 1)
-        t0 = a.time_steps()[0]
-        a.step(1)
-        return 0.5 * (t0 + a.time_steps()[0])
-
-
 def fp64_to(sc, t_end, max_steps):
-    with Simulation(sc) as a:
-        n = a.run_to(t_end, max_steps)
-        assert a.time_steps()[0] == t_end
-        return a.state(), a.mass(), n
+    return run_to_time(sc, t_end, max_steps)
 
 
 def fp32_to(sc, t_end, max_steps):
-    with Simulation(sc, precision="fp32") as b:
-        n = b.run_to(t_end, max_steps)
-        assert b.time_steps()[0] == t_end
-        return b.state(), b.mass(), n
-
-
-def mass_rel(m, ref):
-    return max(abs(x - y) / abs(y) for x, y in zip(m, ref))
+    return run_to_time(sc, t_end, max_steps, "fp32")
 
 
 def test_fp32_initial_state_is_rounded_fp64():
@@ -100,21 +80,10 @@ def test_fp32_within_conditioning_floor(case):
     steps = 1000
     t = mid_step_time(sc, steps)
     u64, m64, _ = fp64_to(sc, t, 2 * steps)
-    rng = np.random.default_rng(1)
-    if sc.mode == 1:
-        # an inflow run starts dry, so a relative perturbation of its state changes
-        # nothing: perturb the hydrograph depths instead (they enter as float, so about
-        # half of them move by one float ulp)
-        sp = dataclasses.replace(
-            sc, inflow_h=sc.inflow_h * (1 + 1e-7 * rng.standard_normal(sc.inflow_h.shape)))
-    else:
-        sp = sc
-    with Simulation(sp) as p:   # the perturbed FP64 run, to the same time
-        u = p.state()
-        if sc.mode == 0:
-            p.set_state(u * (1 + 1e-7 * rng.standard_normal(u.shape)))
-        p.run_to(t, 20 * steps)
-        up = p.state()
+    # the perturbed FP64 run, to the same time (an inflow run starts dry, so a relative
+    # perturbation of its state changes nothing: its hydrograph depths are perturbed
+    # instead; they enter as float, so about half of them move by one float ulp)
+    up, _, _ = run_to_time(sc, t, 20 * steps, perturb=1e-7)
     floor = depth_error(up, u64)["l1"]
     assert floor > 0
     u32, m32, n = fp32_to(sc, t, 20 * steps)
