@@ -255,11 +255,23 @@ int setup(m2p_ctx* c) {
     m2p::launch_face_metric(c->g, c->dM, c->dM + m2p::MC_FX * P, c->s);
   }
   HIPCHK(c, hipGetLastError());
+  int uni = 0;
+  if (c->prec == M2P_FP64) {
+    // may the step form the inverse Jacobian instead of loading it?  (the mass buffer is
+    // free during set-up and holds the flag)
+    int* ok = (int*)c->dMass;
+    uni = 1;
+    HIPCHK(c, hipMemcpyAsync(ok, &uni, sizeof uni, hipMemcpyHostToDevice, c->s));
+    m2p::launch_metric_uniform_check(c->g, c->dM, ok, c->s);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(&uni, ok, sizeof uni, hipMemcpyDeviceToHost, c->s));
+  }
   // the first predictor's corner ghosts come from the initial state's own ghost rows
   HIPCHK(c, hipMemcpyAsync(c->dU1, c->dU0, 7 * P * c->es, hipMemcpyDeviceToDevice, c->s));
   rescan_dry(c, c->dry, c->s);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->s));
+  c->uni_metric = uni != 0;
   if (c->prec == M2P_FP32) (void)hipFree(dU64);
   // the grids are only needed by the setup
   (void)hipFree(c->dTopo);
@@ -377,7 +389,7 @@ void enqueue_step_t(m2p_ctx* c, int ntf) {
   }
   {
     Timer t(c, "cfl");
-    m2p::launch_cfl(c->g, c->ph, (const T*)U0, MT, c->dSt, c->dPart, c->dry, c->s);
+    m2p::launch_cfl(c->g, c->ph, (const T*)U0, MT, c->dSt, c->dPart, c->dry, c->uni_metric, c->s);
   }
   {
     Timer t(c, "dt");
@@ -389,7 +401,8 @@ void enqueue_step_t(m2p_ctx* c, int ntf) {
     if (c->dry.nseg > 0)
       m2p::launch_plan(0, c->g, (const T*)U0, c->dry, c->plan[0], c->dPart, c->dSt, c->s);
     m2p::launch_stage(0, c->g, c->ph, (const T*)U0, (const T*)U0, U1, ONE, MT, c->dLIM, c->dSt,
-                      c->dPart, c->dry, c->dry.nseg > 0 ? c->plan[0] : m2p::TilePlan{}, c->s);
+                      c->dPart, c->dry, c->dry.nseg > 0 ? c->plan[0] : m2p::TilePlan{},
+                      c->uni_metric, c->s);
   }
   {
     Timer t(c, "ghost");
@@ -400,7 +413,8 @@ void enqueue_step_t(m2p_ctx* c, int ntf) {
     if (c->dry.nseg > 0)
       m2p::launch_plan(1, c->g, (const T*)U1, c->dry, c->plan[1], c->dPart, c->dSt, c->s);
     m2p::launch_stage(1, c->g, c->ph, (const T*)U1, (const T*)U0, U0, ONE, MT, c->dLIM, c->dSt,
-                      c->dPart, c->dry, c->dry.nseg > 0 ? c->plan[1] : m2p::TilePlan{}, c->s);
+                      c->dPart, c->dry, c->dry.nseg > 0 ? c->plan[1] : m2p::TilePlan{},
+                      c->uni_metric, c->s);
   }
   if (c->vstop_on) m2p::launch_vmax(c->dSt, c->g, c->prec, c->dPart, c->s);
 }
@@ -907,6 +921,10 @@ int m2p_get_field(m2p_ctx* c, const char* name, double* out) {
   if (n == "uone45") {
     if (int rc = download_state(c, c->dONE, 2, out)) return rc;
     return 2;
+  }
+  if (n == "metric_uniform") {
+    out[0] = c->uni_metric ? 1.0 : 0.0;
+    return 1;
   }
   auto it = tab.find(n);
   if (it == tab.end()) return fail(c, M2P_EINVAL, "unknown field " + n);

@@ -118,6 +118,11 @@ struct DevParams {
 // planes of M; F is M + MC_FX * P in FP64, the float copy's face planes in FP32.
 template <typename TO>
 void launch_face_metric(const Grid& g, const double* M, TO* F, hipStream_t s);
+// The uniform-metric check: *ok (device int, set to 1 by the caller) is cleared unless the
+// stored FP64 metric M equals, bit for bit, what FM_UNIFORM forms (face averages on faces
+// [1, N-3]^2 in both directions, the inverse Jacobian of cells [3, N-4]^2), so that the
+// step kernels may form it instead of loading it.
+void launch_metric_uniform_check(const Grid& g, const double* M, int* ok, hipStream_t s);
 void launch_metric_setup(const Grid& g, int mode, const double* topo, const double* depth,
                          const double* phicell, double phiS0, double* M, double* U,
                          hipStream_t s);
@@ -140,9 +145,10 @@ void launch_zero_ghost_momenta(const Grid& g, T* U, hipStream_t s);
 // Per-block partial maxima of one step (CFL box blocks, CFL ring blocks, corrector tiles),
 // reduced by the dt kernel: partial_count() doubles per scenario.
 size_t partial_count(const Grid& g, int prec);
+// uni (FP64 only): the grid passed the uniform-metric check (launch_metric_uniform_check).
 template <typename T>
 void launch_cfl(const Grid& g, const Phys& ph, const T* U, const T* M, DevState* st, double* part,
-                const DryMap& dm, hipStream_t s);
+                const DryMap& dm, bool uni, hipStream_t s);
 // plan_cnt: the PLAN_COUNTERS plan counters of the step's two stages (zeroed here), or null.
 void launch_dt(const DevParams& dp, DevState* st, const double* tf, const double* itime,
                const Grid& g, int prec, const double* part, int* plan_cnt, hipStream_t s);
@@ -158,7 +164,7 @@ void launch_vmax(DevState* st, const Grid& g, int prec, const double* part, hipS
 template <typename T>
 void launch_stage(int io, const Grid& g, const Phys& ph, const T* Uin, const T* Uzero, T* Uout,
                   T* ONE, const T* M, unsigned char* LIM, DevState* st, double* part,
-                  const DryMap& dm, const TilePlan& pl, hipStream_t s);
+                  const DryMap& dm, const TilePlan& pl, bool uni, hipStream_t s);
 // Compensated mass totals: each of mass_blocks() blocks writes 4 partials (solid hi, lo,
 // fluid hi, lo) of sum u0*Det and u3*Det over its share of the interior cells into
 // out[4 b .. 4 b + 3], every slot every call.  det is the Detmin plane.

@@ -13,6 +13,8 @@
 // bodies are device functions so the batched-ensemble kernels reuse them.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <algorithm>
 
 #include "m2p_kernels.h"
@@ -216,7 +218,7 @@ __global__ void k_zero_ghost_momenta(Grid g, T* __restrict__ U) {
 // each (one wave per 32 faces), in the same launch.
 constexpr int CFL_BLOCKS = 4096;
 
-template <typename T>
+template <typename T, bool UNI>
 __global__ void __launch_bounds__(256) k_cfl(Grid g, Phys ph, const T* __restrict__ U,
                                              const T* __restrict__ M, DevState* __restrict__ st,
                                              double* __restrict__ part, int nbx, int nbox,
@@ -225,11 +227,12 @@ __global__ void __launch_bounds__(256) k_cfl(Grid g, Phys ph, const T* __restric
   double m = 0.0;
   if ((int)blockIdx.x >= nbb) {   // ring faces
     const int t = ((int)blockIdx.x - nbb) * 256 + threadIdx.y * 64 + threadIdx.x;
-    block_cfl_store(cfl_ring_face(U, M, g, ph, t, dm), part + blockIdx.x);
+    block_cfl_store(cfl_ring_face<T, UNI>(U, M, g, ph, t, dm), part + blockIdx.x);
     return;
   }
   if (dm.nseg == 0) {
-    for (int t = blockIdx.x; t < nbox; t += nbb) m = cfl_join(m, cfl_box(U, M, g, ph, t % nbx, t / nbx));
+    for (int t = blockIdx.x; t < nbox; t += nbb)
+      m = cfl_join(m, cfl_box<T, UNI>(U, M, g, ph, t % nbx, t / nbx));
     block_cfl_store(m, part + blockIdx.x);
     return;
   }
@@ -255,7 +258,7 @@ __global__ void __launch_bounds__(256) k_cfl(Grid g, Phys ph, const T* __restric
     for (int k = 0; k < 8; k++) {
       const int t = t0 + k * nbb;
       if (t >= nbox) break;
-      m = cfl_join(m, wetk[k] ? cfl_box(U, M, g, ph, t % nbx, t / nbx) : fl);
+      m = cfl_join(m, wetk[k] ? cfl_box<T, UNI>(U, M, g, ph, t % nbx, t / nbx) : fl);
     }
   }
   block_cfl_store(m, part + blockIdx.x);
@@ -390,6 +393,43 @@ void launch_face_metric(const Grid& g, const double* M, TO* F, hipStream_t s) {
 template void launch_face_metric<double>(const Grid&, const double*, double*, hipStream_t);
 template void launch_face_metric<float>(const Grid&, const double*, float*, hipStream_t);
 
+// One thread per padded cell (r, c): the x- and y-face averages at (r, c) when it is a face
+// of [1, N-3]^2, and the cell's inverse Jacobian when (r, c) is in [3, N-4]^2, each formed
+// as FM_UNIFORM forms them and compared bit for bit with what is stored.
+__global__ void k_metric_uniform_check(Grid g, const double* __restrict__ M, int* __restrict__ ok) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y * blockDim.y + threadIdx.y;
+  if (r >= g.nyd || c >= g.nxd) return;
+  const long long P = g.P, i = idx(g, r, c);
+  auto same = [](double a, double b) {
+    return __double_as_longlong(a) == __double_as_longlong(b);
+  };
+  bool good = true;
+  if (r >= 1 && r <= g.nyd - 3 && c >= 1 && c <= g.nxd - 3) {
+    for (int dir = 0; dir < 2; dir++) {
+      FaceMet<double> f;
+      face_metric<double, FM_UNIFORM>(M, g, dir, r, c, f);
+      const double* Fq = M + (dir == 0 ? MC_FX : MC_FY) * P;
+      good = good && same(f.j11, Fq[FQ_I11 * P + i]) && same(f.j12, Fq[FQ_I12 * P + i]) &&
+             same(f.j13, Fq[FQ_I13 * P + i]) && same(f.j21, Fq[FQ_I21 * P + i]) &&
+             same(f.j22, Fq[FQ_I22 * P + i]) && same(f.j23, Fq[FQ_I23 * P + i]) &&
+             same(f.xa, Fq[FQ_XA * P + i]) && same(f.xb, Fq[FQ_XB * P + i]);
+    }
+  }
+  if (r >= MD && r <= g.nyd - 4 && c >= MD && c <= g.nxd - 4) {
+    CellMet<double> a, b;
+    cell_metric<double, true>(M, P, i, a);
+    cell_metric<double, false>(M, P, i, b);
+    good = good && same(a.i11, b.i11) && same(a.i12, b.i12) && same(a.i13, b.i13) &&
+           same(a.i21, b.i21) && same(a.i22, b.i22) && same(a.i23, b.i23);
+  }
+  if (!good) atomicAnd(ok, 0);
+}
+
+void launch_metric_uniform_check(const Grid& g, const double* M, int* ok, hipStream_t s) {
+  const dim3 b(64, 4);
+  k_metric_uniform_check<<<grid2(g.nxd, g.nyd, b), b, 0, s>>>(g, M, ok);
+}
+
 template <typename T>
 void launch_convert(const double* a, T* b, long long n, hipStream_t s) {
   k_convert<T><<<1024, 256, 0, s>>>(a, b, n);
@@ -489,10 +529,18 @@ size_t partial_count(const Grid& g, int prec) {
 
 template <typename T>
 void launch_cfl(const Grid& g, const Phys& ph, const T* U, const T* M, DevState* st, double* part,
-                const DryMap& dm, hipStream_t s) {
+                const DryMap& dm, bool uni, hipStream_t s) {
   const dim3 b(64, 4);
-  k_cfl<T><<<cfl_box_blocks(g) + cfl_ring_blocks(g), b, 0, s>>>(
-      g, ph, U, M, st, part, (g.nx + 63) / 64, cfl_boxes(g), cfl_box_blocks(g), dm);
+  const int nb = cfl_box_blocks(g) + cfl_ring_blocks(g);
+  if constexpr (std::is_same<T, double>::value) {
+    if (uni) {
+      k_cfl<T, true><<<nb, b, 0, s>>>(g, ph, U, M, st, part, (g.nx + 63) / 64, cfl_boxes(g),
+                                      cfl_box_blocks(g), dm);
+      return;
+    }
+  }
+  k_cfl<T, false><<<nb, b, 0, s>>>(g, ph, U, M, st, part, (g.nx + 63) / 64, cfl_boxes(g),
+                                   cfl_box_blocks(g), dm);
 }
 
 void launch_dt(const DevParams& dp, DevState* st, const double* tf, const double* itime,
@@ -567,7 +615,7 @@ void launch_mass_sum(const double* part, int nb, double* out, hipStream_t s) {
                                 hipStream_t);                                                 \
   template void launch_zero_ghost_momenta<T>(const Grid&, T*, hipStream_t);                     \
   template void launch_cfl<T>(const Grid&, const Phys&, const T*, const T*, DevState*, double*, \
-                              const DryMap&, hipStream_t);                                     \
+                              const DryMap&, bool, hipStream_t);                               \
   template void launch_plan<T>(int, const Grid&, const T*, const DryMap&, const TilePlan&,     \
                                double*, const DevState*, hipStream_t);                         \
   template void launch_mass<T>(const Grid&, const T*, const double*, double*, hipStream_t);

@@ -173,11 +173,46 @@ __device__ __forceinline__ T avg(const T* a, long long i, long long j) {
   return T(0.5) * (a[i] + a[j]);
 }
 
+// The inverse Jacobian of makeTopo9Kernel (L/K:210-252) at a cell of a grid whose
+// coordinate derivatives are exactly the identity (dxdxi11 = dxdxi22 = 1, dxdxi12 =
+// dxdxi21 = +0, as makeTopo4Kernel gives them on the reference's uniform grid): then
+// i11 = i22 = 1 and i12 = i21 = -0 (L/K:197-203), and the reference's expressions
+//   invJ11 = i11 - i11 s0 s0 - i12 s1 s0,   invJ12 = i12 - i11 s0 s1 - i12 s1 s1,
+//   invJ21 = i21 - i21 s0 s0 - i22 s1 s0,   invJ22 = i22 - i21 s0 s1 - i22 s1 s1,
+//   invJ13 = c (s0 i11 + s1 i12),           invJ23 = c (s0 i21 + s1 i22)
+// reduce, operation for operation and including the sign of a zero result, to the ones
+// below: a product with -0 is a zero that cannot change a nonzero sum or difference, and
+// the zero results come out +0 both ways.  The library uses them only after a set-up
+// check has found them bit-identical to the stored planes wherever the step reads them.
+template <typename T>
+struct InvJ {
+  T j11, j12, j13, j21, j22, j23;
+};
+
+template <typename T>
+__device__ __forceinline__ InvJ<T> invj_uniform(T s0, T s1, T c) {
+  InvJ<T> v;
+  v.j11 = T(1.0) - s0 * s0;
+  v.j12 = T(0.0) - s0 * s1;
+  v.j21 = v.j12;
+  v.j22 = T(1.0) - s1 * s1;
+  v.j13 = c * (s0 + s1 * T(-0.0));
+  v.j23 = c * (s0 * T(-0.0) + s1);
+  return v;
+}
+
+// Where face_metric takes the face averages of MeanKernel from: FM_CELLS forms them from
+// the stored cell planes (the FP32 stage: fewer bytes, and its float averages differ from
+// the rounded FP64 ones by an ulp); FM_STORED reads the stored face planes; FM_UNIFORM (a
+// grid whose coordinate derivatives are exactly the identity, checked at set-up) forms
+// the inverse-Jacobian averages from the cells' svec and cvalue, which it reads anyway,
+// takes the coordinate-derivative averages as the constants 1 and +0, and reads only
+// the Detmin average.  All three give the stored values bit for bit in FP64.
+enum FaceMetSrc { FM_CELLS = 0, FM_STORED = 1, FM_UNIFORM = 2 };
+
 // dir 0: face (r,c)-(r,c+1); dir 1: face (r,c)-(r+1,c).  Valid for faces in [1,N-3]^2,
-// where MeanKernel writes the averages.  STORED reads the face planes; otherwise the
-// averages are formed here from the cell planes in T (the FP32 stage does this: fewer
-// bytes per cell, and its float averages differ from the rounded FP64 ones by an ulp).
-template <typename T, bool STORED = true>
+// where MeanKernel writes the averages.
+template <typename T, int SRC = FM_STORED>
 __device__ __forceinline__ void face_metric(const T* __restrict__ M, const Grid& g, int dir, int r,
                                             int c, FaceMet<T>& f) {
   const long long i = idx(g, r, c);
@@ -187,7 +222,13 @@ __device__ __forceinline__ void face_metric(const T* __restrict__ M, const Grid&
   const T* S0 = M + MC_S0 * P;
   const T* S1 = M + MC_S1 * P;
   const T* Fq = M + (dir == 0 ? MC_FX : MC_FY) * P;   // the stored face averages
-  if constexpr (STORED) {
+  f.sL0 = S0[i];
+  f.sL1 = S1[i];
+  f.cL = C[i];
+  f.sR0 = S0[j];
+  f.sR1 = S1[j];
+  f.cR = C[j];
+  if constexpr (SRC == FM_STORED) {
     f.j11 = Fq[FQ_I11 * P + i];
     f.j12 = Fq[FQ_I12 * P + i];
     f.j13 = Fq[FQ_I13 * P + i];
@@ -197,6 +238,17 @@ __device__ __forceinline__ void face_metric(const T* __restrict__ M, const Grid&
     f.det = Fq[FQ_DET * P + i];
     f.xa = Fq[FQ_XA * P + i];
     f.xb = Fq[FQ_XB * P + i];
+  } else if constexpr (SRC == FM_UNIFORM) {
+    const InvJ<T> a = invj_uniform(f.sL0, f.sL1, f.cL), b = invj_uniform(f.sR0, f.sR1, f.cR);
+    f.j11 = T(0.5) * (a.j11 + b.j11);
+    f.j12 = T(0.5) * (a.j12 + b.j12);
+    f.j13 = T(0.5) * (a.j13 + b.j13);
+    f.j21 = T(0.5) * (a.j21 + b.j21);
+    f.j22 = T(0.5) * (a.j22 + b.j22);
+    f.j23 = T(0.5) * (a.j23 + b.j23);
+    f.det = Fq[FQ_DET * P + i];
+    f.xa = dir == 0 ? T(1.0) : T(0.0);   // dxdxi11 avg (x) / dxdxi12 avg (y)
+    f.xb = dir == 0 ? T(0.0) : T(1.0);   // dxdxi21 avg (x) / dxdxi22 avg (y)
   } else {
     f.j11 = avg(M + MC_I11 * P, i, j);
     f.j12 = avg(M + MC_I12 * P, i, j);
@@ -208,12 +260,6 @@ __device__ __forceinline__ void face_metric(const T* __restrict__ M, const Grid&
     f.xa = avg(M + (dir == 0 ? MC_X11 : MC_X12) * P, i, j);
     f.xb = avg(M + (dir == 0 ? MC_X21 : MC_X22) * P, i, j);
   }
-  f.sL0 = S0[i];
-  f.sL1 = S1[i];
-  f.cL = C[i];
-  f.sR0 = S0[j];
-  f.sR1 = S1[j];
-  f.cR = C[j];
   f.j31 = T(0.5) * ((-f.sL0) + (-f.sR0));
   f.j32 = T(0.5) * ((-f.sL1) + (-f.sR1));
   f.j33 = T(0.5) * (f.cL + f.cR);
@@ -376,19 +422,30 @@ struct CellMet {
   T j13x, j23x, j33x, j13e, j23e, j33e;
 };
 
-template <typename T>
+// UNI: the inverse Jacobian from svec and cvalue (invj_uniform) instead of its planes.
+template <typename T, bool UNI = false>
 __device__ __forceinline__ void cell_metric(const T* __restrict__ M, long long P, long long i,
                                             CellMet<T>& m) {
   m.c = M[MC_C * P + i];
   m.s0 = M[MC_S0 * P + i];
   m.s1 = M[MC_S1 * P + i];
   m.det = M[MC_DET * P + i];
-  m.i11 = M[MC_I11 * P + i];
-  m.i12 = M[MC_I12 * P + i];
-  m.i13 = M[MC_I13 * P + i];
-  m.i21 = M[MC_I21 * P + i];
-  m.i22 = M[MC_I22 * P + i];
-  m.i23 = M[MC_I23 * P + i];
+  if constexpr (UNI) {
+    const InvJ<T> v = invj_uniform(m.s0, m.s1, m.c);
+    m.i11 = v.j11;
+    m.i12 = v.j12;
+    m.i13 = v.j13;
+    m.i21 = v.j21;
+    m.i22 = v.j22;
+    m.i23 = v.j23;
+  } else {
+    m.i11 = M[MC_I11 * P + i];
+    m.i12 = M[MC_I12 * P + i];
+    m.i13 = M[MC_I13 * P + i];
+    m.i21 = M[MC_I21 * P + i];
+    m.i22 = M[MC_I22 * P + i];
+    m.i23 = M[MC_I23 * P + i];
+  }
   m.j13x = M[MC_J13X * P + i];
   m.j23x = M[MC_J23X * P + i];
   m.j33x = M[MC_J33X * P + i];

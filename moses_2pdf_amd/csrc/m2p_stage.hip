@@ -5,6 +5,8 @@
 // stage_tile (m2p_tiles.h), shared with the batched kernels.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "m2p_kernels.h"
 #include "m2p_numerics.h"
 #include "m2p_tiles.h"
@@ -16,8 +18,8 @@
 namespace m2p {
 
 // PLANNED: dry-tile skipping, one block per plan-list slot (a kernel of its own, so the
-// dense kernel keeps its registers).
-template <int IO, typename T, int TY, int W, bool PLANNED>
+// dense kernel keeps its registers).  UNI: the uniform-metric form (stage_tile).
+template <int IO, typename T, int TY, int W, bool PLANNED, bool UNI>
 __global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per_eu(W, W)))
     k_stage(Grid g, Phys ph, const T* __restrict__ Uin, const T* Uzero, T* Uout,
             T* __restrict__ ONE, const T* __restrict__ M, unsigned char* __restrict__ LIM,
@@ -42,15 +44,15 @@ __global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per
   }
   int tx, ty;
   if (!tile_of(g, TY, t, tx, ty)) return;
-  const double v2 = stage_tile<IO, T, TY, PLANNED>(F, g, ph, Uin, Uzero, Uout, ONE, M, LIM,
+  const double v2 = stage_tile<IO, T, TY, PLANNED, UNI>(F, g, ph, Uin, Uzero, Uout, ONE, M, LIM,
                                                    IO == 0 ? st->dt_pred : st->dt_corr, tx, ty, dm);
   if (IO == 1 && st->vstop2 > 0.0) block_max_store(v2, part + t);
 }
 
-template <typename T>
-void launch_stage(int io, const Grid& g, const Phys& ph, const T* Uin, const T* Uzero, T* Uout,
-                  T* ONE, const T* M, unsigned char* LIM, DevState* st, double* part,
-                  const DryMap& dm, const TilePlan& pl, hipStream_t s) {
+template <typename T, bool UNI>
+static void launch_stage_t(int io, const Grid& g, const Phys& ph, const T* Uin, const T* Uzero,
+                           T* Uout, T* ONE, const T* M, unsigned char* LIM, DevState* st,
+                           double* part, const DryMap& dm, const TilePlan& pl, hipStream_t s) {
   constexpr int TY = PlainStage<T>::TY, W = PlainStage<T>::W;
   const dim3 b(64, TY + 1);
   const dim3 gr(pl.list ? plan_blocks(stage_tiles(g, TY))
@@ -58,21 +60,34 @@ void launch_stage(int io, const Grid& g, const Phys& ph, const T* Uin, const T* 
   double* sp = part + stage_partials_offset(g);
   if (pl.list) {
     if (io == 0)
-      k_stage<0, T, TY, W, true><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
+      k_stage<0, T, TY, W, true, UNI><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
     else
-      k_stage<1, T, TY, W, true><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
+      k_stage<1, T, TY, W, true, UNI><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
   } else {
     if (io == 0)
-      k_stage<0, T, TY, W, false><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
+      k_stage<0, T, TY, W, false, UNI><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
     else
-      k_stage<1, T, TY, W, false><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
+      k_stage<1, T, TY, W, false, UNI><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
   }
+}
+
+template <typename T>
+void launch_stage(int io, const Grid& g, const Phys& ph, const T* Uin, const T* Uzero, T* Uout,
+                  T* ONE, const T* M, unsigned char* LIM, DevState* st, double* part,
+                  const DryMap& dm, const TilePlan& pl, bool uni, hipStream_t s) {
+  if constexpr (std::is_same<T, double>::value) {
+    if (uni) {
+      launch_stage_t<T, true>(io, g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, part, dm, pl, s);
+      return;
+    }
+  }
+  launch_stage_t<T, false>(io, g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, part, dm, pl, s);
 }
 
 template void launch_stage<M2P_STAGE_T>(int, const Grid&, const Phys&, const M2P_STAGE_T*,
                                         const M2P_STAGE_T*, M2P_STAGE_T*, M2P_STAGE_T*,
                                         const M2P_STAGE_T*, unsigned char*, DevState*, double*,
-                                        const DryMap&, const TilePlan&, hipStream_t);
+                                        const DryMap&, const TilePlan&, bool, hipStream_t);
 
 #ifdef M2P_STAGE_PROBE   // in the FP64 object only
 // Confirms the library holds a code object for the current device (a missing gfx950 image
@@ -80,7 +95,7 @@ template void launch_stage<M2P_STAGE_T>(int, const Grid&, const Phys&, const M2P
 hipError_t probe_kernels() {
   hipFuncAttributes a;
   return hipFuncGetAttributes(&a, reinterpret_cast<const void*>(
-                                   &k_stage<0, double, PlainStage<double>::TY, PlainStage<double>::W, false>));
+                                   &k_stage<0, double, PlainStage<double>::TY, PlainStage<double>::W, false, false>));
 }
 
 #endif

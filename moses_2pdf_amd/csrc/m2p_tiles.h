@@ -174,7 +174,7 @@ __device__ __forceinline__ double wave_max(double v) {
 // the larger magnitude, negative if either value is, so the magnitude of a reduction is
 // the same fmax the dt control has always used, and its sign says whether any face met a
 // non-finite state.  Slots with no faces hold +0.
-template <typename T>
+template <typename T, bool UNI = false>
 __device__ __forceinline__ double face_cfl(const T* __restrict__ U, const T* __restrict__ M,
                                            const Grid& g, const Phys& ph, int dir, int r, int c,
                                            bool edge) {
@@ -184,7 +184,7 @@ __device__ __forceinline__ double face_cfl(const T* __restrict__ U, const T* __r
   else
     base_states(U, g, dir, r, c, L, R);
   FaceMet<T> f;
-  face_metric(M, g, dir, r, c, f);
+  face_metric<T, UNI ? FM_UNIFORM : FM_STORED>(M, g, dir, r, c, f);
   FaceSpeed<T> s;
   face_speed(dir, L, R, f, ph, s);
   const double d = dir == 0 ? g.dx : g.dy;
@@ -356,15 +356,15 @@ __device__ __forceinline__ DryRegion box_region(const Grid& g, int bx, int by) {
   return dry_region_of(g, rlo - 1, rlo + 5, clo - 1, clo + 65);
 }
 
-template <typename T>
+template <typename T, bool UNI = false>
 __device__ __forceinline__ double cfl_box(const T* __restrict__ U, const T* __restrict__ M,
                                           const Grid& g, const Phys& ph, int bx, int by) {
   const int c = bx * 64 + threadIdx.x + MD;
   const int r = by * 4 + threadIdx.y + MD;
   double m = 0.0;
   if (r <= g.ny + MD - 1 && c <= g.nx + MD - 1) {
-    m = face_cfl(U, M, g, ph, 0, r, c, false);
-    m = cfl_join(m, face_cfl(U, M, g, ph, 1, r, c, false));
+    m = face_cfl<T, UNI>(U, M, g, ph, 0, r, c, false);
+    m = cfl_join(m, face_cfl<T, UNI>(U, M, g, ph, 1, r, c, false));
   }
   return m;
 }
@@ -405,7 +405,7 @@ __device__ __forceinline__ bool ring_face_pos(const Grid& g, int t, int& r, int&
 // bounding box of those effective rows and columns, widened by 1 before and 2 after.  When
 // it is dry, each face gives the floor 1e-10 over its spacing, as computing it would.  A
 // wave whose faces lie on two strips far apart (its box would span the grid) is not tested.
-template <typename T>
+template <typename T, bool UNI = false>
 __device__ __forceinline__ double cfl_ring_face(const T* __restrict__ U, const T* __restrict__ M,
                                                 const Grid& g, const Phys& ph, int t,
                                                 const DryMap& dm) {
@@ -431,7 +431,7 @@ __device__ __forceinline__ double cfl_ring_face(const T* __restrict__ U, const T
                                     (int)(threadIdx.x & 63), 4 * 64))
       return valid ? (double)T(1.0e-10) / (dir == 0 ? g.dx : g.dy) : 0.0;
   }
-  return valid ? face_cfl(U, M, g, ph, dir, r, c, true) : 0.0;
+  return valid ? face_cfl<T, UNI>(U, M, g, ph, dir, r, c, true) : 0.0;
 }
 
 // CFL3Kernel + the host's dt control + CFL4Kernel (L/K:2006-2036, L/R:978-991; inflow
@@ -568,13 +568,14 @@ __device__ __forceinline__ void load_states(T (*Tl)[TY + 1][64], int w, int lane
 
 // Phase C: fluxes of one face from its final states, into the tile.
 // pl, pr: the viscous quotients of the face's left/lower and right/upper cells.
-template <typename T, int TY>
+template <typename T, int TY, bool UNI>
 __device__ __forceinline__ void flux_store(T (*Tl)[TY + 1][64], int w, int lane,
                                            const T* __restrict__ M, const Grid& g,
                                            const Phys& ph, int dir, int r, int c, T L[6], T R[6],
                                            const PDSide<T>& pl, const PDSide<T>& pr) {
   FaceMet<T> f;
-  face_metric<T, std::is_same<T, double>::value>(M, g, dir, r, c, f);
+  constexpr int SRC = !std::is_same<T, double>::value ? FM_CELLS : UNI ? FM_UNIFORM : FM_STORED;
+  face_metric<T, SRC>(M, g, dir, r, c, f);
   FaceSpeed<T> sp;
   face_speed(dir, L, R, f, ph, sp);
   FaceFlux<T> o;
@@ -594,8 +595,10 @@ __device__ __forceinline__ void flux_store(T (*Tl)[TY + 1][64], int w, int lane,
 constexpr double WET_SPEED = 1.0e-2;  // scaled depth (1 mm)
 
 // FLAGS: write the dry-tile flags of the output rows (only the planned stage kernels run
-// with dry-tile skipping on; the others are compiled without the flag code).
-template <int IO, typename T, int TY, bool FLAGS>
+// with dry-tile skipping on; the others are compiled without the flag code).  UNI (FP64
+// only): the grid's metric passed the uniform-metric check, so the inverse Jacobian is
+// formed from svec and cvalue instead of loaded (FM_UNIFORM; the same values bit for bit).
+template <int IO, typename T, int TY, bool FLAGS, bool UNI = false>
 __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, const Phys& ph,
                                            const T* __restrict__ Uin, const T* Uzero, T* Uout,
                                            T* __restrict__ ONE, const T* __restrict__ M,
@@ -637,7 +640,7 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
     } else if (has) {
       base_states(Uin, g, 0, r, c, L, R);
     }
-    if (has) flux_store<T, TY>(F, w, lane, M, g, ph, 0, r, c, L, R, Q[w * 65 + lane], Q[w * 65 + lane + 1]);
+    if (has) flux_store<T, TY, UNI>(F, w, lane, M, g, ph, 0, r, c, L, R, Q[w * 65 + lane], Q[w * 65 + lane + 1]);
   }
   __syncthreads();
 
@@ -690,7 +693,7 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
     } else if (has) {
       base_states(Uin, g, 1, rf, cf, L, R);
     }
-    if (has) flux_store<T, TY>(F, w, lane, M, g, ph, 1, rf, cf, L, R, Q[w * 64 + lane], Q[(w + 1) * 64 + lane]);
+    if (has) flux_store<T, TY, UNI>(F, w, lane, M, g, ph, 1, rf, cf, L, R, Q[w * 64 + lane], Q[(w + 1) * 64 + lane]);
   }
   __syncthreads();
   if (!active) return 0.0;
@@ -699,7 +702,7 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
 #pragma unroll
   for (int k = 0; k < 7; k++) u[k] = Uin[k * P + i];
   CellMet<T> cm;
-  cell_metric(M, P, i, cm);
+  cell_metric<T, UNI>(M, P, i, cm);
   T s[6];
   cell_source(u, cm, ph, s);
   T uz[6];
