@@ -111,7 +111,7 @@ int rebuild(m2p_batch* b) {
   std::vector<m2p::Member> M(n);
   std::vector<int> mg, mb, mr, ms;
   b->any_inflow = false;
-  bool any_vstop = false, any_dry = false;
+  bool any_vstop = false, any_dry = false, all_uni = true;
   for (int k = 0; k < n; k++) {
     m2p_ctx* c = b->mem[ord[k]];
     m2p::Member& m = M[k];
@@ -144,6 +144,7 @@ int rebuild(m2p_batch* b) {
     if (c->mode == 1) b->any_inflow = true;
     if (c->vstop_on) any_vstop = true;
     if (c->dry.nseg > 0) any_dry = true;
+    if (!c->uni_metric) all_uni = false;
     m.box_n = m.nbx * m2p::box_blocks_y(c->g);
     m.ring_n = m2p::ring_blocks(c->g);
     m.stage_n = m2p::stage_tiles(c->g, b->ty);
@@ -208,6 +209,7 @@ int rebuild(m2p_batch* b) {
   B.cfl_blocks = b->cfl_blocks;
   B.vmax = any_vstop ? 1 : 0;
   B.plan_on = any_dry ? 1 : 0;
+  B.uni = (all_uni && n > 0 && b->prec == M2P_FP64) ? 1 : 0;
   B.plan_cnt = b->dPlan;
   for (int io = 0; io < 2; io++) {
     B.plan[io].cnt = b->dPlan + 8 * io;

@@ -6,6 +6,8 @@
 // Compiled once per scalar type: M2P_BATCH_T is double or float.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "m2p_batch.h"
 #include "m2p_tiles.h"
 
@@ -49,7 +51,7 @@ __global__ void __launch_bounds__(GHOST_LINES) kb_ghost(BatchDev B, int post) {
 constexpr int CFL_GROUP = 64;
 constexpr int CFL_LANES = 256 / CFL_GROUP;
 
-template <typename T>
+template <typename T, bool UNI>
 __global__ void __launch_bounds__(256) kb_cfl(BatchDev B, int nbb) {
   __shared__ int state[CFL_GROUP];   // 0: dry, 1: compute, 2: halted or past the end
   const int tid = threadIdx.y * 64 + threadIdx.x;
@@ -59,7 +61,7 @@ __global__ void __launch_bounds__(256) kb_cfl(BatchDev B, int nbb) {
     const Member& m = B.mem[B.mapRing[rb]];
     if (halted_pre(m.st)) return;
     const int t = (rb - m.ring0) * 64 + threadIdx.x;
-    double v = cfl_ring_face((const T*)m.U0, (const T*)m.MT, m.g, m.ph, t, m.dry);
+    double v = cfl_ring_face<T, UNI>((const T*)m.U0, (const T*)m.MT, m.g, m.ph, t, m.dry);
     v = wave_cfl(v);
     if (threadIdx.x == 0) B.part[B.nBox + rb] = v;
     return;
@@ -102,7 +104,8 @@ __global__ void __launch_bounds__(256) kb_cfl(BatchDev B, int nbb) {
       const int t = t0 + k * nbb;
       const Member& m = B.mem[B.mapBox[t]];
       const int lb = t - m.box0;
-      block_cfl_store(cfl_box((const T*)m.U0, (const T*)m.MT, m.g, m.ph, lb % m.nbx, lb / m.nbx),
+      block_cfl_store(cfl_box<T, UNI>((const T*)m.U0, (const T*)m.MT, m.g, m.ph, lb % m.nbx,
+                                      lb / m.nbx),
                       B.part + t);
     }
   }
@@ -143,7 +146,15 @@ hipError_t launch_batch_steps(const BatchDev& B, int ty, int w, bool any_inflow,
     kb_ghost<T><<<dim3(B.nGhost, MN), GHOST_LINES, 0, s>>>(B, 0);
     {
       const int nbb = B.nBox < B.cfl_blocks ? B.nBox : B.cfl_blocks;
-      kb_cfl<T><<<nbb + (B.nRing + 3) / 4, dim3(64, 4), 0, s>>>(B, nbb);
+      const unsigned nb = nbb + (B.nRing + 3) / 4;
+      bool done = false;
+      if constexpr (std::is_same<T, double>::value) {
+        if (B.uni) {
+          kb_cfl<T, true><<<nb, dim3(64, 4), 0, s>>>(B, nbb);
+          done = true;
+        }
+      }
+      if (!done) kb_cfl<T, false><<<nb, dim3(64, 4), 0, s>>>(B, nbb);
     }
     kb_dt<T><<<B.nmem, 256, 0, s>>>(B);
     if (!launch_batch_stage<T>(B, ty, w, 0, s)) return hipErrorInvalidValue;

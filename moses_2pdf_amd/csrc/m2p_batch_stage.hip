@@ -38,8 +38,9 @@ __global__ void __launch_bounds__(256) kb_plan(BatchDev B) {
 }
 
 // PLANNED: dry-tile skipping, one block per plan-list slot (halted members have no
-// entries); a kernel of its own, so the dense kernel keeps its registers.
-template <int IO, typename T, int TY, int W, bool PLANNED>
+// entries); a kernel of its own, so the dense kernel keeps its registers.  UNI: every
+// member's metric has the uniform form (BatchDev::uni).
+template <int IO, typename T, int TY, int W, bool PLANNED, bool UNI>
 __global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per_eu(W, W)))
     kb_stage(BatchDev B) {
   __shared__ T F[ST_NQ][TY + 1][64];
@@ -67,14 +68,14 @@ __global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per
     stage_zero<T, TY>(m.g, Uout, (T*)m.ONE, m.dry, tx, ty);
     return;
   }
-  const double v2 = stage_tile<IO, T, TY, PLANNED>(F, m.g, m.ph, Uin, (const T*)m.U0, Uout, (T*)m.ONE,
+  const double v2 = stage_tile<IO, T, TY, PLANNED, UNI>(F, m.g, m.ph, Uin, (const T*)m.U0, Uout, (T*)m.ONE,
                                           (const T*)m.MT, m.LIM,
                                           IO == 0 ? st->dt_pred : st->dt_corr, tx, ty, m.dry);
   if (IO == 1 && st->vstop2 > 0.0) block_max_store(v2, B.part + B.nBox + B.nRing + t);
 }
 
-template <typename T, int TY, int W>
-static void stage_pair(const BatchDev& B, int io, hipStream_t s) {
+template <typename T, int TY, int W, bool UNI>
+static void stage_pair_t(const BatchDev& B, int io, hipStream_t s) {
   const dim3 blk(64, TY + 1);
   const unsigned nb = B.plan_on ? (unsigned)plan_blocks(B.nStage)
                    : B.remap ? (unsigned)(((B.nStage + N_XCD - 1) / N_XCD) * N_XCD) : (unsigned)B.nStage;
@@ -82,17 +83,28 @@ static void stage_pair(const BatchDev& B, int io, hipStream_t s) {
   if (B.plan_on) {
     if (io == 0) {
       kb_plan<0, T, TY><<<np, dim3(64, 4), 0, s>>>(B);
-      kb_stage<0, T, TY, W, true><<<nb, blk, 0, s>>>(B);
+      kb_stage<0, T, TY, W, true, UNI><<<nb, blk, 0, s>>>(B);
     } else {
       kb_plan<1, T, TY><<<np, dim3(64, 4), 0, s>>>(B);
-      kb_stage<1, T, TY, W, true><<<nb, blk, 0, s>>>(B);
+      kb_stage<1, T, TY, W, true, UNI><<<nb, blk, 0, s>>>(B);
     }
   } else {
     if (io == 0)
-      kb_stage<0, T, TY, W, false><<<nb, blk, 0, s>>>(B);
+      kb_stage<0, T, TY, W, false, UNI><<<nb, blk, 0, s>>>(B);
     else
-      kb_stage<1, T, TY, W, false><<<nb, blk, 0, s>>>(B);
+      kb_stage<1, T, TY, W, false, UNI><<<nb, blk, 0, s>>>(B);
   }
+}
+
+template <typename T, int TY, int W>
+static void stage_pair(const BatchDev& B, int io, hipStream_t s) {
+  if constexpr (std::is_same<T, double>::value) {
+    if (B.uni) {
+      stage_pair_t<T, TY, W, true>(B, io, s);
+      return;
+    }
+  }
+  stage_pair_t<T, TY, W, false>(B, io, s);
 }
 
 // The compiled (tile height, waves per SIMD) variants; the list in stage_variants() must
