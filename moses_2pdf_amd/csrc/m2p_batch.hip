@@ -121,6 +121,8 @@ int rebuild(m2p_batch* b) {
     m.dp.cfl = c->par.cfl;
     m.dp.dt_min = c->par.dt_min;
     m.dp.dt_max = c->par.dt_max;
+    m.dp.dx = c->dx;
+    m.dp.dy = c->dy;
     m.dp.ntf = 0;
     m.dp.nsamp = c->mode == 1 ? c->nsamp : 0;
     m.mode = c->mode;

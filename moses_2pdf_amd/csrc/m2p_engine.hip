@@ -376,6 +376,8 @@ void enqueue_step_t(m2p_ctx* c, int ntf) {
   dp.cfl = c->par.cfl;
   dp.dt_min = c->par.dt_min;
   dp.dt_max = c->par.dt_max;
+  dp.dx = c->dx;
+  dp.dy = c->dy;
   dp.ntf = ntf;
   dp.nsamp = c->mode == 1 ? c->nsamp : 0;
   if (c->mode == 1) {
@@ -557,8 +559,8 @@ int m2p_open_grid(int mode, int NX, int NY, double cellsize, const double* topo,
   c->nxd = NX + 2 * MD;
   c->nyd = NY + 2 * MD;
   c->P = (long long)c->nxd * c->nyd;
-  c->dx = cellsize * 10;  // L/R:181-182
-  c->dy = cellsize * 10;
+  c->dx = cellsize * 10;  // L/R:181-182; the cells are square (the CFL max relies on it)
+  c->dy = c->dx;
   c->par = *par;
   c->topo = pad_plane(NX, NY, topo);
   c->depth = (mode == 0) ? pad_plane(NX, NY, depth) : std::vector<double>((size_t)c->P, 0.0);

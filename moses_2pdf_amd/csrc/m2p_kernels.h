@@ -38,6 +38,9 @@ struct DevState {
   // reference's fmaxf does): the state after nonfinite_nt steps holds a NaN or an Inf.
   int nonfinite;
   int nonfinite_nt;   // -1: never
+  // dt/dx and dt/dy of the predictor and the corrector (Flux18Kernel's dt/dx, dt/dy,
+  // L/K:2641-2652), formed once by the dt kernel in FP64 instead of in every thread
+  double dtdx_pred, dtdy_pred, dtdx_corr, dtdy_corr;
 };
 
 // Dry-tile skipping.  A stage or CFL tile whose every input cell is zero in every
@@ -110,6 +113,7 @@ __device__ __forceinline__ bool halted_post(const DevState* st) { return st->ski
 
 struct DevParams {
   double cfl, dt_min, dt_max;
+  double dx, dy;   // cell size (scaled), for the stages' dt/dx and dt/dy
   int ntf;     // entries of tf (0: no output clipping)
   int nsamp;   // hydrograph samples (0: landslide)
 };

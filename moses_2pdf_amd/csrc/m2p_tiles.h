@@ -167,8 +167,13 @@ __device__ __forceinline__ double wave_max(double v) {
 // (InterfacesKernel states); its ring blocks the rest of [1, N-3]^2, where Interfaces2Kernel
 // overrides the states.  Speeds are formed in T; the quotient by dx and the max in FP64.
 //
+// The reference divides each face's speed by the cell size before the max (CFL1Kernel,
+// L/K:1939-1957).  The cells are square (dx = dy; one cellsize is read) and a correctly
+// rounded division by a positive constant is monotonic, so the max of the quotients is the
+// quotient of the max: the faces here give em, and the dt kernel divides the max once.
+//
 // CFL values carry a non-finite mark in their sign.  A face value is at least the floor
-// 1e-10/dx > 0; it is negated when one of the face's four one-sided speeds (apE, apW, apFE,
+// 1e-10 > 0; it is negated when one of the face's four one-sided speeds (apE, apW, apFE,
 // apFW) is not finite.  The reference's speed floor and maxima drop a NaN there without a
 // trace (fmax, fmaxf), so this is the only place it can still be seen.  cfl_join keeps
 // the larger magnitude, negative if either value is, so the magnitude of a reduction is
@@ -187,8 +192,7 @@ __device__ __forceinline__ double face_cfl(const T* __restrict__ U, const T* __r
   face_metric<T, UNI ? FM_UNIFORM : FM_STORED>(M, g, dir, r, c, f);
   FaceSpeed<T> s;
   face_speed(dir, L, R, f, ph, s);
-  const double d = dir == 0 ? g.dx : g.dy;
-  const double v = fmax((double)s.emS / d, (double)s.emF / d);
+  const double v = fmax((double)s.emS, (double)s.emF);
   return s.finite ? v : -v;
 }
 
@@ -348,7 +352,8 @@ __device__ __forceinline__ bool dry_region_wave(const T* __restrict__ U, const G
 // is all zero gives the wave-speed floor 1e-10 over dx and dy on every face.
 template <typename T>
 __device__ __forceinline__ double cfl_floor(const Grid& g) {
-  return fmax((double)T(1.0e-10) / g.dx, (double)T(1.0e-10) / g.dy);
+  (void)g;
+  return (double)T(1.0e-10);
 }
 
 __device__ __forceinline__ DryRegion box_region(const Grid& g, int bx, int by) {
@@ -429,15 +434,15 @@ __device__ __forceinline__ double cfl_ring_face(const T* __restrict__ U, const T
     }
     if (rb >= ra && dry_region_wave(U, g, dm.z0, dm.nseg, ra - 1, rb + 2, ca - 1, cb + 2,
                                     (int)(threadIdx.x & 63), 4 * 64))
-      return valid ? (double)T(1.0e-10) / (dir == 0 ? g.dx : g.dy) : 0.0;
+      return valid ? (double)T(1.0e-10) : 0.0;
   }
   return valid ? face_cfl<T, UNI>(U, M, g, ph, dir, r, c, true) : 0.0;
 }
 
 // CFL3Kernel + the host's dt control + CFL4Kernel (L/K:2006-2036, L/R:978-991; inflow
 // sample advance I/R:1078-1081), on the device.  mx is the max wave speed over the CFL
-// faces, negative when some face met a non-finite state (face_cfl); its magnitude is
-// rounded to float as the reference's fmaxf reduction does (L/K:1988, 1996).
+// faces, negative when some face met a non-finite state (face_cfl); its magnitude over dx
+// is rounded to float as the reference's fmaxf reduction does (L/K:1988, 1996).
 // Also the halting rules, which have no reference counterpart: a step budget (halt_at),
 // an end time (t_end: the step that reaches it is shortened to end on it) and a stop speed
 // (vstop2: once the squared max flow speed v2 of a corrector has reached it and a later one
@@ -460,6 +465,7 @@ __device__ __forceinline__ void dt_body(const DevParams& dp, DevState* __restric
     st->nonfinite = 1;
     mx = -mx;
   }
+  mx = mx / dp.dx;   // the CFL faces give em; dx = dy
   if (st->vstop2 > 0.0 && st->v2_nt == st->nt) {
     const double v2 = st->v2;
     if (v2 >= st->vstop2)
@@ -495,6 +501,10 @@ __device__ __forceinline__ void dt_body(const DevParams& dp, DevState* __restric
   st->dtval = dtval;
   st->dt_pred = dtval / (2.0 - 0);
   st->dt_corr = dtval / (2.0 - 1);
+  st->dtdx_pred = st->dt_pred / dp.dx;
+  st->dtdy_pred = st->dt_pred / dp.dy;
+  st->dtdx_corr = st->dt_corr / dp.dx;
+  st->dtdy_corr = st->dt_corr / dp.dy;
   st->time = st->dt_corr + T;
   st->nt = st->nt + 1;
 }
@@ -603,7 +613,8 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
                                            const T* __restrict__ Uin, const T* Uzero, T* Uout,
                                            T* __restrict__ ONE, const T* __restrict__ M,
                                            unsigned char* __restrict__ LIM, double dt_d, int tx,
-                                           int ty, const DryMap& dm) {
+                                           int ty, const DryMap& dm, double dtdx_d = 0.0,
+                                           double dtdy_d = 0.0) {
   const int lane = threadIdx.x;
   const int w = threadIdx.y;
   const int c0 = tx * ST_TX + MD;
@@ -648,7 +659,10 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
   const bool active = (w < TY) && (lane < ST_TX) && r <= ymax && c <= xmax;
   const long long i = idx(g, r, c);
   const T dt = T(dt_d);
-  const T dtdx = qdiv(dt, T(g.dx)), dtdy = qdiv(dt, T(g.dy));
+  // FP64: dt/dx and dt/dy as the dt kernel formed them (the same IEEE quotients)
+  constexpr bool F64 = std::is_same<T, double>::value;
+  const T dtdx = F64 ? T(dtdx_d) : qdiv(dt, T(g.dx));
+  const T dtdy = F64 ? T(dtdy_d) : qdiv(dt, T(g.dy));
   const T epsilon0 = T(1.0);
   T phi = T(0), fP = T(0), fD = T(0);
   T tH[6], tP[4], tD[2];
