@@ -153,7 +153,10 @@ int m2p_get_field(m2p_ctx* ctx, const char* name, double* out);
 
 /* Mean device time per launch of each kernel family since the last reset, measured with
  * HIP events on the context's stream when profiling is enabled (work of another context
- * queued on the same pool stream in between would be counted too). */
+ * queued on the same pool stream in between would be counted too).  enable: 0 off, 1 every
+ * family ("inflow", "ghost", "cfl", "dt", "stage0", "stage1", "vmax"), 2 + k only the
+ * k-th of them in that order (fewer events in the stream: each pair of event records
+ * between two kernels costs a few microseconds of a step). */
 int m2p_profile(m2p_ctx* ctx, int enable);
 int m2p_profile_get(m2p_ctx* ctx, const char* kernel, double* ms_per_launch, int* launches);
 

@@ -60,6 +60,7 @@ struct m2p_ctx {
   m2p::Phys ph{};
   std::string err;
   bool prof = false;
+  const char* prof_only = nullptr;   // profile only this kernel family (nullptr: every one)
   std::map<std::string, std::pair<double, int>> pstats;
   std::vector<hipEvent_t> evpool;
   size_t evnext = 0;

@@ -325,7 +325,7 @@ struct Timer {
   const char* name;
   hipEvent_t a = nullptr;
   Timer(m2p_ctx* c_, const char* n) : c(c_), name(n) {
-    if (c->prof) {
+    if (c->prof && (!c->prof_only || std::strcmp(c->prof_only, n) == 0)) {
       a = pool_event(c);
       (void)hipEventRecord(a, c->s);
     }
@@ -343,7 +343,7 @@ hipEvent_t pool_event(m2p_ctx* c) {
 }
 
 Timer::~Timer() {
-  if (c->prof) {
+  if (a) {
     hipEvent_t b = pool_event(c);
     (void)hipEventRecord(b, c->s);
     c->pending.push_back({name, {a, b}});
@@ -940,8 +940,11 @@ int m2p_get_field(m2p_ctx* c, const char* name, double* out) {
 
 int m2p_profile(m2p_ctx* c, int enable) {
   if (!c) return M2P_ESTATE;
+  static const char* const fam[] = {"inflow", "ghost", "cfl", "dt", "stage0", "stage1", "vmax"};
+  if (enable < 0 || enable > 2 + 6) return fail(c, M2P_EINVAL, "profile: enable must be 0, 1 or 2..8");
   resolve_profile(c);
   c->prof = enable != 0;
+  c->prof_only = enable >= 2 ? fam[enable - 2] : nullptr;
   c->pstats.clear();
   return 0;
 }
