@@ -145,10 +145,11 @@ int m2p_scalars(m2p_ctx* ctx, double* dt, double* dmax, int* iter);
 /* Copies a named nxd*nyd (or k*nxd*nyd) device field out, for verification: "cvalue",
  * "svec" (2 planes), "Detmin", "invJ11".."invJ23", "dxdxi11".."dxdxi22", "J13dxi",
  * "J23dxi", "J33dxi", "J13det", "J23det", "J33det", "uone45" (2 planes); and
- * "metric_uniform", one value: 1 when the FP64 step forms the inverse Jacobian from svec
- * and cvalue instead of loading it (the grid's coordinate derivatives are exactly the
- * identity, and a set-up check found the formed values bit-identical to the stored ones
- * wherever the step reads them), else 0.  Returns planes copied, or a negative code. */
+ * "metric_uniform", one value: 1 when the step forms the inverse Jacobian from svec and
+ * cvalue instead of loading it (the grid's coordinate derivatives are exactly the identity;
+ * in FP64 a set-up check found the formed values bit-identical to the stored ones wherever
+ * the step reads them, in FP32 they are formed in float), else 0.  Returns planes copied,
+ * or a negative code. */
 int m2p_get_field(m2p_ctx* ctx, const char* name, double* out);
 
 /* Mean device time per launch of each kernel family since the last reset, measured with

@@ -211,7 +211,7 @@ int rebuild(m2p_batch* b) {
   B.cfl_blocks = b->cfl_blocks;
   B.vmax = any_vstop ? 1 : 0;
   B.plan_on = any_dry ? 1 : 0;
-  B.uni = (all_uni && n > 0 && b->prec == M2P_FP64) ? 1 : 0;
+  B.uni = (all_uni && n > 0) ? 1 : 0;
   B.plan_cnt = b->dPlan;
   for (int io = 0; io < 2; io++) {
     B.plan[io].cnt = b->dPlan + 8 * io;

@@ -100,13 +100,10 @@ static void stage_pair_t(const BatchDev& B, int io, hipStream_t s) {
 
 template <typename T, int TY, int W>
 static void stage_pair(const BatchDev& B, int io, hipStream_t s) {
-  if constexpr (std::is_same<T, double>::value) {
-    if (B.uni) {
-      stage_pair_t<T, TY, W, true>(B, io, s);
-      return;
-    }
-  }
-  stage_pair_t<T, TY, W, false>(B, io, s);
+  if (B.uni)
+    stage_pair_t<T, TY, W, true>(B, io, s);
+  else
+    stage_pair_t<T, TY, W, false>(B, io, s);
 }
 
 // The compiled (tile height, waves per SIMD) variants; the list in stage_variants() must

@@ -255,14 +255,16 @@ int setup(m2p_ctx* c) {
     m2p::launch_face_metric(c->g, c->dM, c->dM + m2p::MC_FX * P, c->s);
   }
   HIPCHK(c, hipGetLastError());
-  int uni = 0;
-  if (c->prec == M2P_FP64) {
-    // may the step form the inverse Jacobian instead of loading it?  (the mass buffer is
-    // free during set-up and holds the flag)
+  // may the step form the inverse Jacobian instead of loading it?  (the mass buffer is
+  // free during set-up and holds the flag)
+  int uni = 1;
+  {
     int* ok = (int*)c->dMass;
-    uni = 1;
     HIPCHK(c, hipMemcpyAsync(ok, &uni, sizeof uni, hipMemcpyHostToDevice, c->s));
-    m2p::launch_metric_uniform_check(c->g, c->dM, ok, c->s);
+    if (c->prec == M2P_FP64)
+      m2p::launch_metric_uniform_check(c->g, c->dM, ok, c->s);
+    else
+      m2p::launch_metric_uniform_cells_check(c->g, c->dM, ok, c->s);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(&uni, ok, sizeof uni, hipMemcpyDeviceToHost, c->s));
   }

@@ -127,6 +127,10 @@ void launch_face_metric(const Grid& g, const double* M, TO* F, hipStream_t s);
 // [1, N-3]^2 in both directions, the inverse Jacobian of cells [3, N-4]^2), so that the
 // step kernels may form it instead of loading it.
 void launch_metric_uniform_check(const Grid& g, const double* M, int* ok, hipStream_t s);
+// The cell-level form of that check, for the FP32 variant (which keeps no FP64 face planes):
+// *ok is cleared unless, on cells [1, N-2]^2, the coordinate-derivative planes are exactly
+// 1 and +0 and the inverse-Jacobian planes equal invj_uniform of svec and cvalue bit for bit.
+void launch_metric_uniform_cells_check(const Grid& g, const double* M, int* ok, hipStream_t s);
 void launch_metric_setup(const Grid& g, int mode, const double* topo, const double* depth,
                          const double* phicell, double phiS0, double* M, double* U,
                          hipStream_t s);

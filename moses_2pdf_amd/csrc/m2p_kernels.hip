@@ -430,6 +430,29 @@ void launch_metric_uniform_check(const Grid& g, const double* M, int* ok, hipStr
   k_metric_uniform_check<<<grid2(g.nxd, g.nyd, b), b, 0, s>>>(g, M, ok);
 }
 
+__global__ void k_metric_uniform_cells_check(Grid g, const double* __restrict__ M,
+                                             int* __restrict__ ok) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y * blockDim.y + threadIdx.y;
+  if (r < 1 || c < 1 || r > g.nyd - 2 || c > g.nxd - 2) return;
+  const long long P = g.P, i = idx(g, r, c);
+  auto same = [](double a, double b) {
+    return __double_as_longlong(a) == __double_as_longlong(b);
+  };
+  CellMet<double> a, b;
+  cell_metric<double, true>(M, P, i, a);
+  cell_metric<double, false>(M, P, i, b);
+  const bool good = same(M[MC_X11 * P + i], 1.0) && same(M[MC_X22 * P + i], 1.0) &&
+                    same(M[MC_X12 * P + i], 0.0) && same(M[MC_X21 * P + i], 0.0) &&
+                    same(a.i11, b.i11) && same(a.i12, b.i12) && same(a.i13, b.i13) &&
+                    same(a.i21, b.i21) && same(a.i22, b.i22) && same(a.i23, b.i23);
+  if (!good) atomicAnd(ok, 0);
+}
+
+void launch_metric_uniform_cells_check(const Grid& g, const double* M, int* ok, hipStream_t s) {
+  const dim3 b(64, 4);
+  k_metric_uniform_cells_check<<<grid2(g.nxd, g.nyd, b), b, 0, s>>>(g, M, ok);
+}
+
 template <typename T>
 void launch_convert(const double* a, T* b, long long n, hipStream_t s) {
   k_convert<T><<<1024, 256, 0, s>>>(a, b, n);

@@ -77,13 +77,10 @@ template <typename T>
 void launch_stage(int io, const Grid& g, const Phys& ph, const T* Uin, const T* Uzero, T* Uout,
                   T* ONE, const T* M, unsigned char* LIM, DevState* st, double* part,
                   const DryMap& dm, const TilePlan& pl, bool uni, hipStream_t s) {
-  if constexpr (std::is_same<T, double>::value) {
-    if (uni) {
-      launch_stage_t<T, true>(io, g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, part, dm, pl, s);
-      return;
-    }
-  }
-  launch_stage_t<T, false>(io, g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, part, dm, pl, s);
+  if (uni)
+    launch_stage_t<T, true>(io, g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, part, dm, pl, s);
+  else
+    launch_stage_t<T, false>(io, g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, part, dm, pl, s);
 }
 
 template void launch_stage<M2P_STAGE_T>(int, const Grid&, const Phys&, const M2P_STAGE_T*,

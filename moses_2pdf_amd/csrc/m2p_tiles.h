@@ -584,7 +584,7 @@ __device__ __forceinline__ void flux_store(T (*Tl)[TY + 1][64], int w, int lane,
                                            const Phys& ph, int dir, int r, int c, T L[6], T R[6],
                                            const PDSide<T>& pl, const PDSide<T>& pr) {
   FaceMet<T> f;
-  constexpr int SRC = !std::is_same<T, double>::value ? FM_CELLS : UNI ? FM_UNIFORM : FM_STORED;
+  constexpr int SRC = UNI ? FM_UNIFORM : std::is_same<T, double>::value ? FM_STORED : FM_CELLS;
   face_metric<T, SRC>(M, g, dir, r, c, f);
   FaceSpeed<T> sp;
   face_speed(dir, L, R, f, ph, sp);
@@ -605,9 +605,10 @@ __device__ __forceinline__ void flux_store(T (*Tl)[TY + 1][64], int w, int lane,
 constexpr double WET_SPEED = 1.0e-2;  // scaled depth (1 mm)
 
 // FLAGS: write the dry-tile flags of the output rows (only the planned stage kernels run
-// with dry-tile skipping on; the others are compiled without the flag code).  UNI (FP64
-// only): the grid's metric passed the uniform-metric check, so the inverse Jacobian is
-// formed from svec and cvalue instead of loaded (FM_UNIFORM; the same values bit for bit).
+// with dry-tile skipping on; the others are compiled without the flag code).  UNI: the
+// grid's metric passed the uniform-metric check, so the inverse Jacobian is formed from
+// svec and cvalue instead of loaded (FM_UNIFORM): in FP64 the same values bit for bit, in
+// FP32 formed in float from the rounded svec and cvalue.
 template <int IO, typename T, int TY, bool FLAGS, bool UNI = false>
 __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, const Phys& ph,
                                            const T* __restrict__ Uin, const T* Uzero, T* Uout,

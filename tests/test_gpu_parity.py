@@ -56,15 +56,16 @@ def test_metric_parity(case):
 
 @pytest.mark.parametrize("case", ["plane48", "hsiaolin", "df004", "fractal64"])
 def test_uniform_metric_form_is_taken(case):
-    """On the reference's uniform grids the FP64 step forms the inverse Jacobian from svec
-    and cvalue instead of loading it (its set-up check found the formed values equal to
-    the stored ones bit for bit); every parity test below runs that form."""
+    """On the reference's uniform grids the step forms the inverse Jacobian from svec and
+    cvalue instead of loading it (in FP64 its set-up check found the formed values equal to
+    the stored ones bit for bit; every parity test below runs that form); so does the FP32
+    variant, in float."""
     sc = {"plane48": lambda: S.landslide_plane(n=48), "hsiaolin": hsiaolin, "df004": df004,
           "fractal64": lambda: S.landslide_fractal(n=64, seed=3)}[case]()
     with Simulation(sc) as g:
         assert g.field("metric_uniform")[0, 0] == 1.0
     with Simulation(sc, precision="fp32") as g:
-        assert g.field("metric_uniform")[0, 0] == 0.0
+        assert g.field("metric_uniform")[0, 0] == 1.0
 
 
 def run_pair(sc, steps, chunks, label):

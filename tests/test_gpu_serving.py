@@ -30,6 +30,26 @@ def test_results_same_alone_and_in_the_mix(precision):
         assert np.array_equal(a.state, b.state), a.rid
 
 
+def test_fp64_mix_results_equal_the_oracle():
+    """In FP64 a request served in the mix ends bit-identical to the CPU oracle run of the
+    same scenario for the same number of steps (landslide and inflow requests)."""
+    import os
+    from oracle import oracle as O
+    from oracle.oracle import Oracle
+    O.threads(int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
+    reqs = small_requests()
+    srv = Server(precision="fp64", tick_steps=20, max_steps=200, vstop=0.01, keep_state=True,
+                 graph_steps=5)
+    mix = srv.run(reqs, realtime=False)
+    for mode in (0, 1):
+        k = next(i for i, r in enumerate(reqs) if r.mode == mode)
+        o = Oracle(reqs[k].scenario)
+        o.step(mix[k].steps)
+        uo = o.get("u")[:, 3:-3, 3:-3]
+        assert np.array_equal(mix[k].state[:, 3:-3, 3:-3], uo), (k, mode)
+        assert mix[k].time == o.scalar("time")
+
+
 def test_admission_keeps_footprint_within_budget():
     reqs = small_requests(8)
     peaks = [footprint(r.scenario.mode, r.n, r.n, "fp32",
