@@ -376,7 +376,7 @@ def dry_skip_run(sc, args, u_dense, dense_value) -> dict:
     import numpy as np
     from moses_2pdf_amd import Simulation
     with Simulation(sc, precision=args.precision, dry_skip=True) as sim:
-        sim.step(args.warmup)
+        sim.step(args.warmup + profiled_steps(args))   # the dense run's untimed steps
         sim.sync()
         t0 = time.perf_counter()
         sim.step(args.steps)
@@ -422,6 +422,11 @@ def shard_run(sc, args, dense_value, parts: int = 8) -> dict:
                     "tile computed; halo exchange and dt all-reduce not included"}
 
 
+def profiled_steps(args) -> int:
+    """Untimed steps after the warmup with HIP events around every kernel family."""
+    return min(args.steps, 20)
+
+
 def limiter(kernel: str, args) -> dict:
     """What holds `kernel` below its HBM roofline, from the committed PMC counters."""
     v = valu_from_profiles(kernel, args.n, args.precision)
@@ -452,7 +457,7 @@ def landslide(args) -> int:
     # kernel only (each event pair between two kernels costs a few microseconds)
     families = ("inflow", "ghost", "cfl", "dt", "stage0", "stage1", "vmax")
     sim.profile(True)
-    sim.step(min(args.steps, 20))
+    sim.step(profiled_steps(args))
     kern = {k: sim.profile_get(k) for k in ("ghost", "cfl", "dt", "stage0", "stage1")}
     totals = {k: v[0] * v[1] for k, v in kern.items()}
     dom = max(("cfl", "stage0", "stage1"), key=lambda k: totals[k])
