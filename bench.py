@@ -515,6 +515,7 @@ def landslide(args) -> int:
                              "path is a stencil, no MFMA); limiter names what the counters show "
                              "holds the kernel below it"},
         "step_hbm_alg_GBs": step_alg * args.steps / elapsed / 1e9,
+        "step_alg_frac": step_alg * args.steps / elapsed / 1e9 / HBM_PEAK_GBS,
         "kernels_ms": {k: v[0] for k, v in kern.items()},
         "kernels_ms_note": "each family timed with HIP events over 20 untimed steps; the "
                            "dominant kernel's figure in roofline comes from the timed steps",

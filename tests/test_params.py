@@ -96,3 +96,32 @@ def test_malformed_par_list_value_is_rejected(tmp_path, monkeypatch):
     with pytest.raises(E.M2PError) as e:
         E.Simulation(par_list=par, mode=0)
     assert "N_R" in str(e.value)
+
+
+@pytest.mark.parametrize("exe,mode", [("MoSES_2PDF_landslide", 0), ("MoSES_2PDF_inflow", 1)])
+def test_cli_reports_invalid_par_list_values(tmp_path, exe, mode):
+    # The executables print the reason and exit 1, before any device work, where the
+    # reference would run (or divide by zero) with the values it read.
+    import subprocess
+    from conftest import ROOT
+    path = os.path.join(ROOT, "moses_2pdf_amd", "bin", exe)
+    if not os.path.exists(path):
+        pytest.skip("executables not built")
+    if mode == 0:
+        sc = S.Scenario(mode=0, topo=np.zeros((8, 8)), depth=np.ones((8, 8)), cellsize=20.0,
+                        total_sim=10.0, each_out=1.0)
+    else:
+        sc = S.inflow_valley(nx=16, ny=16, nloc=4)
+    S.write_case(sc, str(tmp_path))
+    par = tmp_path / "par_List"
+    lines = par.read_text().split("\n")
+    good = list(lines)
+    lines[4] = "0.0   # value of N_R"
+    par.write_text("\n".join(lines))
+    r = subprocess.run([path], cwd=str(tmp_path), capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert "N_R" in r.stdout
+    par.write_text("\n".join(good))
+    r = subprocess.run([path, "cfl=fast"], cwd=str(tmp_path), capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 1 and "overlay" in r.stdout
