@@ -69,6 +69,13 @@ typedef struct m2p_params {
   int dry_skip;       /* 1 (default): tiles whose whole stencil is dry (every component of
                        * every cell zero) are skipped; their results are exactly those of
                        * computing them.  0: every tile is computed, as the reference does. */
+  int metric_form;    /* 0 (default): when a set-up check finds the inverse Jacobian formed
+                       * from svec and cvalue bit-identical to the stored metric planes
+                       * (every uniform grid), the step forms it instead of loading it.
+                       * 1: the step always loads the stored planes (what it does anyway
+                       * on a grid that fails the check).  FP64 results are the same bit
+                       * for bit either way; FP32 ones differ by float rounding.  No
+                       * reference counterpart (it loads the planes). */
 } m2p_params;
 
 /* Fills *p with the reference's values for `mode` (par_List values of the shipped
@@ -80,7 +87,8 @@ void m2p_default_params(int mode, m2p_params* p);
  * the files it names (relative paths resolve against the current directory, as in the
  * reference), parses them with the reference's precision rules and uploads them.
  * `overlay` is NULL or "key=value;key=value" over cfl, rho_ratio, dt_min, dt_max,
- * precision (fp64 | fp32) and dry_skip (0 | 1); a number must be the whole value. */
+ * precision (fp64 | fp32), dry_skip (0 | 1) and metric_form (0 | 1); a number must be the
+ * whole value. */
 int m2p_open(const char* par_list, int mode, const char* overlay, m2p_ctx** out);
 
 /* In-memory constructor (no files).  topo[NY*NX] and depth[NY*NX] hold the grid values

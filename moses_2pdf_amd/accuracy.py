@@ -51,17 +51,17 @@ def mid_step_time(sc, steps: int) -> float:
 
 
 def run_to_time(sc, t_end: float, max_steps: int, precision: str = "fp64", perturb: float = 0.0,
-                seed: int = 1):
+                seed: int = 1, metric_form: int = 0):
     """(state, mass, steps) of `sc` stepped with m2p_run_to to the scaled time t_end.
     perturb > 0: the initial state (an inflow run, which starts dry: the hydrograph depths)
-    carries that relative random perturbation first."""
+    carries that relative random perturbation first.  metric_form: m2p_params.metric_form."""
     import dataclasses
     from .engine import Simulation
     rng = np.random.default_rng(seed)
     if perturb and sc.mode == 1:
         sc = dataclasses.replace(
             sc, inflow_h=sc.inflow_h * (1 + perturb * rng.standard_normal(sc.inflow_h.shape)))
-    with Simulation(sc, precision=precision) as s:
+    with Simulation(sc, precision=precision, metric_form=metric_form) as s:
         if perturb and sc.mode == 0:
             u = s.state()
             s.set_state(u * (1 + perturb * rng.standard_normal(u.shape)))

@@ -257,8 +257,8 @@ int setup(m2p_ctx* c) {
   HIPCHK(c, hipGetLastError());
   // may the step form the inverse Jacobian instead of loading it?  (the mass buffer is
   // free during set-up and holds the flag)
-  int uni = 1;
-  {
+  int uni = c->par.metric_form == 0 ? 1 : 0;
+  if (uni) {
     int* ok = (int*)c->dMass;
     HIPCHK(c, hipMemcpyAsync(ok, &uni, sizeof uni, hipMemcpyHostToDevice, c->s));
     if (c->prec == M2P_FP64)
@@ -443,9 +443,9 @@ int parse_overlay(const char* ov, m2p_params& p) {
     size_t eq = kv.find('=');
     if (eq == std::string::npos) return -1;
     std::string k = kv.substr(0, eq), val = kv.substr(eq + 1);
-    if (k == "dry_skip") {
-      if (val == "0" || val == "1") p.dry_skip = val[0] - '0';
-      else return -1;
+    if (k == "dry_skip" || k == "metric_form") {
+      if (val != "0" && val != "1") return -1;
+      (k == "dry_skip" ? p.dry_skip : p.metric_form) = val[0] - '0';
       continue;
     }
     if (k == "precision") {
@@ -484,6 +484,7 @@ std::string check_params(int mode, const m2p_params& p) {
   if (!fin(p.dt_max) || p.dt_max < p.dt_min) return "dt_max must be finite and >= dt_min";
   if (p.precision != M2P_FP64 && p.precision != M2P_FP32) return "precision must be M2P_FP64 or M2P_FP32";
   if (p.dry_skip != 0 && p.dry_skip != 1) return "dry_skip must be 0 or 1";
+  if (p.metric_form != 0 && p.metric_form != 1) return "metric_form must be 0 or 1";
   return "";
 }
 
@@ -536,6 +537,7 @@ void m2p_default_params(int mode, m2p_params* p) {
   p->dt_max = 2.0;
   p->precision = M2P_FP64;
   p->dry_skip = 1;
+  p->metric_form = 0;
 }
 
 int m2p_open_grid(int mode, int NX, int NY, double cellsize, const double* topo,

@@ -42,7 +42,8 @@ class Params(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
                 ("total_sim", "each_out", "delta0", "Cd", "N_R", "varTheta", "phiS0", "cfl",
                  "rho_ratio", "dt_min", "dt_max")] + [("precision", ctypes.c_int),
-                                                       ("dry_skip", ctypes.c_int)]
+                                                       ("dry_skip", ctypes.c_int),
+                                                       ("metric_form", ctypes.c_int)]
 
 
 def lib():
@@ -127,21 +128,27 @@ class Simulation:
     """One scenario resident on the GPU.  ``precision`` is "fp64" (the reference's
     arithmetic) or "fp32" (FP32-bulk: float state and per-step arithmetic, FP64 metric
     setup, CFL max, dt control and mass totals).  ``dry_skip`` leaves out tiles whose
-    inputs are all zero (the results are the same; see m2p_params.dry_skip)."""
+    inputs are all zero (the results are the same; see m2p_params.dry_skip).
+    ``metric_form`` 1 makes the step load the stored metric planes even where the set-up
+    check allows forming them (m2p_params.metric_form)."""
 
     def __init__(self, sc=None, *, par_list: Optional[str] = None, mode: Optional[int] = None,
-                 overlay: Optional[str] = None, precision: str = "fp64", dry_skip: bool = True):
+                 overlay: Optional[str] = None, precision: str = "fp64", dry_skip: bool = True,
+                 metric_form: int = 0):
         L = lib()
         self._h = ctypes.c_void_p()
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
         self.precision = precision
         self._dry_skip = bool(dry_skip)
+        self._metric_form = int(metric_form)
         if par_list is not None:
             if precision != "fp64":
                 overlay = ((overlay + ";") if overlay else "") + "precision=" + precision
             if not dry_skip:
                 overlay = ((overlay + ";") if overlay else "") + "dry_skip=0"
+            if self._metric_form != 0:
+                overlay = ((overlay + ";") if overlay else "") + f"metric_form={self._metric_form}"
             rc = L.m2p_open(par_list.encode(), int(mode), overlay.encode() if overlay else None,
                             ctypes.byref(self._h))
         else:
@@ -162,6 +169,7 @@ class Simulation:
         p.cfl, p.rho_ratio, p.dt_min, p.dt_max = sc.cfl, sc.rho_ratio, sc.dt_floor(), sc.dt_max
         p.precision = PRECISIONS[self.precision]
         p.dry_skip = 1 if self._dry_skip else 0
+        p.metric_form = self._metric_form
         f32 = lambda a: None if a is None else np.ascontiguousarray(
             np.asarray(a, np.float64).astype(np.float32).astype(np.float64))
         topo = f32(sc.topo)

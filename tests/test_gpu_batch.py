@@ -32,17 +32,17 @@ def population():
     return mem
 
 
-def alone(scs, steps, precision):
+def alone(scs, steps, precision, metric_form=0):
     out = []
     for sc in scs:
-        with Simulation(sc, precision=precision) as s:
+        with Simulation(sc, precision=precision, metric_form=metric_form) as s:
             s.step(steps)
             out.append((s.state(), s.time_steps()))
     return out
 
 
-def batched(scs, steps, precision, chunks=None, setup=None):
-    sims = [Simulation(sc, precision=precision) for sc in scs]
+def batched(scs, steps, precision, chunks=None, setup=None, metric_form=0):
+    sims = [Simulation(sc, precision=precision, metric_form=metric_form) for sc in scs]
     b = Batch(precision)
     for s in sims:
         b.add(s)
@@ -71,6 +71,22 @@ def test_batch_equals_alone(precision):
         assert x[1] == y[1], (k, x[1], y[1])
         assert np.array_equal(x[0], y[0]), k
     say(precision, "batch of", len(scs), "equal to alone after 37 steps")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_with_stored_metric_equals_alone(precision):
+    """Members that load the stored metric planes (m2p_params.metric_form = 1) make the
+    batch take its stored-metric kernels; every member still ends equal to itself stepped
+    alone, and in FP64 equal to the formed-metric run as well."""
+    scs = population()
+    ref = alone(scs, 37, precision, metric_form=1)
+    got = batched(scs, 37, precision, chunks=[1, 20, 16], metric_form=1)
+    for k, (x, y) in enumerate(zip(got, ref)):
+        assert x[1] == y[1], (k, x[1], y[1])
+        assert np.array_equal(x[0], y[0]), k
+    if precision == "fp64":
+        assert same(got, alone(scs, 37, precision))
+    say(precision, "stored-metric batch of", len(scs), "equal to alone after 37 steps")
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])

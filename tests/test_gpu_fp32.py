@@ -47,8 +47,8 @@ def fp64_to(sc, t_end, max_steps):
     return run_to_time(sc, t_end, max_steps)
 
 
-def fp32_to(sc, t_end, max_steps):
-    return run_to_time(sc, t_end, max_steps, "fp32")
+def fp32_to(sc, t_end, max_steps, metric_form=0):
+    return run_to_time(sc, t_end, max_steps, "fp32", metric_form=metric_form)
 
 
 def test_fp32_initial_state_is_rounded_fp64():
@@ -68,6 +68,23 @@ def test_fp32_short_horizon(case):
     e = depth_error(u32, u64)
     say(f"{case}: {n64} steps (FP32 {n}) L1 {e['l1']:.2e} Linf {e['linf']:.2e} "
         f"mass {mass_rel(m32, m64):.2e}")
+    assert n64 == steps and n == steps
+    assert np.isfinite(u32).all()
+    assert e["l1"] <= 1e-4, e
+    assert mass_rel(m32, m64) <= 1e-6
+
+
+@pytest.mark.parametrize("case", ["hsiaolin", "df004"])
+def test_fp32_stored_metric_short_horizon(case):
+    """The FP32 step with the stored metric planes (m2p_params.metric_form = 1: float face
+    averages of the rounded cell planes) against FP64, to the short-horizon bounds."""
+    sc = CASES[case]()
+    steps = 10 if case == "df004" else 30
+    t = mid_step_time(sc, steps)
+    u64, m64, n64 = fp64_to(sc, t, 2 * steps)
+    u32, m32, n = fp32_to(sc, t, 2 * steps, metric_form=1)
+    e = depth_error(u32, u64)
+    say(f"{case} stored metric: {n64} steps (FP32 {n}) L1 {e['l1']:.2e} mass {mass_rel(m32, m64):.2e}")
     assert n64 == steps and n == steps
     assert np.isfinite(u32).all()
     assert e["l1"] <= 1e-4, e

@@ -68,10 +68,10 @@ def test_uniform_metric_form_is_taken(case):
         assert g.field("metric_uniform")[0, 0] == 1.0
 
 
-def run_pair(sc, steps, chunks, label):
+def run_pair(sc, steps, chunks, label, **kw):
     o = Oracle(sc)
     det = o.get("Detmin")
-    with Simulation(sc) as g:
+    with Simulation(sc, **kw) as g:
         done = 0
         for n in chunks:
             o.step(n)
@@ -92,6 +92,35 @@ def run_pair(sc, steps, chunks, label):
         assert ntg == done
         assert tg == o.scalar("time")
     return ndiff
+
+
+@pytest.mark.parametrize("case,chunks", [("plane48", [1, 9, 30]), ("hsiaolin", [1, 4, 25]),
+                                         ("df004", [1, 9, 50]), ("fractal64", [2, 28])])
+def test_stored_metric_form_matches_oracle(case, chunks):
+    """metric_form = 1: the step loads the stored inverse Jacobian and coordinate-derivative
+    averages, as it does on any grid that fails the uniform-metric check.  That path gives
+    the oracle's states, dt and mass as well."""
+    sc = {"plane48": lambda: S.landslide_plane(n=48), "hsiaolin": hsiaolin, "df004": df004,
+          "fractal64": lambda: S.landslide_fractal(n=64, seed=5)}[case]()
+    with Simulation(sc, metric_form=1) as g:
+        assert g.field("metric_uniform")[0, 0] == 0.0
+    run_pair(sc, sum(chunks), chunks, case + "-stored-metric", metric_form=1)
+
+
+def test_metric_forms_agree_bit_for_bit():
+    """FP64 on a 160x96 fractal DEM: the formed metric (with dry-tile skipping) and the
+    loaded one (every tile computed) give the same states, dt and times after 150 steps."""
+    topo = S.fractal_dem(160, 96, 400.0, 0.8, 11)
+    depth = S.paraboloid(160, 96, 20.0, 1600.0, 960.0, 1.0e6, 300.0)
+    sc = S.Scenario(mode=0, topo=topo, depth=depth, cellsize=20.0, phiS0=0.5, delta0=20.0)
+    with Simulation(sc) as a, Simulation(sc, metric_form=1, dry_skip=False) as b:
+        assert a.field("metric_uniform")[0, 0] == 1.0 and b.field("metric_uniform")[0, 0] == 0.0
+        for n in (1, 49, 100):
+            a.step(n)
+            b.step(n)
+            assert a.scalars() == b.scalars()
+            assert np.array_equal(a.state(), b.state())
+        assert a.time_steps() == b.time_steps()
 
 
 def test_parity_plane_small():

@@ -50,6 +50,7 @@ def _open_grid(mode=0, **over):
     ("phiS0", 1.5, "phiS0"), ("phiS0", -0.1, "phiS0"),
     ("each_out", 0.0, "each_out"), ("total_sim", -1.0, "total_sim"),
     ("precision", 7, "precision"), ("dry_skip", 2, "dry_skip"),
+    ("metric_form", 2, "metric_form"), ("metric_form", -1, "metric_form"),
 ])
 def test_out_of_range_parameters_are_rejected(field, value, word):
     rc, msg = _open_grid(0, **{field: value})
@@ -71,7 +72,8 @@ def _case(tmp_path, monkeypatch):
 
 @pytest.mark.parametrize("overlay", ["cfl=abc", "cfl=0.4x", "cfl=", "dt_min=nan",
                                      "dt_max=inf", "rho_ratio=1e", "unknown=1", "cfl",
-                                     "precision=fp16", "dry_skip=yes"])
+                                     "precision=fp16", "dry_skip=yes", "metric_form=2",
+                                     "metric_form=stored"])
 def test_malformed_overlay_is_rejected(tmp_path, monkeypatch, overlay):
     par = _case(tmp_path, monkeypatch)
     with pytest.raises(E.M2PError) as e:
