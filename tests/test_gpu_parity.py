@@ -123,6 +123,23 @@ def test_metric_forms_agree_bit_for_bit():
         assert a.time_steps() == b.time_steps()
 
 
+@pytest.mark.parametrize("nx,ny", [(2, 2), (3, 5), (5, 3), (7, 70), (64, 9), (65, 66),
+                                   (127, 4), (130, 129)])
+def test_parity_odd_grid_shapes(nx, ny):
+    """Grids narrower than a stage tile or CFL box, and widths and heights one past a tile
+    or box multiple: partial tiles, boxes and row segments at every edge, a release that
+    reaches the ghost ring."""
+    dx = 20.0
+    r = max(0.35 * min(nx, ny) * dx, 1.6 * dx)
+    depth = S.paraboloid(nx, ny, dx, 0.5 * (nx - 1) * dx, 0.5 * (ny - 1) * dx,
+                         10.0 * math.pi * r * r / 2.0, r)
+    topo = S.inclined_plane(nx, ny, dx, angle_deg=25.0, x_break=0.6 * nx * dx,
+                            blend=0.2 * nx * dx) + S.gaussian_bump(nx, ny, dx, 0.3 * nx * dx,
+                                                                    0.7 * ny * dx, 8.0, 3 * dx)
+    sc = S.Scenario(mode=0, topo=topo, depth=depth, cellsize=dx, phiS0=0.55, delta0=22.0)
+    run_pair(sc, 25, [1, 4, 20], f"odd{nx}x{ny}")
+
+
 def test_parity_plane_small():
     sc = S.landslide_plane(n=48)
     assert run_pair(sc, 60, [1, 1, 3, 15, 40], "plane48") == 0
