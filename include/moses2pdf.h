@@ -46,13 +46,14 @@ enum {
 };
 
 /* Run parameters.  The first seven are par_List lines 1-7 (adNOC_2Pb_runKernels.cu:21-40);
- * the last four are constants the reference hard-codes (adNOC_2Pb_kernels.cuh:16,
- * adNOC_2Pb_kernels.cu:1710 and 2014-2017; inflow 2060) and default to those values.
- * The reference reads the first seven with sscanf and checks nothing; m2p_open and
- * m2p_open_grid return M2P_EINVAL with the reason in m2p_last_error unless every value is
- * finite and total_sim >= 0, each_out > 0, 0 <= delta0 < 90, Cd >= 0, N_R > 0,
- * varTheta >= 0, 0 <= phiS0 <= 1 (inflow: 0 < phiS0 <= 1, its injection divides by it),
- * 0 < cfl <= 1, 0 <= rho_ratio < 1 and 0 < dt_min <= dt_max. */
+ * the next four (cfl .. dt_max) are constants the reference hard-codes
+ * (adNOC_2Pb_kernels.cuh:16, adNOC_2Pb_kernels.cu:1710 and 2014-2017; inflow 2060) and
+ * default to those values; the last three choose how the step is computed.  The reference
+ * reads the first seven with sscanf and checks nothing; m2p_open and m2p_open_grid return
+ * M2P_EINVAL with the reason in m2p_last_error unless every value is finite and
+ * total_sim >= 0, each_out > 0, 0 <= delta0 < 90, Cd >= 0, N_R > 0, varTheta >= 0,
+ * 0 <= phiS0 <= 1 (inflow: 0 < phiS0 <= 1, its injection divides by it), 0 < cfl <= 1,
+ * 0 <= rho_ratio < 1, 0 < dt_min <= dt_max, and the three ints take the values listed. */
 typedef struct m2p_params {
   double total_sim;   /* seconds */
   double each_out;    /* seconds */
@@ -156,8 +157,8 @@ int m2p_scalars(m2p_ctx* ctx, double* dt, double* dmax, int* iter);
  * "metric_uniform", one value: 1 when the step forms the inverse Jacobian from svec and
  * cvalue instead of loading it (the grid's coordinate derivatives are exactly the identity;
  * in FP64 a set-up check found the formed values bit-identical to the stored ones wherever
- * the step reads them, in FP32 they are formed in float), else 0.  Returns planes copied,
- * or a negative code. */
+ * the step reads them, in FP32 they are formed in float), else 0 (also when
+ * m2p_params.metric_form is 1).  Returns planes copied, or a negative code. */
 int m2p_get_field(m2p_ctx* ctx, const char* name, double* out);
 
 /* Mean device time per launch of each kernel family since the last reset, measured with
@@ -200,15 +201,16 @@ int m2p_mass_noalloc(const void* u, int precision, const double* detmin, int nx,
  * renamed): a header (magic "M2PCKPT1", version 2, mode, NX, NY, precision, output-time
  * count, P, and an FNV-1a hash of the padded topography, the run parameters and the
  * hydrograph tables), the device loop state (time, dt, output and inflow counters, step
- * count, halting rules, non-finite flag), the Time.dat history of m2p_run, the 7 state planes of u^n and of the
- * half-step buffer (its ghost rows feed the next step's landslide corners), the FP32
- * limiter flags, and an FNV-1a checksum.  Stepping a context restored from it gives
- * bit-identical results to never having stopped. */
+ * count, halting rules, non-finite flag), the Time.dat history of m2p_run, the 7 state
+ * planes of u^n and of the half-step buffer (its ghost rows feed the next step's
+ * landslide corners), the FP32 limiter flags, and an FNV-1a checksum.  Stepping a context
+ * restored from it gives bit-identical results to never having stopped. */
 int m2p_checkpoint(m2p_ctx* ctx, const char* path);
 
 /* Loads a checkpoint into a context opened on the same scenario (mode, grid, precision,
  * output times, topography, parameters and hydrograph tables must match; M2P_EINVAL
- * otherwise, with the reason in m2p_last_error).  m2p_run then continues the output sequence where it stopped. */
+ * otherwise, with the reason in m2p_last_error).  m2p_run then continues the output
+ * sequence where it stopped. */
 int m2p_restore(m2p_ctx* ctx, const char* path);
 
 /* ---- Halting rules and status (no reference counterpart) -------------------------- */
