@@ -83,20 +83,52 @@ __device__ __forceinline__ T tvd_slope(T um, T u0, T up) {
 // (component 6 of a face state is always replaced by KeepPositivi2Kernel).
 //   dir 0: x-face between (r,c) and (r,c+1): L = W state, R = E state.
 //   dir 1: y-face between (r,c) and (r+1,c): L = S state, R = N state.
+// A face's loads are split from its arithmetic: the four cells of each component are
+// loaded first (load_stencil), and __builtin_amdgcn_sched_barrier(0) keeps the compiler's
+// scheduler from moving the arithmetic above the loads.  Left to itself, the FP64 stage
+// kernel's scheduler (it keeps the kernel at 128 VGPRs) waits for each component's loads
+// before issuing the next component's, six dependent memory round trips per face instead of
+// one.  Callers that load more for the same face (its metric, the cell's phi) issue those
+// loads before the barrier too.
 template <typename T>
-__device__ __forceinline__ void base_states(const T* __restrict__ U, const Grid& g, int dir, int r,
-                                            int c, T L[6], T R[6]) {
+struct Stencil {
+  T v[6][4];   // components 0..5 at i - st, i, i + st, i + 2 st
+};
+
+template <typename T>
+__device__ __forceinline__ void load_stencil(const T* __restrict__ U, const Grid& g, int dir, int r,
+                                             int c, Stencil<T>& s) {
   const long long st = dir == 0 ? 1 : g.nxd;
   const long long i = idx(g, r, c);
 #pragma unroll
   for (int k = 0; k < 6; k++) {
     const T* a = U + k * g.P;
-    const T um = a[i - st], u0 = a[i], u1 = a[i + st], u2 = a[i + 2 * st];
+    s.v[k][0] = a[i - st];
+    s.v[k][1] = a[i];
+    s.v[k][2] = a[i + st];
+    s.v[k][3] = a[i + 2 * st];
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void stencil_states(const Stencil<T>& s, T L[6], T R[6]) {
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const T um = s.v[k][0], u0 = s.v[k][1], u1 = s.v[k][2], u2 = s.v[k][3];
     const T s0 = tvd_slope(um, u0, u1);
     const T s1 = tvd_slope(u0, u1, u2);
     L[k] = u0 + T(0.5) * s0;
     R[k] = u1 - T(0.5) * s1;
   }
+}
+
+template <typename T>
+__device__ __forceinline__ void base_states(const T* __restrict__ U, const Grid& g, int dir, int r,
+                                            int c, T L[6], T R[6]) {
+  Stencil<T> s;
+  load_stencil(U, g, dir, r, c, s);
+  __builtin_amdgcn_sched_barrier(0);
+  stencil_states(s, L, R);
 }
 
 // Face states after Interfaces2Kernel (L/K:1254-1337), taken as: all x-ghost overrides
@@ -168,11 +200,6 @@ struct FaceMet {
   T sL0, sL1, cL;   // of the W / S cell
 };
 
-template <typename T>
-__device__ __forceinline__ T avg(const T* a, long long i, long long j) {
-  return T(0.5) * (a[i] + a[j]);
-}
-
 // The inverse Jacobian of makeTopo9Kernel (L/K:210-252) at a cell of a grid whose
 // coordinate derivatives are exactly the identity (dxdxi11 = dxdxi22 = 1, dxdxi12 =
 // dxdxi21 = +0, as makeTopo4Kernel gives them on the reference's uniform grid): then
@@ -211,33 +238,64 @@ __device__ __forceinline__ InvJ<T> invj_uniform(T s0, T s1, T c) {
 enum FaceMetSrc { FM_CELLS = 0, FM_STORED = 1, FM_UNIFORM = 2 };
 
 // dir 0: face (r,c)-(r,c+1); dir 1: face (r,c)-(r+1,c).  Valid for faces in [1,N-3]^2,
-// where MeanKernel writes the averages.
-template <typename T, int SRC = FM_STORED>
-__device__ __forceinline__ void face_metric(const T* __restrict__ M, const Grid& g, int dir, int r,
-                                            int c, FaceMet<T>& f) {
+// where MeanKernel writes the averages.  face_metric_load issues the loads (raw plane
+// values), face_metric_form does the arithmetic, so that a caller can issue a face's metric
+// loads together with its state loads.
+template <typename T, int SRC>
+struct FaceMetIn {
+  T sL0, sL1, cL, sR0, sR1, cR;
+  // FM_STORED: the 9 face planes (FaceQ order); FM_UNIFORM: the Detmin average;
+  // FM_CELLS: the cell planes at (r,c) and the neighbour, in FaceQ order, interleaved
+  T a[SRC == FM_CELLS ? 2 * FQ_COUNT : SRC == FM_STORED ? FQ_COUNT : 1];
+};
+
+template <typename T, int SRC>
+__device__ __forceinline__ void face_metric_load(const T* __restrict__ M, const Grid& g, int dir,
+                                                 int r, int c, FaceMetIn<T, SRC>& m) {
   const long long i = idx(g, r, c);
   const long long j = i + (dir == 0 ? 1 : g.nxd);
   const long long P = g.P;
-  const T* C = M + MC_C * P;
-  const T* S0 = M + MC_S0 * P;
-  const T* S1 = M + MC_S1 * P;
   const T* Fq = M + (dir == 0 ? MC_FX : MC_FY) * P;   // the stored face averages
-  f.sL0 = S0[i];
-  f.sL1 = S1[i];
-  f.cL = C[i];
-  f.sR0 = S0[j];
-  f.sR1 = S1[j];
-  f.cR = C[j];
+  m.sL0 = M[MC_S0 * P + i];
+  m.sL1 = M[MC_S1 * P + i];
+  m.cL = M[MC_C * P + i];
+  m.sR0 = M[MC_S0 * P + j];
+  m.sR1 = M[MC_S1 * P + j];
+  m.cR = M[MC_C * P + j];
   if constexpr (SRC == FM_STORED) {
-    f.j11 = Fq[FQ_I11 * P + i];
-    f.j12 = Fq[FQ_I12 * P + i];
-    f.j13 = Fq[FQ_I13 * P + i];
-    f.j21 = Fq[FQ_I21 * P + i];
-    f.j22 = Fq[FQ_I22 * P + i];
-    f.j23 = Fq[FQ_I23 * P + i];
-    f.det = Fq[FQ_DET * P + i];
-    f.xa = Fq[FQ_XA * P + i];
-    f.xb = Fq[FQ_XB * P + i];
+#pragma unroll
+    for (int q = 0; q < FQ_COUNT; q++) m.a[q] = Fq[q * P + i];
+  } else if constexpr (SRC == FM_UNIFORM) {
+    m.a[0] = Fq[FQ_DET * P + i];
+  } else {
+    const int pl[FQ_COUNT] = {MC_I11, MC_I12, MC_I13, MC_I21, MC_I22, MC_I23, MC_DET,
+                              dir == 0 ? MC_X11 : MC_X12, dir == 0 ? MC_X21 : MC_X22};
+#pragma unroll
+    for (int q = 0; q < FQ_COUNT; q++) {
+      m.a[2 * q] = M[pl[q] * P + i];
+      m.a[2 * q + 1] = M[pl[q] * P + j];
+    }
+  }
+}
+
+template <typename T, int SRC>
+__device__ __forceinline__ void face_metric_form(const FaceMetIn<T, SRC>& m, int dir, FaceMet<T>& f) {
+  f.sL0 = m.sL0;
+  f.sL1 = m.sL1;
+  f.cL = m.cL;
+  f.sR0 = m.sR0;
+  f.sR1 = m.sR1;
+  f.cR = m.cR;
+  if constexpr (SRC == FM_STORED) {
+    f.j11 = m.a[FQ_I11];
+    f.j12 = m.a[FQ_I12];
+    f.j13 = m.a[FQ_I13];
+    f.j21 = m.a[FQ_I21];
+    f.j22 = m.a[FQ_I22];
+    f.j23 = m.a[FQ_I23];
+    f.det = m.a[FQ_DET];
+    f.xa = m.a[FQ_XA];
+    f.xb = m.a[FQ_XB];
   } else if constexpr (SRC == FM_UNIFORM) {
     const InvJ<T> a = invj_uniform(f.sL0, f.sL1, f.cL), b = invj_uniform(f.sR0, f.sR1, f.cR);
     f.j11 = T(0.5) * (a.j11 + b.j11);
@@ -246,24 +304,33 @@ __device__ __forceinline__ void face_metric(const T* __restrict__ M, const Grid&
     f.j21 = T(0.5) * (a.j21 + b.j21);
     f.j22 = T(0.5) * (a.j22 + b.j22);
     f.j23 = T(0.5) * (a.j23 + b.j23);
-    f.det = Fq[FQ_DET * P + i];
+    f.det = m.a[0];
     f.xa = dir == 0 ? T(1.0) : T(0.0);   // dxdxi11 avg (x) / dxdxi12 avg (y)
     f.xb = dir == 0 ? T(0.0) : T(1.0);   // dxdxi21 avg (x) / dxdxi22 avg (y)
   } else {
-    f.j11 = avg(M + MC_I11 * P, i, j);
-    f.j12 = avg(M + MC_I12 * P, i, j);
-    f.j13 = avg(M + MC_I13 * P, i, j);
-    f.j21 = avg(M + MC_I21 * P, i, j);
-    f.j22 = avg(M + MC_I22 * P, i, j);
-    f.j23 = avg(M + MC_I23 * P, i, j);
-    f.det = avg(M + MC_DET * P, i, j);
-    f.xa = avg(M + (dir == 0 ? MC_X11 : MC_X12) * P, i, j);
-    f.xb = avg(M + (dir == 0 ? MC_X21 : MC_X22) * P, i, j);
+    // MeanKernel's average 0.5 * (a[i] + a[j]) of each cell plane
+    f.j11 = T(0.5) * (m.a[2 * FQ_I11] + m.a[2 * FQ_I11 + 1]);
+    f.j12 = T(0.5) * (m.a[2 * FQ_I12] + m.a[2 * FQ_I12 + 1]);
+    f.j13 = T(0.5) * (m.a[2 * FQ_I13] + m.a[2 * FQ_I13 + 1]);
+    f.j21 = T(0.5) * (m.a[2 * FQ_I21] + m.a[2 * FQ_I21 + 1]);
+    f.j22 = T(0.5) * (m.a[2 * FQ_I22] + m.a[2 * FQ_I22 + 1]);
+    f.j23 = T(0.5) * (m.a[2 * FQ_I23] + m.a[2 * FQ_I23 + 1]);
+    f.det = T(0.5) * (m.a[2 * FQ_DET] + m.a[2 * FQ_DET + 1]);
+    f.xa = T(0.5) * (m.a[2 * FQ_XA] + m.a[2 * FQ_XA + 1]);
+    f.xb = T(0.5) * (m.a[2 * FQ_XB] + m.a[2 * FQ_XB + 1]);
   }
   f.j31 = T(0.5) * ((-f.sL0) + (-f.sR0));
   f.j32 = T(0.5) * ((-f.sL1) + (-f.sR1));
   f.j33 = T(0.5) * (f.cL + f.cR);
   f.cav = f.j33;
+}
+
+template <typename T, int SRC = FM_STORED>
+__device__ __forceinline__ void face_metric(const T* __restrict__ M, const Grid& g, int dir, int r,
+                                            int c, FaceMet<T>& f) {
+  FaceMetIn<T, SRC> m;
+  face_metric_load(M, g, dir, r, c, m);
+  face_metric_form(m, dir, f);
 }
 
 // Wave speeds of one face (Flux1..6Kernel, L/K:1426-1848): em for the solid (emS) and the
@@ -423,22 +490,15 @@ struct CellMet {
 };
 
 // UNI: the inverse Jacobian from svec and cvalue (invj_uniform) instead of its planes.
+// cell_metric_load issues the loads, cell_metric_form the arithmetic (as for faces).
 template <typename T, bool UNI = false>
-__device__ __forceinline__ void cell_metric(const T* __restrict__ M, long long P, long long i,
-                                            CellMet<T>& m) {
+__device__ __forceinline__ void cell_metric_load(const T* __restrict__ M, long long P, long long i,
+                                                 CellMet<T>& m) {
   m.c = M[MC_C * P + i];
   m.s0 = M[MC_S0 * P + i];
   m.s1 = M[MC_S1 * P + i];
   m.det = M[MC_DET * P + i];
-  if constexpr (UNI) {
-    const InvJ<T> v = invj_uniform(m.s0, m.s1, m.c);
-    m.i11 = v.j11;
-    m.i12 = v.j12;
-    m.i13 = v.j13;
-    m.i21 = v.j21;
-    m.i22 = v.j22;
-    m.i23 = v.j23;
-  } else {
+  if constexpr (!UNI) {
     m.i11 = M[MC_I11 * P + i];
     m.i12 = M[MC_I12 * P + i];
     m.i13 = M[MC_I13 * P + i];
@@ -452,6 +512,26 @@ __device__ __forceinline__ void cell_metric(const T* __restrict__ M, long long P
   m.j13e = M[MC_J13E * P + i];
   m.j23e = M[MC_J23E * P + i];
   m.j33e = M[MC_J33E * P + i];
+}
+
+template <typename T, bool UNI = false>
+__device__ __forceinline__ void cell_metric_form(CellMet<T>& m) {
+  if constexpr (UNI) {
+    const InvJ<T> v = invj_uniform(m.s0, m.s1, m.c);
+    m.i11 = v.j11;
+    m.i12 = v.j12;
+    m.i13 = v.j13;
+    m.i21 = v.j21;
+    m.i22 = v.j22;
+    m.i23 = v.j23;
+  }
+}
+
+template <typename T, bool UNI = false>
+__device__ __forceinline__ void cell_metric(const T* __restrict__ M, long long P, long long i,
+                                            CellMet<T>& m) {
+  cell_metric_load<T, UNI>(M, P, i, m);
+  cell_metric_form<T, UNI>(m);
 }
 
 // Flux11..17Kernel (L/K:2349-2621): the source vector s (components 1,2,4,5; s0 = s3 = 0).

@@ -184,12 +184,20 @@ __device__ __forceinline__ double face_cfl(const T* __restrict__ U, const T* __r
                                            const Grid& g, const Phys& ph, int dir, int r, int c,
                                            bool edge) {
   T L[6], R[6];
-  if (edge)
+  constexpr int SRC = UNI ? FM_UNIFORM : FM_STORED;
+  FaceMetIn<T, SRC> fm;
+  if (edge) {
     final_states_edge(U, g, dir, r, c, L, R);
-  else
-    base_states(U, g, dir, r, c, L, R);
+    face_metric_load(M, g, dir, r, c, fm);
+  } else {   // the face's metric and state loads issued together
+    Stencil<T> sn;
+    face_metric_load(M, g, dir, r, c, fm);
+    load_stencil(U, g, dir, r, c, sn);
+    __builtin_amdgcn_sched_barrier(0);
+    stencil_states(sn, L, R);
+  }
   FaceMet<T> f;
-  face_metric<T, UNI ? FM_UNIFORM : FM_STORED>(M, g, dir, r, c, f);
+  face_metric_form(fm, dir, f);
   FaceSpeed<T> s;
   face_speed(dir, L, R, f, ph, s);
   const double v = fmax((double)s.emS, (double)s.emF);
@@ -576,16 +584,21 @@ __device__ __forceinline__ void load_states(T (*Tl)[TY + 1][64], int w, int lane
   }
 }
 
-// Phase C: fluxes of one face from its final states, into the tile.
+// Where a stage takes its face metric from (face_metric).
+template <typename T, bool UNI>
+constexpr int stage_fm_src() {
+  return UNI ? FM_UNIFORM : std::is_same<T, double>::value ? FM_STORED : FM_CELLS;
+}
+
+// Phase C: fluxes of one face from its final states and its loaded metric, into the tile.
 // pl, pr: the viscous quotients of the face's left/lower and right/upper cells.
-template <typename T, int TY, bool UNI>
+template <typename T, int TY, int SRC>
 __device__ __forceinline__ void flux_store(T (*Tl)[TY + 1][64], int w, int lane,
-                                           const T* __restrict__ M, const Grid& g,
-                                           const Phys& ph, int dir, int r, int c, T L[6], T R[6],
+                                           const FaceMetIn<T, SRC>& fm, const Grid& g,
+                                           const Phys& ph, int dir, T L[6], T R[6],
                                            const PDSide<T>& pl, const PDSide<T>& pr) {
   FaceMet<T> f;
-  constexpr int SRC = UNI ? FM_UNIFORM : std::is_same<T, double>::value ? FM_STORED : FM_CELLS;
-  face_metric<T, SRC>(M, g, dir, r, c, f);
+  face_metric_form(fm, dir, f);
   FaceSpeed<T> sp;
   face_speed(dir, L, R, f, ph, sp);
   FaceFlux<T> o;
@@ -637,38 +650,51 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
   }
   __syncthreads();
 
+  // this thread's cell; its phi is loaded now, with the x-phase's loads
+  const int r = r0 + w, c = c0 + lane;
+  const bool active = (w < TY) && (lane < ST_TX) && r <= ymax && c <= xmax;
+  const long long i = idx(g, r, c);
+  T phi = T(0);
+  if (active) phi = Uin[6 * P + i];
+
   // x faces (r0+w, c0-1+lane), faces (r, c)-(r, c+1).  Only tiles in the first tile column
   // hold the overridden face at column 2; the others form their face states in registers
-  // and skip the LDS round trip (the branch is uniform over the block).
+  // and skip the LDS round trip (the branch is uniform over the block).  A face's metric
+  // and state loads are issued together, ahead of its arithmetic.
+  constexpr int SRC = stage_fm_src<T, UNI>();
   {
     const int r = r0 + w, c = c0 - 1 + lane;
     const bool has = (w < TY && r <= ymax && c <= xmax);
     T L[6], R[6];
+    FaceMetIn<T, SRC> fm;
     if (tx == 0) {
       if (has) stage_states<T, TY>(F, w, lane, Uin, g, 0, r, c);
       __syncthreads();
-      if (has) load_states<T, TY>(F, w, lane, c == 2, 0, L, R);
+      if (has) {
+        face_metric_load(M, g, 0, r, c, fm);
+        load_states<T, TY>(F, w, lane, c == 2, 0, L, R);
+      }
       __syncthreads();
     } else if (has) {
-      base_states(Uin, g, 0, r, c, L, R);
+      Stencil<T> sn;
+      face_metric_load(M, g, 0, r, c, fm);
+      load_stencil(Uin, g, 0, r, c, sn);
+      __builtin_amdgcn_sched_barrier(0);
+      stencil_states(sn, L, R);
     }
-    if (has) flux_store<T, TY, UNI>(F, w, lane, M, g, ph, 0, r, c, L, R, Q[w * 65 + lane], Q[w * 65 + lane + 1]);
+    if (has) flux_store<T, TY, SRC>(F, w, lane, fm, g, ph, 0, L, R, Q[w * 65 + lane], Q[w * 65 + lane + 1]);
   }
   __syncthreads();
 
-  const int r = r0 + w, c = c0 + lane;
-  const bool active = (w < TY) && (lane < ST_TX) && r <= ymax && c <= xmax;
-  const long long i = idx(g, r, c);
   const T dt = T(dt_d);
   // FP64: dt/dx and dt/dy as the dt kernel formed them (the same IEEE quotients)
   constexpr bool F64 = std::is_same<T, double>::value;
   const T dtdx = F64 ? T(dtdx_d) : qdiv(dt, T(g.dx));
   const T dtdy = F64 ? T(dtdy_d) : qdiv(dt, T(g.dy));
   const T epsilon0 = T(1.0);
-  T phi = T(0), fP = T(0), fD = T(0);
+  T fP = T(0), fD = T(0);
   T tH[6], tP[4], tD[2];
   if (active) {
-    phi = Uin[6 * P + i];
     fP = epsilon0 * phi;
     fD = qdiv(epsilon0 * (T(1.0) - phi), T(ph.N_R));
 #pragma unroll
@@ -700,29 +726,46 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
     const int rf = r0 - 1 + w, cf = c0 + lane;
     const bool has = (lane < ST_TX && rf <= ymax && cf <= xmax);
     T L[6], R[6];
+    FaceMetIn<T, SRC> fm;
     if (ty == 0) {
       if (has) stage_states<T, TY>(F, w, lane, Uin, g, 1, rf, cf);
       __syncthreads();
-      if (has) load_states<T, TY>(F, w, lane, rf == 2, 1, L, R);
+      if (has) {
+        face_metric_load(M, g, 1, rf, cf, fm);
+        load_states<T, TY>(F, w, lane, rf == 2, 1, L, R);
+      }
       __syncthreads();
     } else if (has) {
-      base_states(Uin, g, 1, rf, cf, L, R);
+      Stencil<T> sn;
+      face_metric_load(M, g, 1, rf, cf, fm);
+      load_stencil(Uin, g, 1, rf, cf, sn);
+      __builtin_amdgcn_sched_barrier(0);
+      stencil_states(sn, L, R);
     }
-    if (has) flux_store<T, TY, UNI>(F, w, lane, M, g, ph, 1, rf, cf, L, R, Q[w * 64 + lane], Q[(w + 1) * 64 + lane]);
+    if (has) flux_store<T, TY, SRC>(F, w, lane, fm, g, ph, 1, L, R, Q[w * 64 + lane], Q[(w + 1) * 64 + lane]);
   }
   __syncthreads();
   if (!active) return 0.0;
 
+  // the update's loads, issued together: the cell's state, metric, uzero (the same buffer as
+  // Uin in the predictor, so taken from u there) and the old fluid momenta of the corrector
   T u[7];
 #pragma unroll
   for (int k = 0; k < 7; k++) u[k] = Uin[k * P + i];
   CellMet<T> cm;
-  cell_metric<T, UNI>(M, P, i, cm);
-  T s[6];
-  cell_source(u, cm, ph, s);
+  cell_metric_load<T, UNI>(M, P, i, cm);
   T uz[6];
 #pragma unroll
-  for (int k = 0; k < 6; k++) uz[k] = Uzero[k * P + i];
+  for (int k = 0; k < 6; k++) uz[k] = IO == 0 ? u[k] : Uzero[k * P + i];
+  T one4 = T(0), one5 = T(0);
+  if (IO == 1) {
+    one4 = ONE[i];
+    one5 = ONE[P + i];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  cell_metric_form<T, UNI>(cm);
+  T s[6];
+  cell_source(u, cm, ph, s);
 
   // Flux18Kernel L/K:2641-2652 (Pp and PD are stored zeros for the components they skip)
   T vnew[6];
@@ -762,7 +805,7 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
   if (IO == 0) {
     o0 = uz[0]; o1 = uz[1]; o2 = uz[2]; o3 = uz[3]; o4 = uz[4]; o5 = uz[5];
   } else {
-    o0 = u[0]; o1 = u[1]; o2 = u[2]; o3 = u[3]; o4 = ONE[i]; o5 = ONE[P + i];
+    o0 = u[0]; o1 = u[1]; o2 = u[2]; o3 = u[3]; o4 = one4; o5 = one5;
   }
   const T usxnew = qdiv(un[1], un[0] + h0);
   const T ufxnew = qdiv(un[4], un[3] + h0);
