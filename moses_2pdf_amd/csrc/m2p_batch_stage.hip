@@ -68,7 +68,8 @@ __global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per
     stage_zero<T, TY>(m.g, Uout, (T*)m.ONE, m.dry, tx, ty);
     return;
   }
-  const double v2 = stage_tile<IO, T, TY, PLANNED, UNI>(F, m.g, m.ph, Uin, (const T*)m.U0, Uout, (T*)m.ONE,
+  constexpr bool LB = !(std::is_same<T, float>::value && W >= 8);
+  const double v2 = stage_tile<IO, T, TY, PLANNED, UNI, LB>(F, m.g, m.ph, Uin, (const T*)m.U0, Uout, (T*)m.ONE,
                                           (const T*)m.MT, m.LIM,
                                           IO == 0 ? st->dt_pred : st->dt_corr, tx, ty, m.dry,
                                           IO == 0 ? st->dtdx_pred : st->dtdx_corr,

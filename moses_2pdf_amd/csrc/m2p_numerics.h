@@ -122,12 +122,14 @@ __device__ __forceinline__ void stencil_states(const Stencil<T>& s, T L[6], T R[
   }
 }
 
-template <typename T>
+// GROUPED = false leaves the order of the loads to the compiler (the FP32 kernels held to
+// 64 VGPRs spill when a face's loads are all in flight at once).
+template <typename T, bool GROUPED = true>
 __device__ __forceinline__ void base_states(const T* __restrict__ U, const Grid& g, int dir, int r,
                                             int c, T L[6], T R[6]) {
   Stencil<T> s;
   load_stencil(U, g, dir, r, c, s);
-  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (GROUPED) __builtin_amdgcn_sched_barrier(0);
   stencil_states(s, L, R);
 }
 

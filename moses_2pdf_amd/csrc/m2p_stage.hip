@@ -55,7 +55,7 @@ template <typename T, bool UNI>
 static void launch_stage_t(int io, const Grid& g, const Phys& ph, const T* Uin, const T* Uzero,
                            T* Uout, T* ONE, const T* M, unsigned char* LIM, DevState* st,
                            double* part, const DryMap& dm, const TilePlan& pl, hipStream_t s) {
-  constexpr int TY = PlainStage<T>::TY, W = PlainStage<T>::W;
+  constexpr int TY = PlainStage<T>::TY, W = PlainStage<T>::W, WP = PlainStage<T>::W_PRED;
   const dim3 b(64, TY + 1);
   const dim3 gr(pl.list ? plan_blocks(stage_tiles(g, TY))
                         : ((stage_tiles(g, TY) + N_XCD - 1) / N_XCD) * N_XCD);
@@ -67,7 +67,7 @@ static void launch_stage_t(int io, const Grid& g, const Phys& ph, const T* Uin, 
       k_stage<1, T, TY, W, true, UNI><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
   } else {
     if (io == 0)
-      k_stage<0, T, TY, W, false, UNI><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
+      k_stage<0, T, TY, WP, false, UNI><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
     else
       k_stage<1, T, TY, W, false, UNI><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
   }

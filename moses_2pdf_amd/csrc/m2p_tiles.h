@@ -550,12 +550,12 @@ __device__ __forceinline__ PDSide<T> pd_side(const T* __restrict__ U, const Grid
 constexpr int ST_TX = 63, ST_NQ = 12;
 
 // Phase A: InterfacesKernel states of one face into the tile (L: q 0-5, R: q 6-11).
-template <typename T, int TY>
+template <typename T, int TY, bool LB>
 __device__ __forceinline__ void stage_states(T (*Tl)[TY + 1][64], int w, int lane,
                                              const T* __restrict__ U, const Grid& g, int dir, int r,
                                              int c) {
   T L[6], R[6];
-  base_states(U, g, dir, r, c, L, R);
+  base_states<T, LB>(U, g, dir, r, c, L, R);
 #pragma unroll
   for (int k = 0; k < 6; k++) {
     Tl[k][w][lane] = L[k];
@@ -621,8 +621,10 @@ constexpr double WET_SPEED = 1.0e-2;  // scaled depth (1 mm)
 // with dry-tile skipping on; the others are compiled without the flag code).  UNI: the
 // grid's metric passed the uniform-metric check, so the inverse Jacobian is formed from
 // svec and cvalue instead of loaded (FM_UNIFORM): in FP64 the same values bit for bit, in
-// FP32 formed in float from the rounded svec and cvalue.
-template <int IO, typename T, int TY, bool FLAGS, bool UNI = false>
+// FP32 formed in float from the rounded svec and cvalue.  LB: issue each face's loads, and
+// the update's, together ahead of their arithmetic (base_states); off for the FP32 kernels
+// held to 64 VGPRs, which spill with it.  The results are the same either way.
+template <int IO, typename T, int TY, bool FLAGS, bool UNI = false, bool LB = true>
 __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, const Phys& ph,
                                            const T* __restrict__ Uin, const T* Uzero, T* Uout,
                                            T* __restrict__ ONE, const T* __restrict__ M,
@@ -668,7 +670,7 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
     T L[6], R[6];
     FaceMetIn<T, SRC> fm;
     if (tx == 0) {
-      if (has) stage_states<T, TY>(F, w, lane, Uin, g, 0, r, c);
+      if (has) stage_states<T, TY, LB>(F, w, lane, Uin, g, 0, r, c);
       __syncthreads();
       if (has) {
         face_metric_load(M, g, 0, r, c, fm);
@@ -679,7 +681,7 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
       Stencil<T> sn;
       face_metric_load(M, g, 0, r, c, fm);
       load_stencil(Uin, g, 0, r, c, sn);
-      __builtin_amdgcn_sched_barrier(0);
+      if constexpr (LB) __builtin_amdgcn_sched_barrier(0);
       stencil_states(sn, L, R);
     }
     if (has) flux_store<T, TY, SRC>(F, w, lane, fm, g, ph, 0, L, R, Q[w * 65 + lane], Q[w * 65 + lane + 1]);
@@ -728,7 +730,7 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
     T L[6], R[6];
     FaceMetIn<T, SRC> fm;
     if (ty == 0) {
-      if (has) stage_states<T, TY>(F, w, lane, Uin, g, 1, rf, cf);
+      if (has) stage_states<T, TY, LB>(F, w, lane, Uin, g, 1, rf, cf);
       __syncthreads();
       if (has) {
         face_metric_load(M, g, 1, rf, cf, fm);
@@ -739,7 +741,7 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
       Stencil<T> sn;
       face_metric_load(M, g, 1, rf, cf, fm);
       load_stencil(Uin, g, 1, rf, cf, sn);
-      __builtin_amdgcn_sched_barrier(0);
+      if constexpr (LB) __builtin_amdgcn_sched_barrier(0);
       stencil_states(sn, L, R);
     }
     if (has) flux_store<T, TY, SRC>(F, w, lane, fm, g, ph, 1, L, R, Q[w * 64 + lane], Q[(w + 1) * 64 + lane]);
@@ -762,7 +764,7 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
     one4 = ONE[i];
     one5 = ONE[P + i];
   }
-  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (LB) __builtin_amdgcn_sched_barrier(0);
   cell_metric_form<T, UNI>(cm);
   T s[6];
   cell_source(u, cm, ph, s);
@@ -947,10 +949,12 @@ __device__ __forceinline__ void plan_append(const TilePlan& pl, bool leader, int
 }
 
 // Tile height and the waves-per-SIMD cap of the single-scenario stage kernel: FP64 is held
-// to 128 VGPRs (4 waves); the FP32 body needs ~80 VGPRs and runs at 6.
+// to 128 VGPRs (4 waves); the FP32 body needs ~70-80 VGPRs and runs at 6, except the
+// dense FP32 predictor, which fits 64 VGPRs without spilling and runs at 8 (W_PRED; the
+// corrector and the planned kernels spill at 8 and are slower there).
 template <typename T> struct PlainStage;
-template <> struct PlainStage<double> { static constexpr int TY = 7, W = 4; };
-template <> struct PlainStage<float> { static constexpr int TY = 7, W = 6; };
+template <> struct PlainStage<double> { static constexpr int TY = 7, W = 4, W_PRED = 4; };
+template <> struct PlainStage<float> { static constexpr int TY = 7, W = 6, W_PRED = 8; };
 
 // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).  Block b runs on
 // XCD b % 8; it is given tile number (b % 8) * per + b / 8 in row-major tile order, so
