@@ -55,9 +55,9 @@ def traffic_from_profiles(kernel: str, n: int, precision: str = "fp64"):
 
 
 def valu_from_profiles(kernel: str, n: int, precision: str = "fp64"):
-    """(VALU issue share, wait share) of `kernel` at grid n from the committed rocprofv3 PMC
-    summary (profiles/valu_counters_<n>.txt lines '<precision> <kernel>: ... VALU issue share
-    A%, wait share B%'), or None."""
+    """(VALU issue share, wait share, VALU busy share or None) of `kernel` at grid n from the
+    committed rocprofv3 PMC summary (profiles/valu_counters_<n>.txt lines '<precision>
+    <kernel>: ... VALU issue share A%, wait share B%, VALU busy C%'), or None."""
     path = os.path.join(ROOT, "profiles", f"valu_counters_{n}.txt")
     if not os.path.exists(path):
         return None
@@ -66,7 +66,9 @@ def valu_from_profiles(kernel: str, n: int, precision: str = "fp64"):
             try:
                 a = float(line.split("VALU issue share")[1].split("%")[0])
                 b = float(line.split("wait share")[1].split("%")[0])
-                return a / 100.0, b / 100.0
+                c = (float(line.split("VALU busy")[1].split("%")[0]) / 100.0
+                     if "VALU busy" in line else None)
+                return a / 100.0, b / 100.0, c
             except (IndexError, ValueError):
                 return None
     return None
@@ -431,11 +433,12 @@ def limiter(kernel: str, args) -> dict:
     """What holds `kernel` below its HBM roofline, from the committed PMC counters."""
     v = valu_from_profiles(kernel, args.n, args.precision)
     why = ("every FP64 divide and square root stays the IEEE sequence (bit parity with the "
-           "oracle), so VALU issue and the latency of dependent loads, not bytes, set the time"
+           "oracle): about half of the VALU time, with the quarter-rate reciprocal; the vector "
+           "ALU and its dependent FP64 chains at 4 waves per SIMD, not bytes, set the time"
            if args.precision == "fp64" else
            "FP32 arithmetic with hardware reciprocals; VALU issue sets the time")
-    return {"kind": "valu issue and latency", "valu_issue_share": v[0] if v else None,
-            "wait_share": v[1] if v else None,
+    return {"kind": "valu", "valu_issue_share": v[0] if v else None,
+            "valu_busy_share": v[2] if v else None, "wait_share": v[1] if v else None,
             "source": f"profiles/valu_counters_{args.n}.txt" if v else None, "why": why}
 
 
