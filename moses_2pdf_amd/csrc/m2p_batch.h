@@ -50,9 +50,9 @@ struct BatchDev {
   // dry-tile skipping (some member has it on): the stage plans (8 bands whatever remap
   // says) and their PLAN_COUNTERS counters, zeroed by block 0 of the dt kernel
   int plan_on;
-  // every member passed the uniform-metric check: the stage kernels (and in FP64 the CFL
-  // kernel) form the inverse Jacobian instead of loading it (FP64: the same values bit for
-  // bit; FP32: in float from the rounded svec and cvalue)
+  // every member passed the uniform-metric check: the stage and CFL kernels form the
+  // inverse Jacobian instead of loading it (FP64: the same values bit for bit; FP32: in
+  // float from the rounded svec and cvalue)
   int uni;
   TilePlan plan[2];
   int* plan_cnt;

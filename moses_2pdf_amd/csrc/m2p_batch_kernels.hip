@@ -148,11 +148,9 @@ hipError_t launch_batch_steps(const BatchDev& B, int ty, int w, bool any_inflow,
       const int nbb = B.nBox < B.cfl_blocks ? B.nBox : B.cfl_blocks;
       const unsigned nb = nbb + (B.nRing + 3) / 4;
       bool done = false;
-      if constexpr (std::is_same<T, double>::value) {
-        if (B.uni) {
-          kb_cfl<T, true><<<nb, dim3(64, 4), 0, s>>>(B, nbb);
-          done = true;
-        }
+      if (B.uni) {
+        kb_cfl<T, true><<<nb, dim3(64, 4), 0, s>>>(B, nbb);
+        done = true;
       }
       if (!done) kb_cfl<T, false><<<nb, dim3(64, 4), 0, s>>>(B, nbb);
     }

@@ -67,9 +67,9 @@ struct m2p_ctx {
   std::vector<std::pair<const char*, std::pair<hipEvent_t, hipEvent_t>>> pending;
   m2p_batch* batch = nullptr;  // the batch that steps this scenario, if any
   bool vstop_on = false;       // a stop speed is set (the step reduces corrector speeds)
-  // the metric passed the uniform-metric check, so the stage kernels (and in FP64 the CFL
-  // kernel) form the inverse Jacobian from svec and cvalue instead of loading it (FP64: the
-  // same values bit for bit; FP32: in float from the rounded planes)
+  // the metric passed the uniform-metric check, so the stage and CFL kernels form the
+  // inverse Jacobian from svec and cvalue instead of loading it (FP64: the same values bit
+  // for bit; FP32: in float from the rounded planes)
   bool uni_metric = false;
   // FNV-1a hash of what a checkpoint's state is only valid with: the padded topography,
   // the run parameters and the hydrograph tables (set when the context is opened)

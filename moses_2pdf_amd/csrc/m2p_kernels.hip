@@ -555,12 +555,10 @@ void launch_cfl(const Grid& g, const Phys& ph, const T* U, const T* M, DevState*
                 const DryMap& dm, bool uni, hipStream_t s) {
   const dim3 b(64, 4);
   const int nb = cfl_box_blocks(g) + cfl_ring_blocks(g);
-  if constexpr (std::is_same<T, double>::value) {
-    if (uni) {
-      k_cfl<T, true><<<nb, b, 0, s>>>(g, ph, U, M, st, part, (g.nx + 63) / 64, cfl_boxes(g),
-                                      cfl_box_blocks(g), dm);
-      return;
-    }
+  if (uni) {
+    k_cfl<T, true><<<nb, b, 0, s>>>(g, ph, U, M, st, part, (g.nx + 63) / 64, cfl_boxes(g),
+                                    cfl_box_blocks(g), dm);
+    return;
   }
   k_cfl<T, false><<<nb, b, 0, s>>>(g, ph, U, M, st, part, (g.nx + 63) / 64, cfl_boxes(g),
                                    cfl_box_blocks(g), dm);
