@@ -545,9 +545,12 @@ static int cfl_boxes(const Grid& g) { return ((g.nx + 63) / 64) * ((g.ny + 3) / 
 static int cfl_box_blocks(const Grid& g) { return std::min(cfl_boxes(g), CFL_BLOCKS); }
 static int cfl_ring_blocks(const Grid& g) { return (ring_threads(g) + 255) / 256; }
 
+static int plain_tile_height(const Grid& g, int prec) {
+  return plain_ty(g, prec == 1 ? PlainStage<float>::TY : PlainStage<double>::TY);
+}
+
 size_t partial_count(const Grid& g, int prec) {
-  const int ty = prec == 1 ? PlainStage<float>::TY : PlainStage<double>::TY;
-  return (size_t)cfl_box_blocks(g) + cfl_ring_blocks(g) + stage_tiles(g, ty);
+  return (size_t)cfl_box_blocks(g) + cfl_ring_blocks(g) + stage_tiles(g, plain_tile_height(g, prec));
 }
 
 template <typename T>
@@ -572,13 +575,12 @@ void launch_dt(const DevParams& dp, DevState* st, const double* tf, const double
 }
 
 int plain_stage_tiles(const Grid& g, int prec) {
-  return stage_tiles(g, prec == 1 ? PlainStage<float>::TY : PlainStage<double>::TY);
+  return stage_tiles(g, plain_tile_height(g, prec));
 }
 
-template <typename T>
-void launch_plan(int io, const Grid& g, const T* Uin, const DryMap& dm, const TilePlan& pl,
-                 double* part, const DevState* st, hipStream_t s) {
-  constexpr int TY = PlainStage<T>::TY;
+template <typename T, int TY>
+static void launch_plan_ty(int io, const Grid& g, const T* Uin, const DryMap& dm,
+                           const TilePlan& pl, double* part, const DevState* st, hipStream_t s) {
   const unsigned nb = (unsigned)((stage_tiles(g, TY) + 4 * (64 / PLAN_LANES) - 1) / (4 * (64 / PLAN_LANES)));
   double* sp = part + cfl_box_blocks(g) + cfl_ring_blocks(g);
   if (io == 0)
@@ -587,9 +589,18 @@ void launch_plan(int io, const Grid& g, const T* Uin, const DryMap& dm, const Ti
     k_plan<1, T, TY><<<nb, dim3(64, 4), 0, s>>>(g, Uin, dm, pl, sp, st);
 }
 
+template <typename T>
+void launch_plan(int io, const Grid& g, const T* Uin, const DryMap& dm, const TilePlan& pl,
+                 double* part, const DevState* st, hipStream_t s) {
+  if (plain_ty(g, PlainStage<T>::TY) == PLAIN_TY_SMALL)
+    launch_plan_ty<T, PLAIN_TY_SMALL>(io, g, Uin, dm, pl, part, st, s);
+  else
+    launch_plan_ty<T, PlainStage<T>::TY>(io, g, Uin, dm, pl, part, st, s);
+}
+
 void launch_vmax(DevState* st, const Grid& g, int prec, const double* part, hipStream_t s) {
-  const int ty = prec == 1 ? PlainStage<float>::TY : PlainStage<double>::TY;
-  k_vmax<<<1, 256, 0, s>>>(st, part + cfl_box_blocks(g) + cfl_ring_blocks(g), stage_tiles(g, ty));
+  k_vmax<<<1, 256, 0, s>>>(st, part + cfl_box_blocks(g) + cfl_ring_blocks(g),
+                           stage_tiles(g, plain_tile_height(g, prec)));
 }
 
 size_t stage_partials_offset(const Grid& g) {

@@ -51,11 +51,11 @@ __global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per
   if (IO == 1 && st->vstop2 > 0.0) block_max_store(v2, part + t);
 }
 
-template <typename T, bool UNI>
-static void launch_stage_t(int io, const Grid& g, const Phys& ph, const T* Uin, const T* Uzero,
-                           T* Uout, T* ONE, const T* M, unsigned char* LIM, DevState* st,
-                           double* part, const DryMap& dm, const TilePlan& pl, hipStream_t s) {
-  constexpr int TY = PlainStage<T>::TY, W = PlainStage<T>::W, WP = PlainStage<T>::W_PRED;
+template <typename T, bool UNI, int TY>
+static void launch_stage_ty(int io, const Grid& g, const Phys& ph, const T* Uin, const T* Uzero,
+                            T* Uout, T* ONE, const T* M, unsigned char* LIM, DevState* st,
+                            double* part, const DryMap& dm, const TilePlan& pl, hipStream_t s) {
+  constexpr int W = PlainStage<T>::W, WP = PlainStage<T>::W_PRED;
   const dim3 b(64, TY + 1);
   const dim3 gr(pl.list ? plan_blocks(stage_tiles(g, TY))
                         : ((stage_tiles(g, TY) + N_XCD - 1) / N_XCD) * N_XCD);
@@ -71,6 +71,17 @@ static void launch_stage_t(int io, const Grid& g, const Phys& ph, const T* Uin, 
     else
       k_stage<1, T, TY, W, false, UNI><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
   }
+}
+
+// the tile height plain_ty picks for the grid
+template <typename T, bool UNI>
+static void launch_stage_t(int io, const Grid& g, const Phys& ph, const T* Uin, const T* Uzero,
+                           T* Uout, T* ONE, const T* M, unsigned char* LIM, DevState* st,
+                           double* part, const DryMap& dm, const TilePlan& pl, hipStream_t s) {
+  if (plain_ty(g, PlainStage<T>::TY) == PLAIN_TY_SMALL)
+    launch_stage_ty<T, UNI, PLAIN_TY_SMALL>(io, g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, part, dm, pl, s);
+  else
+    launch_stage_ty<T, UNI, PlainStage<T>::TY>(io, g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, part, dm, pl, s);
 }
 
 template <typename T>

@@ -972,4 +972,14 @@ __device__ __forceinline__ bool tile_of(const Grid& g, int TY, int t, int& tx, i
   return true;
 }
 
+// The single-scenario stage's tile height for a grid.  A grid with fewer than
+// PLAIN_SMALL_TILES 7-row tiles does not fill the GPU (two 8-wave blocks on each of 256
+// CUs); it runs 3-row tiles, twice as many blocks with half the dependent chain each
+// (128x128 FP64: 62 -> 43 us per step; 256x256 level; from 512x512 on the 7-row tile, which
+// recomputes fewer y-faces, is faster).
+constexpr int PLAIN_TY_SMALL = 3, PLAIN_SMALL_TILES = 256;
+__host__ __device__ __forceinline__ int plain_ty(const Grid& g, int ty_big) {
+  return stage_tiles(g, ty_big) < PLAIN_SMALL_TILES ? PLAIN_TY_SMALL : ty_big;
+}
+
 }  // namespace m2p
