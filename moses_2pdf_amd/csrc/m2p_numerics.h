@@ -79,6 +79,14 @@ __device__ __forceinline__ T tvd_slope(T um, T u0, T up) {
   return same ? m : T(0);
 }
 
+// FP32: minmod(dA, dB) is the median of dA, dB and 0 (both positive: the smaller; both
+// negative: the one nearer 0; else 0), one v_med3_f32 instead of the compares and selects.
+// It can differ from the FP64 form only in the sign of a zero slope and for NaN inputs.
+template <>
+__device__ __forceinline__ float tvd_slope<float>(float um, float u0, float up) {
+  return __builtin_amdgcn_fmed3f(u0 - um, up - u0, 0.0f);
+}
+
 // Interface states of InterfacesKernel (L/K:1226-1252) for one face, components 0..5
 // (component 6 of a face state is always replaced by KeepPositivi2Kernel).
 //   dir 0: x-face between (r,c) and (r,c+1): L = W state, R = E state.
