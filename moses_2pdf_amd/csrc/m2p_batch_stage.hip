@@ -69,12 +69,13 @@ __global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per
     return;
   }
   constexpr bool LB = !(std::is_same<T, float>::value && W >= 8);
+  const bool want_v2 = IO == 1 && st->vstop2 > 0.0;
   const double v2 = stage_tile<IO, T, TY, PLANNED, UNI, LB>(F, m.g, m.ph, Uin, (const T*)m.U0, Uout, (T*)m.ONE,
                                           (const T*)m.MT, m.LIM,
                                           IO == 0 ? st->dt_pred : st->dt_corr, tx, ty, m.dry,
                                           IO == 0 ? st->dtdx_pred : st->dtdx_corr,
-                                          IO == 0 ? st->dtdy_pred : st->dtdy_corr);
-  if (IO == 1 && st->vstop2 > 0.0) block_max_store(v2, B.part + B.nBox + B.nRing + t);
+                                          IO == 0 ? st->dtdy_pred : st->dtdy_corr, want_v2);
+  if (want_v2) block_max_store(v2, B.part + B.nBox + B.nRing + t);
 }
 
 template <typename T, int TY, int W, bool UNI>

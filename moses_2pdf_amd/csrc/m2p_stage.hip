@@ -44,11 +44,12 @@ __global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per
   }
   int tx, ty;
   if (!tile_of(g, TY, t, tx, ty)) return;
+  const bool want_v2 = IO == 1 && st->vstop2 > 0.0;
   const double v2 = stage_tile<IO, T, TY, PLANNED, UNI>(F, g, ph, Uin, Uzero, Uout, ONE, M, LIM,
                                                    IO == 0 ? st->dt_pred : st->dt_corr, tx, ty, dm,
                                                    IO == 0 ? st->dtdx_pred : st->dtdx_corr,
-                                                   IO == 0 ? st->dtdy_pred : st->dtdy_corr);
-  if (IO == 1 && st->vstop2 > 0.0) block_max_store(v2, part + t);
+                                                   IO == 0 ? st->dtdy_pred : st->dtdy_corr, want_v2);
+  if (want_v2) block_max_store(v2, part + t);
 }
 
 template <typename T, bool UNI, int TY>

@@ -613,8 +613,9 @@ __device__ __forceinline__ void flux_store(T (*Tl)[TY + 1][64], int w, int lane,
   Tl[11][w][lane] = o.D5;
 }
 
-// Returns, for IO = 1, the larger squared flow speed (solid, fluid) of the cell's new state
-// if its flow depth exceeds WET_SPEED (0 otherwise, and for IO = 0).
+// Returns, for IO = 1 and want_v2 (a stop speed is set; uniform over the launch), the larger
+// squared flow speed (solid, fluid) of the cell's new state if its flow depth exceeds
+// WET_SPEED (0 otherwise, and for IO = 0).
 constexpr double WET_SPEED = 1.0e-2;  // scaled depth (1 mm)
 
 // FLAGS: write the dry-tile flags of the output rows (only the planned stage kernels run
@@ -630,7 +631,7 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
                                            T* __restrict__ ONE, const T* __restrict__ M,
                                            unsigned char* __restrict__ LIM, double dt_d, int tx,
                                            int ty, const DryMap& dm, double dtdx_d = 0.0,
-                                           double dtdy_d = 0.0) {
+                                           double dtdy_d = 0.0, bool want_v2 = true) {
   const int lane = threadIdx.x;
   const int w = threadIdx.y;
   const int c0 = tx * ST_TX + MD;
@@ -858,7 +859,7 @@ __device__ __forceinline__ double stage_tile(T (*F)[TY + 1][64], const Grid& g, 
   } else {
 #pragma unroll
     for (int k = 0; k < 7; k++) Uout[k * P + i] = un[k];
-    if ((double)un[0] + (double)un[3] <= WET_SPEED) return 0.0;
+    if (!want_v2 || (double)un[0] + (double)un[3] <= WET_SPEED) return 0.0;
     const double ufx = (double)qdiv(lim4, un[3] + h0), ufy = (double)qdiv(lim5, un[3] + h0);
     const double sx = (double)usxnew, sy = (double)usynew;
     return fmax(sx * sx + sy * sy, ufx * ufx + ufy * ufy);
