@@ -45,7 +45,9 @@ __global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per
   int tx, ty;
   if (!tile_of(g, TY, t, tx, ty)) return;
   const bool want_v2 = IO == 1 && st->vstop2 > 0.0;
-  const double v2 = stage_tile<IO, T, TY, PLANNED, UNI>(F, g, ph, Uin, Uzero, Uout, ONE, M, LIM,
+  // an FP32 corrector above 6 waves keeps the compiler's load order (it spills otherwise)
+  constexpr bool LB = !(std::is_same<T, float>::value && IO == 1 && W > 6);
+  const double v2 = stage_tile<IO, T, TY, PLANNED, UNI, LB>(F, g, ph, Uin, Uzero, Uout, ONE, M, LIM,
                                                    IO == 0 ? st->dt_pred : st->dt_corr, tx, ty, dm,
                                                    IO == 0 ? st->dtdx_pred : st->dtdx_corr,
                                                    IO == 0 ? st->dtdy_pred : st->dtdy_corr, want_v2);
@@ -56,7 +58,8 @@ template <typename T, bool UNI, int TY>
 static void launch_stage_ty(int io, const Grid& g, const Phys& ph, const T* Uin, const T* Uzero,
                             T* Uout, T* ONE, const T* M, unsigned char* LIM, DevState* st,
                             double* part, const DryMap& dm, const TilePlan& pl, hipStream_t s) {
-  constexpr int W = PlainStage<T>::W, WP = PlainStage<T>::W_PRED;
+  constexpr int W = PlainStage<T>::W;
+  constexpr int WP = UNI ? PlainStage<T>::W_PRED : W, WC = UNI ? PlainStage<T>::W_CORR : W;
   const dim3 b(64, TY + 1);
   const dim3 gr(pl.list ? plan_blocks(stage_tiles(g, TY))
                         : ((stage_tiles(g, TY) + N_XCD - 1) / N_XCD) * N_XCD);
@@ -70,7 +73,7 @@ static void launch_stage_ty(int io, const Grid& g, const Phys& ph, const T* Uin,
     if (io == 0)
       k_stage<0, T, TY, WP, false, UNI><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
     else
-      k_stage<1, T, TY, W, false, UNI><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
+      k_stage<1, T, TY, WC, false, UNI><<<gr, b, 0, s>>>(g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, sp, dm, pl);
   }
 }
 

@@ -950,12 +950,14 @@ __device__ __forceinline__ void plan_append(const TilePlan& pl, bool leader, int
 }
 
 // Tile height and the waves-per-SIMD cap of the single-scenario stage kernel: FP64 is held
-// to 128 VGPRs (4 waves); the FP32 body needs ~70-80 VGPRs and runs at 6, except the
-// dense FP32 predictor, which fits 64 VGPRs without spilling and runs at 8 (W_PRED; the
-// corrector and the planned kernels spill at 8 and are slower there).
+// to 128 VGPRs (4 waves); the FP32 body needs ~70-80 VGPRs and runs at 6.  On the
+// uniform-metric form the dense FP32 predictor fits 64 VGPRs without spilling and runs at
+// 8 (W_PRED), and the dense corrector at 7 (W_CORR, with the compiler's load order, LB off).
+// The corrector spills at 8, the stored-metric kernels at 7 or 8 (16 VGPRs: their
+// predictor at 8 took 1.9x as long), so those and the planned kernels stay at W.
 template <typename T> struct PlainStage;
-template <> struct PlainStage<double> { static constexpr int TY = 7, W = 4, W_PRED = 4; };
-template <> struct PlainStage<float> { static constexpr int TY = 7, W = 6, W_PRED = 8; };
+template <> struct PlainStage<double> { static constexpr int TY = 7, W = 4, W_PRED = 4, W_CORR = 4; };
+template <> struct PlainStage<float> { static constexpr int TY = 7, W = 6, W_PRED = 8, W_CORR = 7; };
 
 // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).  Block b runs on
 // XCD b % 8; it is given tile number (b % 8) * per + b / 8 in row-major tile order, so
