@@ -64,24 +64,24 @@ __device__ __forceinline__ float qsqrt(float a) { return __builtin_amdgcn_sqrtf(
 // TVD1..5Kernel (L/K:1000-1119) for one component: the slope at a cell from its two
 // neighbours, dA = dux[c-1], dB = dux[c].  The reference forms
 //   t1 = (sA+sB)/2 * min(dA*sA, dB*sB),  t2 = (dA+dB)/2,  slope = (sA'+sB')/2 * min(t1*sA', t2*sB')
-// with s = (x > 0 ? 1 : -1).  That equals minmod(dA, dB) for every input: when dA and dB
-// are both positive or both negative it is sign(dA)*min(|dA|,|dB|) exactly (|t2| >= |t1|
-// holds after rounding because rounding is monotonic), otherwise a zero.  The signs are
-// tested directly (a product dA*dB underflows for the tiny differences at a wet/dry front).
-// Only the sign of a zero slope can differ, and a zero's sign never reaches a nonzero value
-// in this scheme (no quantity is divided by a slope and no branch tests the sign of a zero).
+// with s = (x > 0 ? 1 : -1).  For finite dA, dB that equals minmod(dA, dB): when both are
+// positive or both negative it is sign(dA)*min(|dA|,|dB|) exactly (|t2| >= |t1| holds after
+// rounding because rounding is monotonic), otherwise a zero.  minmod(dA, dB) is in turn the
+// median of dA, dB and 0 (both positive: the smaller; both negative: the one nearer 0; else
+// 0), formed here as max(min(dA, dB), min(max(dA, dB), 0)): four min/max instructions, no
+// product (dA*dB underflows for the tiny differences at a wet/dry front) and no sign tests.
+// Only the sign of a zero slope can differ from the reference's, and a zero's sign never
+// reaches a nonzero value in this scheme (no quantity is divided by a slope and no branch
+// tests the sign of a zero).  A NaN difference (a state already non-finite) gives another
+// result than the reference's products, which the run reports through its non-finite flag.
 template <typename T>
 __device__ __forceinline__ T tvd_slope(T um, T u0, T up) {
   const T dA = u0 - um;
   const T dB = up - u0;
-  const T m = copysign(fmin(fabs(dA), fabs(dB)), dA);
-  const bool same = (dA > T(0) && dB > T(0)) || (dA < T(0) && dB < T(0));
-  return same ? m : T(0);
+  return fmax(fmin(dA, dB), fmin(fmax(dA, dB), T(0)));
 }
 
-// FP32: minmod(dA, dB) is the median of dA, dB and 0 (both positive: the smaller; both
-// negative: the one nearer 0; else 0), one v_med3_f32 instead of the compares and selects.
-// It can differ from the FP64 form only in the sign of a zero slope and for NaN inputs.
+// FP32: the same median as one v_med3_f32.
 template <>
 __device__ __forceinline__ float tvd_slope<float>(float um, float u0, float up) {
   return __builtin_amdgcn_fmed3f(u0 - um, up - u0, 0.0f);
