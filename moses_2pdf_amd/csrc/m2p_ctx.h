@@ -42,6 +42,9 @@ struct m2p_ctx {
   hipStream_t s = nullptr;
   int prec = M2P_FP64;     // scalar type of the state and of the per-step metric copy
   size_t es = 8;           // its size in bytes
+  // every device buffer below lies in one of two allocations: dArena (held until the
+  // context closes) and dTmp (the set-up temporaries dTopo, dDepth, dPhi, freed after it)
+  void *dArena = nullptr, *dTmp = nullptr;
   double *dTopo = nullptr, *dDepth = nullptr, *dPhi = nullptr, *dM = nullptr;
   // state planes u^n (U0), u^{n+1/2} (U1), the limited fluid momenta (ONE) and the metric
   // the step kernels read (MT: dM itself in FP64, a float copy in FP32), in type `prec`
@@ -71,7 +74,7 @@ struct m2p_ctx {
   // inverse Jacobian from svec and cvalue instead of loading it (FP64: the same values bit
   // for bit; FP32: in float from the rounded planes)
   bool uni_metric = false;
-  // FNV-1a hash of what a checkpoint's state is only valid with: the padded topography,
+  // a hash of what a checkpoint's state is only valid with: the padded topography,
   // the run parameters and the hydrograph tables (set when the context is opened)
   uint64_t scen_hash = 0;
 };

@@ -103,16 +103,6 @@ double coord_host(int c, int n, double d) {
   return xmin + (c - MD) * d;
 }
 
-template <class T>
-int dalloc(m2p_ctx* c, T** p, size_t n) {
-  if (n == 0) n = 1;
-  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
-  if (e != hipSuccess) return fail(c, M2P_ENOMEM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed");
-  e = hipMemsetAsync(*p, 0, n * sizeof(T), c->s);
-  if (e != hipSuccess) return fail(c, M2P_EHIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
-  return 0;
-}
-
 int upload(m2p_ctx* c, double* d, const std::vector<double>& h) {
   if (h.empty()) return 0;
   HIPCHK(c, hipMemcpyAsync(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->s));
@@ -124,12 +114,9 @@ void release(m2p_ctx* c) {
   for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
   c->evpool.clear();
   c->pending.clear();
-  if (c->dMT == c->dM) c->dMT = nullptr;
-  void* ps[] = {c->dTopo, c->dDepth, c->dPhi, c->dM,     c->dU0,   c->dU1,    c->dONE, c->dMT,
-                c->dTf,   c->dMass,  c->dTable, c->dLoc, c->dItime, c->dDire, c->dSt,  c->dLIM,
-                c->dPart, c->dZ,     c->dPlan};
-  for (void* p : ps)
-    if (p) (void)hipFree(p);
+  if (c->dArena) (void)hipFree(c->dArena);
+  if (c->dTmp) (void)hipFree(c->dTmp);
+  c->dArena = c->dTmp = nullptr;
   c->dTopo = c->dDepth = c->dPhi = c->dM = c->dTf = c->dMass = c->dTable = c->dLoc = c->dItime = nullptr;
   c->dU0 = c->dU1 = c->dONE = c->dMT = nullptr;
   c->dLIM = nullptr;
@@ -164,39 +151,86 @@ void rescan_dry(m2p_ctx* c, const m2p::DryMap& dm, hipStream_t s) {
     m2p::launch_dry_scan<float>(c->g, dm, (const float*)c->dU0, (const float*)c->dU1, s);
 }
 
+// Device buffers carved out of one allocation (one hipMalloc and one clear instead of one
+// per buffer: they dominated the set-up of small grids).  Each starts on a 256-byte boundary.
+struct Arena {
+  struct Item {
+    void** p;
+    size_t off;
+  };
+  std::vector<Item> items;
+  size_t total = 0;
+  template <typename T>
+  void add(T** p, size_t n) {
+    items.push_back({(void**)p, total});
+    total += (std::max<size_t>(n, 1) * sizeof(T) + 255) & ~size_t(255);
+  }
+  // Allocates, clears on the context's stream and sets every pointer; *base owns it.
+  int alloc(m2p_ctx* c, void** base) {
+    hipError_t e = hipMalloc(base, total);
+    if (e != hipSuccess) {
+      *base = nullptr;
+      return fail(c, M2P_ENOMEM, "hipMalloc of " + std::to_string(total) + " bytes failed");
+    }
+    e = hipMemsetAsync(*base, 0, total, c->s);
+    if (e != hipSuccess) return fail(c, M2P_EHIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+    for (const Item& it : items) *it.p = (char*)*base + it.off;
+    return 0;
+  }
+};
+
 // Device allocation, uploads, metric setup (L/R:395-642) and the initial state.
 int setup(m2p_ctx* c) {
   HIPCHK(c, pool_stream(&c->s));
   HIPCHK(c, m2p::probe_kernels());
   const size_t P = (size_t)c->P;
+  c->g.nx = c->NX;
+  c->g.ny = c->NY;
+  c->g.nxd = c->nxd;
+  c->g.nyd = c->nyd;
+  c->g.P = c->P;
+  c->g.dx = c->dx;
+  c->g.dy = c->dy;
   int rc;
   double* dU64 = nullptr;  // FP64 initial state (the metric setup writes it)
-  if ((rc = dalloc(c, &c->dTopo, P)) || (rc = dalloc(c, &c->dDepth, P)) ||
-      // FP32 keeps only the FP64 cell planes; its face planes go straight into the float copy
-      (rc = dalloc(c, &c->dM, (size_t)(c->prec == M2P_FP64 ? m2p::MC_COUNT : m2p::MC_CELL_END) * P)) ||
-      (rc = dalloc(c, &dU64, 7 * P)) ||
-      (rc = dalloc(c, &c->dTf, c->tf.size())) || (rc = dalloc(c, &c->dMass, 4 * (size_t)m2p::mass_blocks())) ||
-      (rc = dalloc(c, &c->dSt, 1))) {
-    if (dU64) (void)hipFree(dU64);
-    return rc;
+  const int per = (m2p::plain_stage_tiles(c->g, c->prec) + 7) / 8;
+  const long long nz = m2p::dry_flag_count(c->g);
+  Arena keep, tmp;   // the context's buffers; the set-up temporaries, freed at its end
+  tmp.add(&c->dTopo, P);
+  tmp.add(&c->dDepth, P);
+  if (!c->phi.empty()) tmp.add(&c->dPhi, P);
+  // FP32 keeps only the FP64 cell planes; its face planes go straight into the float copy
+  keep.add(&c->dM, (size_t)(c->prec == M2P_FP64 ? m2p::MC_COUNT : m2p::MC_CELL_END) * P);
+  if (c->prec == M2P_FP64) {
+    keep.add(&dU64, 7 * P);
+    keep.add((double**)&c->dU1, 7 * P);
+    keep.add((double**)&c->dONE, 2 * P);
+  } else {
+    tmp.add(&dU64, 7 * P);
+    keep.add((float**)&c->dU0, 7 * P);
+    keep.add((float**)&c->dU1, 7 * P);
+    keep.add((float**)&c->dONE, 2 * P);
+    keep.add((float**)&c->dMT, (size_t)m2p::MC_COUNT * P);
+    keep.add(&c->dLIM, P);
   }
+  keep.add(&c->dTf, c->tf.size());
+  keep.add(&c->dMass, 4 * (size_t)m2p::mass_blocks());
+  keep.add(&c->dSt, 1);
+  if (c->mode == 1) {
+    keep.add(&c->dTable, c->table.size());
+    keep.add(&c->dLoc, c->loc.size());
+    keep.add(&c->dItime, c->itime.size());
+    keep.add(&c->dDire, c->dire.size());
+  }
+  keep.add(&c->dPart, m2p::partial_count(c->g, c->prec));
+  keep.add(&c->dZ, 2 * (size_t)nz);
+  keep.add(&c->dPlan, m2p::PLAN_COUNTERS + 16 * (size_t)per);
+  if ((rc = keep.alloc(c, &c->dArena)) || (rc = tmp.alloc(c, &c->dTmp))) return rc;
   if (c->prec == M2P_FP64) {
     c->dU0 = dU64;
     c->dMT = c->dM;
-    if ((rc = dalloc(c, (double**)&c->dU1, 7 * P)) || (rc = dalloc(c, (double**)&c->dONE, 2 * P))) return rc;
-  } else {
-    if ((rc = dalloc(c, (float**)&c->dU0, 7 * P)) || (rc = dalloc(c, (float**)&c->dU1, 7 * P)) ||
-        (rc = dalloc(c, (float**)&c->dONE, 2 * P)) ||
-        (rc = dalloc(c, (float**)&c->dMT, (size_t)m2p::MC_COUNT * P)) || (rc = dalloc(c, &c->dLIM, P))) {
-      (void)hipFree(dU64);
-      return rc;
-    }
   }
-  if (!c->phi.empty() && (rc = dalloc(c, &c->dPhi, P))) return rc;
   if (c->mode == 1) {
-    if ((rc = dalloc(c, &c->dTable, c->table.size())) || (rc = dalloc(c, &c->dLoc, c->loc.size())) ||
-        (rc = dalloc(c, &c->dItime, c->itime.size())) || (rc = dalloc(c, &c->dDire, c->dire.size())))
-      return rc;
     if ((rc = upload(c, c->dTable, c->table)) || (rc = upload(c, c->dLoc, c->loc)) ||
         (rc = upload(c, c->dItime, c->itime)))
       return rc;
@@ -208,28 +242,16 @@ int setup(m2p_ctx* c) {
     return rc;
   if (c->dPhi && (rc = upload(c, c->dPhi, c->phi))) return rc;
 
-  c->g.nx = c->NX;
-  c->g.ny = c->NY;
-  c->g.nxd = c->nxd;
-  c->g.nyd = c->nyd;
-  c->g.P = c->P;
-  c->g.dx = c->dx;
-  c->g.dy = c->dy;
   c->ph.rr = c->par.rho_ratio;
   c->ph.Drtio1 = 1.0 - c->par.rho_ratio;
   c->ph.Cd = c->par.Cd;
   c->ph.N_R = c->par.N_R;
   c->ph.varTheta = c->par.varTheta;
   c->ph.tande = std::tan(c->par.delta0 * 3.14159 / 180.);  // makeTopo11Kernel, L/K:298
-  if ((rc = dalloc(c, &c->dPart, m2p::partial_count(c->g, c->prec)))) return rc;
-  const long long nz = m2p::dry_flag_count(c->g);
-  if ((rc = dalloc(c, &c->dZ, 2 * (size_t)nz))) return rc;
   c->dry.z0 = c->dZ;
   c->dry.z1 = c->dZ + nz;
   c->dry.nseg = c->par.dry_skip && m2p::dry_capable(c->g) ? m2p::dry_segments(c->g) : 0;
   m2p::launch_dry_fill(c->g, c->dry, c->s);
-  const int per = (m2p::plain_stage_tiles(c->g, c->prec) + 7) / 8;
-  if ((rc = dalloc(c, &c->dPlan, m2p::PLAN_COUNTERS + 16 * (size_t)per))) return rc;
   for (int io = 0; io < 2; io++) {
     c->plan[io].cnt = c->dPlan + 8 * io;
     c->plan[io].list = c->dPlan + m2p::PLAN_COUNTERS + 8 * (size_t)per * io;
@@ -274,11 +296,9 @@ int setup(m2p_ctx* c) {
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->s));
   c->uni_metric = uni != 0;
-  if (c->prec == M2P_FP32) (void)hipFree(dU64);
-  // the grids are only needed by the setup
-  (void)hipFree(c->dTopo);
-  (void)hipFree(c->dDepth);
-  if (c->dPhi) (void)hipFree(c->dPhi);
+  // the grids (and the FP32 variant's FP64 initial state) are only needed by the setup
+  (void)hipFree(c->dTmp);
+  c->dTmp = nullptr;
   c->dTopo = c->dDepth = c->dPhi = nullptr;
   return 0;
 }
@@ -488,15 +508,25 @@ std::string check_params(int mode, const m2p_params& p) {
   return "";
 }
 
-// FNV-1a over the scenario's topography, parameters and hydrograph tables.
+// A 64-bit hash of the scenario's topography, parameters and hydrograph tables: FNV-1a's
+// step (xor, then multiply by the odd FNV prime, a bijection) on 64-bit words in four
+// interleaved lanes, so that a 1024^2 grid hashes in well under a millisecond (a serial
+// byte-wise chain took about 10 ms); a difference in any one word always changes the
+// result.  It binds a checkpoint to the scenario that wrote it.
 uint64_t scenario_hash(const m2p_ctx* c) {
-  uint64_t h = 1469598103934665603ull;
+  constexpr uint64_t kBasis = 1469598103934665603ull, kPrime = 1099511628211ull;
+  uint64_t h[4] = {kBasis, kBasis ^ 1, kBasis ^ 2, kBasis ^ 3};
   auto mix = [&](const void* p, size_t n) {
     const unsigned char* b = (const unsigned char*)p;
-    for (size_t i = 0; i < n; i++) {
-      h ^= b[i];
-      h *= 1099511628211ull;
-    }
+    size_t i = 0;
+    for (; i + 32 <= n; i += 32)
+      for (int k = 0; k < 4; k++) {
+        uint64_t w;
+        std::memcpy(&w, b + i + 8 * k, 8);
+        h[k] = (h[k] ^ w) * kPrime;
+      }
+    for (; i < n; i++) h[0] = (h[0] ^ b[i]) * kPrime;
+    for (int k = 0; k < 4; k++) h[k] = (h[k] ^ n) * kPrime;   // the length ends each part
   };
   const double pv[] = {c->par.total_sim, c->par.each_out, c->par.delta0, c->par.Cd, c->par.N_R,
                        c->par.varTheta,  c->par.phiS0,    c->par.cfl,    c->par.rho_ratio,
@@ -506,7 +536,9 @@ uint64_t scenario_hash(const m2p_ctx* c) {
   mix(c->table.data(), c->table.size() * sizeof(double));
   mix(c->loc.data(), c->loc.size() * sizeof(double));
   mix(c->itime.data(), c->itime.size() * sizeof(double));
-  return h;
+  uint64_t r = kBasis;
+  for (int k = 0; k < 4; k++) r = (r ^ h[k]) * kPrime;
+  return r ^ (r >> 29);
 }
 
 // A context that holds only an error message, for the caller to read and close.
