@@ -180,7 +180,7 @@ int rebuild(m2p_batch* b) {
     b->capPart = std::max(npart, 2 * b->capPart);
     BCHK(b, hipMalloc((void**)&b->dPart, b->capPart * sizeof(double)));
   }
-  const int per8 = (int)((ms.size() + 7) / 8);
+  const int per8 = m2p::plan_per((int)ms.size());
   const size_t nplan = m2p::PLAN_COUNTERS + 16 * (size_t)per8;
   if (nplan > b->capPlan) {
     if (b->dPlan) (void)hipFree(b->dPlan);

@@ -33,7 +33,7 @@ __global__ void __launch_bounds__(256) kb_plan(BatchDev B) {
   const int e = plan_tile<IO, T, TY>(g, Uin, dm, tx, ty, t, threadIdx.x % PLAN_LANES);
   const TilePlan& pl = B.plan[IO];
   const bool leader = t >= 0 && threadIdx.x % PLAN_LANES == 0;
-  plan_append(pl, leader, e, t >= 0 ? t / pl.per : 0);
+  plan_append(pl, leader, e, t >= 0 ? plan_list_of(t) : 0);
   if (IO == 1 && leader && e < 0) B.part[B.nBox + B.nRing + t] = 0.0;
 }
 

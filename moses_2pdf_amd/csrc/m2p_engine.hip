@@ -193,7 +193,7 @@ int setup(m2p_ctx* c) {
   c->g.dy = c->dy;
   int rc;
   double* dU64 = nullptr;  // FP64 initial state (the metric setup writes it)
-  const int per = (m2p::plain_stage_tiles(c->g, c->prec) + 7) / 8;
+  const int per = m2p::plan_per(m2p::plain_stage_tiles(c->g, c->prec));
   const long long nz = m2p::dry_flag_count(c->g);
   Arena keep, tmp;   // the context's buffers; the set-up temporaries, freed at its end
   tmp.add(&c->dTopo, P);
@@ -1064,7 +1064,8 @@ int m2p_footprint(int mode, int NX, int NY, int precision, int nsamp, int nloc,
   const long long full = (long long)m2p::MC_COUNT * P * 8, cells = (long long)m2p::MC_CELL_END * P * 8;
   long long b = 4LL * m2p::mass_blocks() * 8 + (long long)sizeof(DevState) + 4096;
   b += (P / 256 + P / 441 + 4LL * (NX + NY) + 64) * 8;   // per-block partial maxima
-  b += 2LL * NY * ((NX + 62) / 63) + 4 * (16 + 16 * (P / 441 / 8 + 1));   // dry flags, plans
+  // dry flags; plans, sized for the most tiles (3-row tiles)
+  b += 2LL * NY * ((NX + 62) / 63) + 4 * (16 + 16LL * m2p::plan_per(((NX + 62) / 63) * ((NY + 2) / 3)));
   if (precision == M2P_FP64)
     b += full + 16 * P * 8;
   else

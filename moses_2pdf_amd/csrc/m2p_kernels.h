@@ -86,22 +86,36 @@ constexpr int COPY_BLOCKS_PER_ITEM = 64;
 void launch_copy_list(const CopyItem* items, int nitems, hipStream_t s);
 
 // Tile plan of one stage launch under dry-tile skipping.  A plan kernel tests every tile
-// (PLAN_LANES = 2 lanes per tile) and lists, per XCD band (band x holds tiles [x * per, (x + 1) * per)),
-// the tiles the stage must visit: those to compute and, for the predictor, dry tiles whose
-// U1 and ONE must be cleared (entry | PLAN_ZERO).  Block b of the stage runs on XCD b % 8
-// and takes entry b / 8 of list b % 8; blocks past the list's end return at once, so a dry
-// tile costs one block launch, and the hardware's block dispatch balances the wet ones
-// (a bounded grid striding over the lists was measured slower: mixed tile costs left it
-// unbalanced).  The list lengths are zeroed by the dt kernel of the step.
+// (PLAN_LANES = 2 lanes per tile) and lists, in one of 8 lists, the tiles the stage must
+// visit: those to compute and, for the predictor, dry tiles whose U1 and ONE must be
+// cleared (entry | PLAN_ZERO).  Block b of the stage runs on XCD b % 8 and takes entry
+// b / 8 of list b % 8; blocks past the list's end return at once, so a dry tile costs one
+// block launch, and the hardware's block dispatch balances the wet ones (a bounded grid
+// striding over the lists was measured slower: mixed tile costs left it unbalanced).
+// Tiles go to the lists in chunks of PLAN_CHUNK consecutive tiles (tile t to list
+// (t / PLAN_CHUNK) % 8): a chunk's tiles share halos in one XCD's L2, and a compact wet
+// region still spreads over all 8 XCDs (with contiguous bands of tiles it fell on one or
+// two, and a batch's large members each on one).  The list lengths are zeroed by the dt
+// kernel of the step.
 constexpr int PLAN_ZERO = 1 << 30;
 constexpr int PLAN_COUNTERS = 16;   // the 8 list lengths of each of the two stages
+#ifndef M2P_PLAN_CHUNK
+#define M2P_PLAN_CHUNK 32
+#endif
+constexpr int PLAN_CHUNK = M2P_PLAN_CHUNK;
 struct TilePlan {
   int* list;   // 8 * per entries
   int* cnt;    // 8 list lengths
-  int per;     // band size: ceil(tiles / 8)
+  int per;     // list capacity, plan_per(tiles)
 };
+// The most tiles one list can receive: every chunk of it full.
+inline int plan_per(int tiles) {
+  const int span = 8 * PLAN_CHUNK;
+  return PLAN_CHUNK * ((tiles + span - 1) / span > 0 ? (tiles + span - 1) / span : 1);
+}
 // Blocks of a planned stage launch: one per list slot.
-inline int plan_blocks(int tiles) { return 8 * ((tiles + 7) / 8 > 0 ? (tiles + 7) / 8 : 1); }
+inline int plan_blocks(int tiles) { return 8 * plan_per(tiles); }
+__host__ __device__ __forceinline__ int plan_list_of(int t) { return (t / PLAN_CHUNK) & 7; }
 
 // A scenario is halted for the kernels that run before the dt kernel of a step
 // (inflow, ghost fill, CFL) once it has done halt_at (or call_halt) steps, and for those

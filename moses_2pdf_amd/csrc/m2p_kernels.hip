@@ -291,7 +291,7 @@ __global__ void __launch_bounds__(256) k_vmax(DevState* __restrict__ st,
   }
 }
 
-// One wave per stage tile: its plan entry into the list of its XCD band (tiles that need
+// Two lanes per stage tile: its plan entry into its list (plan_list_of; tiles that need
 // nothing have no entry; in the corrector their flow-speed partial is set to 0).
 template <int IO, typename T, int TY>
 __global__ void __launch_bounds__(256) k_plan(Grid g, const T* __restrict__ Uin, DryMap dm,
@@ -303,7 +303,7 @@ __global__ void __launch_bounds__(256) k_plan(Grid g, const T* __restrict__ Uin,
   if (!tile_of(g, TY, t, tx, ty)) t = -1;
   const int e = plan_tile<IO, T, TY>(g, Uin, dm, tx, ty, t, threadIdx.x % PLAN_LANES);
   const bool leader = t >= 0 && threadIdx.x % PLAN_LANES == 0;
-  plan_append(pl, leader, e, t >= 0 ? t / pl.per : 0);
+  plan_append(pl, leader, e, t >= 0 ? plan_list_of(t) : 0);
   if (IO == 1 && leader && e < 0) spart[t] = 0.0;
 }
 

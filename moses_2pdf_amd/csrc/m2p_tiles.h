@@ -927,8 +927,8 @@ __device__ __forceinline__ int plan_entry(const TilePlan& pl) {
 }
 
 // Appends the plan entries of the lane groups of one wave (e >= 0 in a group's leader lane,
-// band x) to the band lists with one atomic per band present in the wave (a wave's tiles
-// are consecutive, so usually one or two).  Every lane of the wave calls it.
+// list x) to the lists with one atomic per list present in the wave (a wave's 32 tiles
+// are consecutive, so two or three).  Every lane of the wave calls it.
 __device__ __forceinline__ void plan_append(const TilePlan& pl, bool leader, int e, int x) {
   const int lane = threadIdx.x & 63;
   bool pend = leader && e >= 0;
