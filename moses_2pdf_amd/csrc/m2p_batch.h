@@ -50,6 +50,10 @@ struct BatchDev {
   // dry-tile skipping (some member has it on): the stage plans (8 bands whatever remap
   // says) and their PLAN_COUNTERS counters, zeroed by block 0 of the dt kernel
   int plan_on;
+  // the planned stage launch has one block per list slot for the first plan_cap slots of
+  // each list (dry blocks past a list's end cost a block launch each, which for a large
+  // batch's sparse lists dominated the stage); kb_stage_more takes any entries past them
+  int plan_cap;
   // every member passed the uniform-metric check: the stage and CFL kernels form the
   // inverse Jacobian instead of loading it (FP64: the same values bit for bit; FP32: in
   // float from the rounded svec and cvalue)
@@ -66,6 +70,8 @@ int stage_variants(int prec, const StageVariant** out);
 
 constexpr int GHOST_LINES = 128;   // ghost-fill lines per block
 constexpr int BATCH_CFL_BLOCKS = 4096;   // default of BatchDev::cfl_blocks
+constexpr int BATCH_PLAN_CAP = 4096;     // default of BatchDev::plan_cap
+constexpr int STAGE_MORE_BLOCKS = 1024;  // blocks of kb_stage_more (a multiple of 8)
 int ring_blocks(const Grid& g);
 int box_blocks_x(const Grid& g);
 int box_blocks_y(const Grid& g);

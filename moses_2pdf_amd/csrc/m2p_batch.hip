@@ -44,6 +44,7 @@ struct m2p_batch {
   // schedule
   int ty = 7, w = 4, remap = 1, order = 1, graph_steps = 16;
   int cfl_blocks = m2p::BATCH_CFL_BLOCKS;
+  int plan_cap = m2p::BATCH_PLAN_CAP;
   bool dirty = true;   // members changed: the block maps must be rebuilt
   int built_ty = 0, built_order = -1, built_w = 0;   // schedule the device tables are for
   // device tables
@@ -209,6 +210,7 @@ int rebuild(m2p_batch* b) {
   B.nStage = (int)ms.size();
   B.remap = b->remap;
   B.cfl_blocks = b->cfl_blocks;
+  B.plan_cap = b->plan_cap;
   B.vmax = any_vstop ? 1 : 0;
   B.plan_on = any_dry ? 1 : 0;
   B.uni = (all_uni && n > 0) ? 1 : 0;
@@ -227,17 +229,19 @@ int rebuild(m2p_batch* b) {
 
 // Brings the device tables in line with the members and the schedule.  The block maps are
 // rebuilt when the members, the tile height or the member order changed; the occupancy
-// cap, XCD banding and CFL block count only change kernel arguments (and drop a captured
-// graph, which holds them).
+// cap, XCD banding, CFL block count and plan cap only change kernel arguments (and drop a
+// captured graph, which holds them).
 int refresh(m2p_batch* b) {
   if (b->dirty || b->ty != b->built_ty || b->order != b->built_order) return rebuild(b);
-  if (b->w != b->built_w || b->remap != b->B.remap || b->cfl_blocks != b->B.cfl_blocks) {
+  if (b->w != b->built_w || b->remap != b->B.remap || b->cfl_blocks != b->B.cfl_blocks ||
+      b->plan_cap != b->B.plan_cap) {
     if (!variant_ok(b->prec, b->ty, b->w))
       return bfail(b, M2P_EINVAL, "no stage kernel compiled for ty=" + std::to_string(b->ty) +
                                       " waves=" + std::to_string(b->w));
     drop_graph(b);
     b->B.remap = b->remap;
     b->B.cfl_blocks = b->cfl_blocks;
+    b->B.plan_cap = b->plan_cap;
     b->built_w = b->w;
   }
   return 0;
@@ -395,6 +399,8 @@ int m2p_batch_set(m2p_batch* b, const char* key, int value) {
     b->graph_steps = value;
   } else if (k == "cfl_blocks" && value >= 64) {
     b->cfl_blocks = value;
+  } else if (k == "plan_cap" && value >= 1) {
+    b->plan_cap = value;
   } else {
     return bfail(b, M2P_EINVAL, "unknown key or bad value: " + k);
   }
@@ -410,6 +416,7 @@ int m2p_batch_get(const m2p_batch* b, const char* key) {
   if (k == "order") return b->order;
   if (k == "graph_steps") return b->graph_steps;
   if (k == "cfl_blocks") return b->cfl_blocks;
+  if (k == "plan_cap") return b->plan_cap;
   if (k == "stage_blocks") return b->B.nStage;
   return M2P_EINVAL;
 }

@@ -4,6 +4,7 @@
 // Compiled once per scalar type: M2P_BATCH_T is double or float.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "m2p_batch.h"
@@ -37,25 +38,10 @@ __global__ void __launch_bounds__(256) kb_plan(BatchDev B) {
   if (IO == 1 && leader && e < 0) B.part[B.nBox + B.nRing + t] = 0.0;
 }
 
-// PLANNED: dry-tile skipping, one block per plan-list slot (halted members have no
-// entries); a kernel of its own, so the dense kernel keeps its registers.  UNI: every
-// member's metric has the uniform form (BatchDev::uni).
+// One stage tile t of the batch (with zero: a dry predictor tile to clear).  Every thread
+// of the block calls it (it holds barriers unless the tile is cleared or its member halted).
 template <int IO, typename T, int TY, int W, bool PLANNED, bool UNI>
-__global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per_eu(W, W)))
-    kb_stage(BatchDev B) {
-  __shared__ T F[ST_NQ][TY + 1][64];
-  const int b = blockIdx.x;
-  int t = b;
-  bool zero = false;
-  if (PLANNED) {
-    const int e = plan_entry(B.plan[IO]);
-    if (e < 0) return;
-    t = e & ~PLAN_ZERO;
-    zero = IO == 0 && (e & PLAN_ZERO);
-  } else if (B.remap) {
-    const int per = (B.nStage + N_XCD - 1) / N_XCD;
-    t = (b % N_XCD) * per + b / N_XCD;
-  }
+__device__ __forceinline__ void batch_tile(const BatchDev& B, T (*F)[TY + 1][64], int t, bool zero) {
   if (t >= B.nStage) return;
   const Member& m = B.mem[B.mapStage[t]];
   DevState* st = m.st;
@@ -78,19 +64,67 @@ __global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per
   if (want_v2) block_max_store(v2, B.part + B.nBox + B.nRing + t);
 }
 
+// PLANNED: dry-tile skipping, one block per plan-list slot of the first plan_cap slots of
+// each list (halted members have no entries); a kernel of its own, so the dense kernel
+// keeps its registers.  UNI: every member's metric has the uniform form (BatchDev::uni).
+template <int IO, typename T, int TY, int W, bool PLANNED, bool UNI>
+__global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per_eu(W, W)))
+    kb_stage(BatchDev B) {
+  __shared__ T F[ST_NQ][TY + 1][64];
+  const int b = blockIdx.x;
+  int t = b;
+  bool zero = false;
+  if (PLANNED) {
+    const int e = plan_entry(B.plan[IO]);
+    if (e < 0) return;
+    t = e & ~PLAN_ZERO;
+    zero = IO == 0 && (e & PLAN_ZERO);
+  } else if (B.remap) {
+    const int per = (B.nStage + N_XCD - 1) / N_XCD;
+    t = (b % N_XCD) * per + b / N_XCD;
+  }
+  batch_tile<IO, T, TY, W, PLANNED, UNI>(B, F, t, zero);
+}
+
+// Kept out of line: inlined into the loop below, the tile body spilled 24-29 VGPRs at 8
+// waves per SIMD (6 this way).
+template <int IO, typename T, int TY, int W, bool UNI>
+__device__ __attribute__((noinline)) void batch_tile_call(const BatchDev& B, T (*F)[TY + 1][64], int t, bool zero) {
+  batch_tile<IO, T, TY, W, true, UNI>(B, F, t, zero);
+}
+
+// The plan entries past the first plan_cap of each list: STAGE_MORE_BLOCKS blocks, block b
+// taking entries plan_cap + b / 8, plan_cap + b / 8 + STAGE_MORE_BLOCKS / 8, ... of list
+// b % 8.  Usually there are none and every block returns at once; a tile costs about twice
+// what it does in kb_stage.
+template <int IO, typename T, int TY, int W, bool UNI>
+__global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per_eu(W, W)))
+    kb_stage_more(BatchDev B) {
+  __shared__ T F[ST_NQ][TY + 1][64];
+  const TilePlan& pl = B.plan[IO];
+  const int x = blockIdx.x % 8, n = pl.cnt[x];
+  for (int j = B.plan_cap + (int)blockIdx.x / 8; j < n; j += STAGE_MORE_BLOCKS / 8) {
+    const int e = pl.list[x * pl.per + j];
+    batch_tile_call<IO, T, TY, W, UNI>(B, F, e & ~PLAN_ZERO, IO == 0 && (e & PLAN_ZERO));
+  }
+}
+
 template <typename T, int TY, int W, bool UNI>
 static void stage_pair_t(const BatchDev& B, int io, hipStream_t s) {
   const dim3 blk(64, TY + 1);
-  const unsigned nb = B.plan_on ? (unsigned)plan_blocks(B.nStage)
+  const unsigned nb = B.plan_on ? (unsigned)(8 * std::min(plan_per(B.nStage), B.plan_cap))
                    : B.remap ? (unsigned)(((B.nStage + N_XCD - 1) / N_XCD) * N_XCD) : (unsigned)B.nStage;
   const unsigned np = (unsigned)((B.nStage + 4 * (64 / PLAN_LANES) - 1) / (4 * (64 / PLAN_LANES)));
   if (B.plan_on) {
+    const bool more = plan_per(B.nStage) > B.plan_cap;   // a list can outgrow the capped launch
     if (io == 0) {
       kb_plan<0, T, TY><<<np, dim3(64, 4), 0, s>>>(B);
       kb_stage<0, T, TY, W, true, UNI><<<nb, blk, 0, s>>>(B);
+      if (more) kb_stage_more<0, T, TY, W, UNI><<<STAGE_MORE_BLOCKS, blk, 0, s>>>(B);
     } else {
       kb_plan<1, T, TY><<<np, dim3(64, 4), 0, s>>>(B);
       kb_stage<1, T, TY, W, true, UNI><<<nb, blk, 0, s>>>(B);
+      if (more) kb_stage_more<1, T, TY, W, UNI><<<STAGE_MORE_BLOCKS, blk, 0, s>>>(B);
     }
   } else {
     if (io == 0)

@@ -126,6 +126,23 @@ def test_cfl_block_count_gives_the_same_result(precision):
     say(precision, "cfl_blocks 64 / 1000 / 32768 equal to alone")
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_plan_cap_gives_the_same_result(precision):
+    # with dry-tile skipping the stage launch takes the first plan_cap entries of each tile
+    # list and a second launch the rest: at plan_cap 1 nearly every tile goes through the
+    # second (its loop over entries), at 1024 none does
+    scs = population()
+    ref = alone(scs, 15, precision)
+    for cap in (1, 5, 1024):
+        def setup(b, cap=cap):
+            b.set("plan_cap", cap)
+            b.set("graph_steps", 4)
+            assert b.get("plan_cap") == cap
+        got = batched(scs, 15, precision, chunks=[3, 12], setup=setup)
+        assert same(got, ref), cap
+    say(precision, "plan_cap 1 / 5 / 1024 equal to alone")
+
+
 def test_tune_restores_state_and_keeps_results():
     scs = population()
     ref = alone(scs, 10, "fp32")
