@@ -277,9 +277,10 @@ int m2p_batch_tune(m2p_batch* b, int trial_steps, char* report, int report_len);
 /* Manual schedule: keys "ty", "waves", "remap", "order", "graph_steps" (0: no graph),
  * "cfl_blocks" (>= 64; most blocks of the CFL launch, default 4096), "plan_cap" (>= 1;
  * with dry-tile skipping, the stage launch has a block for each of the first plan_cap
- * entries of each of its 8 tile lists, and a second launch of 1024 blocks takes the rest,
- * about half as fast per tile; default 4096, so 32768 listed tiles per stage; raise it for
- * batches that are mostly wet).  Results do not depend on any of them.
+ * entries of each of its 8 tile lists, and a second launch of 512 blocks takes the rest,
+ * at about half the speed per tile and on fewer blocks; default 4096, so up to 32768
+ * listed tiles per stage run at full speed; raise it for batches that are mostly wet).
+ * Results do not depend on any of them.
  * The (ty, waves) pair must be a compiled variant; m2p_batch_step reports M2P_EINVAL
  * otherwise. */
 int m2p_batch_set(m2p_batch* b, const char* key, int value);

@@ -71,7 +71,7 @@ int stage_variants(int prec, const StageVariant** out);
 constexpr int GHOST_LINES = 128;   // ghost-fill lines per block
 constexpr int BATCH_CFL_BLOCKS = 4096;   // default of BatchDev::cfl_blocks
 constexpr int BATCH_PLAN_CAP = 4096;     // default of BatchDev::plan_cap
-constexpr int STAGE_MORE_BLOCKS = 1024;  // blocks of kb_stage_more (a multiple of 8)
+constexpr int STAGE_MORE_BLOCKS = 512;   // blocks of kb_stage_more (a multiple of 8)
 int ring_blocks(const Grid& g);
 int box_blocks_x(const Grid& g);
 int box_blocks_y(const Grid& g);
