@@ -308,7 +308,8 @@ int save(m2p_batch* b, Snapshot& sn, bool restore) {
       const size_t P = (size_t)c->P, es = c->es;
       const Part q[] = {{c->dU0, 7 * P * es, 0}, {c->dU1, 7 * P * es, 0}, {c->dONE, 2 * P * es, 0},
                         {c->dLIM, c->dLIM ? P : 0, 0}, {c->dSt, sizeof(DevState), 0},
-                        {c->dZ, c->dZ ? 2 * (size_t)m2p::dry_flag_count(c->g) : 0, 0}};
+                        {c->dZ, c->dZ ? 2 * (size_t)m2p::dry_flag_count(c->g) : 0, 0},
+                        {c->dGF, c->dGF ? 2 * (size_t)m2p::ghost_flag_count(c->g) : 0, 0}};
       for (const Part& x : q) {
         if (x.n == 0 || !x.dev) continue;
         parts.push_back({x.dev, x.n, total});

@@ -37,7 +37,8 @@ __global__ void __launch_bounds__(GHOST_LINES) kb_ghost(BatchDev B, int post) {
   if (m.mode == 0 && k % 3 != 0) return;   // a landslide's momenta ghosts hold 0
   T* U = (T*)(post ? m.U1 : m.U0);
   const T* prev = (const T*)(post ? m.U0 : m.U1);
-  ghost_line(m.g, m.mode, U, prev, line, k);
+  ghost_line_skip(m.g, m.mode, U, prev, line, k, post ? m.dry.z1 : m.dry.z0,
+                  post ? m.dry.g1 : m.dry.g0, m.dry.nseg);
 }
 
 // Blocks stride over the batch's CFL boxes, CFL_GROUP at a time: one round of loads tests

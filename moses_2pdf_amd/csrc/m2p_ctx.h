@@ -51,6 +51,7 @@ struct m2p_ctx {
   void *dU0 = nullptr, *dU1 = nullptr, *dONE = nullptr, *dMT = nullptr;
   unsigned char* dLIM = nullptr;  // FP32 only: limiter memory flags, one byte per cell
   unsigned char* dZ = nullptr;    // dry-tile flags of U0 and U1 (m2p::DryMap)
+  unsigned char* dGF = nullptr;   // ghost-fill flags of U0 and U1 (m2p::DryMap)
   m2p::DryMap dry{};              // nseg 0 while dry-tile skipping is off
   int* dPlan = nullptr;           // tile plans of the two stages and their 16 counters
   m2p::TilePlan plan[2]{};

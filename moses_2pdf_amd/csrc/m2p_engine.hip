@@ -122,6 +122,7 @@ void release(m2p_ctx* c) {
   c->dLIM = nullptr;
   c->dPart = nullptr;
   c->dZ = nullptr;
+  c->dGF = nullptr;
   c->dPlan = nullptr;
   c->dry = m2p::DryMap{};
   c->dDire = nullptr;
@@ -224,6 +225,7 @@ int setup(m2p_ctx* c) {
   }
   keep.add(&c->dPart, m2p::partial_count(c->g, c->prec));
   keep.add(&c->dZ, 2 * (size_t)nz);
+  keep.add(&c->dGF, 2 * (size_t)m2p::ghost_flag_count(c->g));
   keep.add(&c->dPlan, m2p::PLAN_COUNTERS + 16 * (size_t)per);
   if ((rc = keep.alloc(c, &c->dArena)) || (rc = tmp.alloc(c, &c->dTmp))) return rc;
   if (c->prec == M2P_FP64) {
@@ -250,6 +252,8 @@ int setup(m2p_ctx* c) {
   c->ph.tande = std::tan(c->par.delta0 * 3.14159 / 180.);  // makeTopo11Kernel, L/K:298
   c->dry.z0 = c->dZ;
   c->dry.z1 = c->dZ + nz;
+  c->dry.g0 = c->dGF;
+  c->dry.g1 = c->dGF + m2p::ghost_flag_count(c->g);
   c->dry.nseg = c->par.dry_skip && m2p::dry_capable(c->g) ? m2p::dry_segments(c->g) : 0;
   m2p::launch_dry_fill(c->g, c->dry, c->s);
   for (int io = 0; io < 2; io++) {
@@ -409,7 +413,8 @@ void enqueue_step_t(m2p_ctx* c, int ntf) {
   }
   {
     Timer t(c, "ghost");
-    m2p::launch_ghost(c->g, c->mode, U0, (const T*)U1, c->dSt, 0, c->s);
+    m2p::launch_ghost(c->g, c->mode, U0, (const T*)U1, c->dSt, 0, c->dry.z0, c->dry.g0,
+                      c->dry.nseg, c->s);
   }
   {
     Timer t(c, "cfl");
@@ -430,7 +435,8 @@ void enqueue_step_t(m2p_ctx* c, int ntf) {
   }
   {
     Timer t(c, "ghost");
-    m2p::launch_ghost(c->g, c->mode, U1, (const T*)U0, c->dSt, 1, c->s);
+    m2p::launch_ghost(c->g, c->mode, U1, (const T*)U0, c->dSt, 1, c->dry.z1, c->dry.g1,
+                      c->dry.nseg, c->s);
   }
   {
     Timer t(c, "stage1");
@@ -1064,6 +1070,7 @@ int m2p_footprint(int mode, int NX, int NY, int precision, int nsamp, int nloc,
   const long long full = (long long)m2p::MC_COUNT * P * 8, cells = (long long)m2p::MC_CELL_END * P * 8;
   long long b = 4LL * m2p::mass_blocks() * 8 + (long long)sizeof(DevState) + 4096;
   b += (P / 256 + P / 441 + 4LL * (NX + NY) + 64) * 8;   // per-block partial maxima
+  b += 2 * 7LL * (NX + NY + 4 * MD);   // ghost-fill flags
   // dry flags; plans, sized for the most tiles (3-row tiles)
   b += 2LL * NY * ((NX + 62) / 63) + 4 * (16 + 16LL * m2p::plan_per(((NX + 62) / 63) * ((NY + 2) / 3)));
   if (precision == M2P_FP64)

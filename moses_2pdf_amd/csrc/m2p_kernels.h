@@ -57,14 +57,22 @@ struct DryMap {
   unsigned char* z0;
   unsigned char* z1;
   int nseg;
+  // Ghost-fill flags of U0 and U1, 7 * (nyd + nxd) each: g[k * (nyd + nxd) + line] is 0 only
+  // if the last fill of that buffer wrote +0 to every ghost cell of plane k on that line
+  // (ghost_line_skip); set to 1 with the segment flags.
+  unsigned char* g0;
+  unsigned char* g1;
 };
 inline int dry_segments(const Grid& g) { return (g.nx + 62) / 63; }
 // The dry test infers the ghost cells from the interior cells two deep along the boundary,
 // so it needs at least two interior rows and columns.
 inline bool dry_capable(const Grid& g) { return g.nx >= 2 && g.ny >= 2; }
 inline long long dry_flag_count(const Grid& g) { return (long long)g.ny * dry_segments(g); }
+inline long long ghost_flag_count(const Grid& g) { return 7LL * (g.nyd + g.nxd); }
 // Sets every flag of both maps to 1 (nothing may be skipped in the next step).
 void launch_dry_fill(const Grid& g, const DryMap& dm, hipStream_t s);
+// Sets both ghost-fill flag arrays to 1 (the next fills write every line).
+void launch_ghost_flags_fill(const Grid& g, const DryMap& dm, hipStream_t s);
 // Sets both maps from the state itself, z0 from U0 and z1 from U1, for dm.nseg segments per
 // row: a segment's flag is 0 when all 7 components of its cells are +0 bit for bit, else 1
 // (a -0 or a NaN counts as wet, so such cells are computed as without skipping).  phi is
@@ -154,9 +162,13 @@ template <typename T>
 void launch_inflow(const Grid& g, T* U, const double* M, const double* table, const double* loc,
                    const int* dire, int nloc, int nsamp, double phiS0, const DevState* st,
                    unsigned char* LIM, const DryMap& dm, hipStream_t s);
+// Ghost fill of Tg (post 0: U0 before the CFL pass; post 1: U1 before the corrector).  With
+// dry-tile skipping, a landslide's line whose source cells are all in dry segments (z of
+// Tg) and whose ghosts the last fill left at +0 (gf) is left alone: the fill would write
+// the same +0.
 template <typename T>
 void launch_ghost(const Grid& g, int mode, T* Tg, const T* prev, const DevState* st, int post,
-                  hipStream_t s);
+                  const unsigned char* z, unsigned char* gf, int nseg, hipStream_t s);
 // A landslide's ghost cells of the momenta (planes 1, 2, 4, 5) hold 0 at all times: every
 // ghost fill would write 0 there (Boundary7/8), nothing else writes a landslide's ghost
 // cells, and a host-side write of the state clears them with this kernel.  So the ghost

@@ -195,12 +195,13 @@ __global__ void k_dry_fill(unsigned char* __restrict__ z, long long n) {
 
 template <typename T>
 __global__ void k_ghost(Grid g, int mode, T* __restrict__ Tg, const T* __restrict__ prev,
-                        const DevState* __restrict__ st, int post) {
+                        const DevState* __restrict__ st, int post, const unsigned char* __restrict__ z,
+                        unsigned char* __restrict__ gf, int nseg) {
   const int line = blockIdx.x * blockDim.x + threadIdx.x;
   if (line >= g.nyd + g.nxd || (post ? halted_post(st) : halted_pre(st))) return;
   // a landslide fills planes 0, 3, 6 only: its momenta ghosts hold 0 (ghost_momenta_kept)
   const int k = mode == 0 ? 3 * (int)blockIdx.y : (int)blockIdx.y;
-  ghost_line(g, mode, Tg, prev, line, k);
+  ghost_line_skip(g, mode, Tg, prev, line, k, z, gf, nseg);
 }
 
 // The ghost ring of the momenta planes (1, 2, 4, 5) set to 0, as a landslide ghost fill
@@ -466,8 +467,6 @@ void launch_inflow(const Grid& g, T* U, const double* M, const double* table, co
   k_inflow<T><<<1, 64, 0, s>>>(g, U, M, table, loc, dire, nloc, nsamp, phiS0, st, LIM, dm);
 }
 
-__device__ __forceinline__ bool nonzero_bits(double v) { return __double_as_longlong(v) != 0; }
-__device__ __forceinline__ bool nonzero_bits(float v) { return __float_as_uint(v) != 0u; }
 
 // One wave per row segment (63 cells, one per lane); blocks of 4 waves.
 template <typename T>
@@ -493,6 +492,7 @@ void launch_dry_scan(const Grid& g, const DryMap& dm, const T* U0, const T* U1, 
   const unsigned nb = (unsigned)((n + 3) / 4);
   k_dry_scan<T><<<nb, dim3(64, 4), 0, s>>>(g, U0, dm.z0, dm.nseg);
   k_dry_scan<T><<<nb, dim3(64, 4), 0, s>>>(g, U1, dm.z1, dm.nseg);
+  launch_ghost_flags_fill(g, dm, s);   // the ghost cells may hold anything: fill every line
 }
 template void launch_dry_scan<double>(const Grid&, const DryMap&, const double*, const double*,
                                       hipStream_t);
@@ -526,14 +526,23 @@ void launch_dry_fill(const Grid& g, const DryMap& dm, hipStream_t s) {
   const unsigned nb = (unsigned)std::min<long long>((n + 255) / 256, 1024);
   k_dry_fill<<<nb, 256, 0, s>>>(dm.z0, n);
   k_dry_fill<<<nb, 256, 0, s>>>(dm.z1, n);
+  launch_ghost_flags_fill(g, dm, s);
+}
+
+void launch_ghost_flags_fill(const Grid& g, const DryMap& dm, hipStream_t s) {
+  if (!dm.g0) return;
+  const long long n = ghost_flag_count(g);
+  const unsigned nb = (unsigned)std::min<long long>((n + 255) / 256, 1024);
+  k_dry_fill<<<nb, 256, 0, s>>>(dm.g0, n);
+  k_dry_fill<<<nb, 256, 0, s>>>(dm.g1, n);
 }
 
 template <typename T>
 void launch_ghost(const Grid& g, int mode, T* Tg, const T* prev, const DevState* st, int post,
-                  hipStream_t s) {
+                  const unsigned char* z, unsigned char* gf, int nseg, hipStream_t s) {
   const int lines = g.nyd + g.nxd;
   k_ghost<T><<<dim3((lines + 127) / 128, mode == 0 ? 3 : MN), 128, 0, s>>>(g, mode, Tg, prev, st,
-                                                                          post);
+                                                                          post, z, gf, nseg);
 }
 
 template <typename T>
@@ -644,7 +653,7 @@ void launch_mass_sum(const double* part, int nb, double* out, hipStream_t s) {
                                  const int*, int, int, double, const DevState*,                \
                                  unsigned char*, const DryMap&, hipStream_t);                  \
   template void launch_ghost<T>(const Grid&, int, T*, const T*, const DevState*, int,         \
-                                hipStream_t);                                                 \
+                                const unsigned char*, unsigned char*, int, hipStream_t);      \
   template void launch_zero_ghost_momenta<T>(const Grid&, T*, hipStream_t);                     \
   template void launch_cfl<T>(const Grid&, const Phys&, const T*, const T*, DevState*, double*, \
                               const DryMap&, bool, hipStream_t);                               \

@@ -16,6 +16,9 @@
 
 namespace m2p {
 
+// Whether a value is anything but +0, bit for bit (-0 and NaN count).
+__device__ __forceinline__ bool nonzero_bits(double v) { return __double_as_longlong(v) != 0; }
+__device__ __forceinline__ bool nonzero_bits(float v) { return __float_as_uint(v) != 0u; }
 
 // Inflow1Kernel + Inflow2Kernel (I/K:818-887).  In the reference every thread of the
 // grid repeats the whole loop; one thread does it here, in the same order (FP64
@@ -76,8 +79,9 @@ __device__ __forceinline__ void extrap3(T a, T b, T& g2, T& g1, T& g0) {
   g0 = T(0.5) * (g1 + g2); // u[0] = (u[1]+u[2])/2
 }
 
+// Returns whether it wrote a value other than +0 (bit for bit).
 template <typename T>
-__device__ __forceinline__ void ghost_line(const Grid& g, int mode, T* __restrict__ Tg,
+__device__ __forceinline__ bool ghost_line(const Grid& g, int mode, T* __restrict__ Tg,
                                            const T* __restrict__ prev, int line, int k) {
   const long long P = g.P;
   T* A = Tg + k * P;
@@ -86,13 +90,13 @@ __device__ __forceinline__ void ghost_line(const Grid& g, int mode, T* __restric
   if (line < g.nyd) {
     const int r = line;
     const bool ghost_row = (r < MD || r >= ny + MD);
-    if (ghost_row && mode == 1) return;
+    if (ghost_row && mode == 1) return true;
     const T* src = (ghost_row ? prev + k * P : A) + (long long)r * g.nxd;
     T* row = A + (long long)r * g.nxd;
     if (mom) {
       row[0] = row[1] = row[2] = T(0);
       row[nx + MD] = row[nx + MD + 1] = row[nx + MD + 2] = T(0);
-      return;
+      return false;
     }
     T g2, g1, g0, h3, h4, h5;
     extrap3(src[MD], src[MD + 1], g2, g1, g0);
@@ -103,16 +107,18 @@ __device__ __forceinline__ void ghost_line(const Grid& g, int mode, T* __restric
     row[nx + MD] = h3;
     row[nx + MD + 1] = h4;
     row[nx + MD + 2] = h5;
+    return nonzero_bits(g0) || nonzero_bits(g1) || nonzero_bits(g2) || nonzero_bits(h3) ||
+           nonzero_bits(h4) || nonzero_bits(h5);
   } else {
     const int c = line - g.nyd;
     const bool ghost_col = (c < MD || c >= nx + MD);
-    if (ghost_col && mode == 0) return;
+    if (ghost_col && mode == 0) return false;
     const long long n = g.nxd;
     T* col = A + c;
     if (mom) {
       col[0] = col[n] = col[2 * n] = T(0);
       col[(ny + MD) * n] = col[(ny + MD + 1) * n] = col[(ny + MD + 2) * n] = T(0);
-      return;
+      return false;
     }
     T a3, a4, b2, b1;
     if (!ghost_col) {
@@ -149,7 +155,48 @@ __device__ __forceinline__ void ghost_line(const Grid& g, int mode, T* __restric
     col[(ny + MD) * n] = h3;
     col[(ny + MD + 1) * n] = h4;
     col[(ny + MD + 2) * n] = h5;
+    return nonzero_bits(g0) || nonzero_bits(g1) || nonzero_bits(g2) || nonzero_bits(h3) ||
+           nonzero_bits(h4) || nonzero_bits(h5);
   }
+}
+
+// Whether a landslide's ghost line reads only cells of dry segments (z: the flags of the
+// buffer it fills, valid for nseg > 0): an interior row reads columns 3, 4, nx+1 and nx+2
+// of its own row, an interior column rows 3, 4, ny+1 and ny+2 of its own column.  The
+// ghost rows (they read the other buffer's ghost rows) and columns (which a landslide's
+// fill leaves alone) count as not dry.
+__device__ __forceinline__ bool ghost_sources_dry(const Grid& g, const unsigned char* __restrict__ z,
+                                                  int nseg, int line) {
+  if (line < g.nyd) {
+    const int r = line;
+    if (r < MD || r >= g.ny + MD) return false;
+    const unsigned char* f = z + (long long)(r - MD) * nseg;
+    return (f[0] | f[(g.nx - 2) / 63] | f[(g.nx - 1) / 63]) == 0;
+  }
+  const int c = line - g.nyd;
+  if (c < MD || c >= g.nx + MD) return false;
+  const int sg = (c - MD) / 63;
+  return (z[sg] | z[nseg + sg] | z[(long long)(g.ny - 2) * nseg + sg] |
+          z[(long long)(g.ny - 1) * nseg + sg]) == 0;
+}
+
+// ghost_line with the dry skip: a landslide line (plane k) whose sources are dry and whose
+// ghosts the last fill left at +0 is not filled (the fill would write the same +0 bits);
+// otherwise it is filled and its flag says whether it wrote anything but +0.
+template <typename T>
+__device__ __forceinline__ void ghost_line_skip(const Grid& g, int mode, T* __restrict__ Tg,
+                                                const T* __restrict__ prev, int line, int k,
+                                                const unsigned char* __restrict__ z,
+                                                unsigned char* __restrict__ gf, int nseg) {
+  if (mode != 0 || nseg <= 0 || !gf) {
+    ghost_line(g, mode, Tg, prev, line, k);
+    return;
+  }
+  unsigned char* f = gf + (long long)k * (g.nyd + g.nxd) + line;
+  const unsigned char was = *f;
+  if (was == 0 && ghost_sources_dry(g, z, nseg, line)) return;
+  const unsigned char now = ghost_line(g, mode, Tg, prev, line, k) ? 1 : 0;
+  if (now != was) *f = now;
 }
 
 // Max of one wave (64 lanes) with fmax semantics (NaN ignored, as fmaxf in reduceKernel).
