@@ -446,7 +446,7 @@ void enqueue_step_t(m2p_ctx* c, int ntf) {
                       c->dPart, c->dry, c->dry.nseg > 0 ? c->plan[1] : m2p::TilePlan{},
                       c->uni_metric, c->s);
   }
-  if (c->vstop_on) m2p::launch_vmax(c->dSt, c->g, c->prec, c->dPart, c->s);
+  if (c->vstop_on) m2p::launch_vmax(c->dSt, c->g, c->prec, c->dry.nseg > 0, c->dPart, c->s);
 }
 
 void enqueue_step(m2p_ctx* c, int ntf) {
@@ -1069,7 +1069,7 @@ int m2p_footprint(int mode, int NX, int NY, int precision, int nsamp, int nloc,
   const long long P = (long long)(NX + 2 * MD) * (NY + 2 * MD);
   const long long full = (long long)m2p::MC_COUNT * P * 8, cells = (long long)m2p::MC_CELL_END * P * 8;
   long long b = 4LL * m2p::mass_blocks() * 8 + (long long)sizeof(DevState) + 4096;
-  b += (P / 256 + P / 441 + 4LL * (NX + NY) + 64) * 8;   // per-block partial maxima
+  b += (P / 256 + P / 189 + 4LL * (NX + NY) + 64) * 8;   // per-block partial maxima
   b += 2 * 7LL * (NX + NY + 4 * MD);   // ghost-fill flags
   // dry flags; plans, sized for the most tiles (3-row tiles)
   b += 2LL * NY * ((NX + 62) / 63) + 4 * (16 + 16LL * m2p::plan_per(((NX + 62) / 63) * ((NY + 2) / 3)));

@@ -188,13 +188,16 @@ void launch_dt(const DevParams& dp, DevState* st, const double* tf, const double
                const Grid& g, int prec, const double* part, int* plan_cnt, hipStream_t s);
 // Offset of the corrector tiles' partials in a scenario's partial array (after the CFL ones).
 size_t stage_partials_offset(const Grid& g);
-// Tiles of the single-scenario stage launch, and the plan kernel before it.
+// The most tiles of a single-scenario stage launch (planned launches, and the plan kernel
+// before them, take 3-row tiles; dense ones plain_ty's height).
 int plain_stage_tiles(const Grid& g, int prec);
 template <typename T>
 void launch_plan(int io, const Grid& g, const T* Uin, const DryMap& dm, const TilePlan& pl,
                  double* part, const DevState* st, hipStream_t s);
-// After a corrector: the squared max flow speed of its tiles into DevState (stop speed).
-void launch_vmax(DevState* st, const Grid& g, int prec, const double* part, hipStream_t s);
+// After a corrector (planned: with dry-tile skipping): the squared max flow speed of its
+// tiles into DevState (stop speed).
+void launch_vmax(DevState* st, const Grid& g, int prec, bool planned, const double* part,
+                 hipStream_t s);
 template <typename T>
 void launch_stage(int io, const Grid& g, const Phys& ph, const T* Uin, const T* Uzero, T* Uout,
                   T* ONE, const T* M, unsigned char* LIM, DevState* st, double* part,

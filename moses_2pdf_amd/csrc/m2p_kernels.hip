@@ -554,12 +554,14 @@ static int cfl_boxes(const Grid& g) { return ((g.nx + 63) / 64) * ((g.ny + 3) / 
 static int cfl_box_blocks(const Grid& g) { return std::min(cfl_boxes(g), CFL_BLOCKS); }
 static int cfl_ring_blocks(const Grid& g) { return (ring_threads(g) + 255) / 256; }
 
-static int plain_tile_height(const Grid& g, int prec) {
-  return plain_ty(g, prec == 1 ? PlainStage<float>::TY : PlainStage<double>::TY);
+// Tile height of the single-scenario stage: planned (dry-tile skipping) launches always take
+// 3-row tiles, dense ones plain_ty's.
+static int plain_tile_height(const Grid& g, int prec, bool planned) {
+  return planned ? PLAIN_TY_SMALL : plain_ty(g, prec == 1 ? PlainStage<float>::TY : PlainStage<double>::TY);
 }
 
 size_t partial_count(const Grid& g, int prec) {
-  return (size_t)cfl_box_blocks(g) + cfl_ring_blocks(g) + stage_tiles(g, plain_tile_height(g, prec));
+  return (size_t)cfl_box_blocks(g) + cfl_ring_blocks(g) + plain_stage_tiles(g, prec);
 }
 
 template <typename T>
@@ -584,7 +586,8 @@ void launch_dt(const DevParams& dp, DevState* st, const double* tf, const double
 }
 
 int plain_stage_tiles(const Grid& g, int prec) {
-  return stage_tiles(g, plain_tile_height(g, prec));
+  return std::max(stage_tiles(g, plain_tile_height(g, prec, true)),
+                  stage_tiles(g, plain_tile_height(g, prec, false)));
 }
 
 template <typename T, int TY>
@@ -601,15 +604,13 @@ static void launch_plan_ty(int io, const Grid& g, const T* Uin, const DryMap& dm
 template <typename T>
 void launch_plan(int io, const Grid& g, const T* Uin, const DryMap& dm, const TilePlan& pl,
                  double* part, const DevState* st, hipStream_t s) {
-  if (plain_ty(g, PlainStage<T>::TY) == PLAIN_TY_SMALL)
-    launch_plan_ty<T, PLAIN_TY_SMALL>(io, g, Uin, dm, pl, part, st, s);
-  else
-    launch_plan_ty<T, PlainStage<T>::TY>(io, g, Uin, dm, pl, part, st, s);
+  launch_plan_ty<T, PLAIN_TY_SMALL>(io, g, Uin, dm, pl, part, st, s);
 }
 
-void launch_vmax(DevState* st, const Grid& g, int prec, const double* part, hipStream_t s) {
+void launch_vmax(DevState* st, const Grid& g, int prec, bool planned, const double* part,
+                 hipStream_t s) {
   k_vmax<<<1, 256, 0, s>>>(st, part + cfl_box_blocks(g) + cfl_ring_blocks(g),
-                           stage_tiles(g, plain_tile_height(g, prec)));
+                           stage_tiles(g, plain_tile_height(g, prec, planned)));
 }
 
 size_t stage_partials_offset(const Grid& g) {

@@ -77,12 +77,14 @@ static void launch_stage_ty(int io, const Grid& g, const Phys& ph, const T* Uin,
   }
 }
 
-// the tile height plain_ty picks for the grid
+// the tile height plain_ty picks for the grid, and 3-row tiles for planned launches (with
+// dry-tile skipping the wet tiles are few: shorter tiles finish sooner; at 2048^2 the
+// skipping step took 13% less time in FP64, 10% in FP32, than with 7-row tiles)
 template <typename T, bool UNI>
 static void launch_stage_t(int io, const Grid& g, const Phys& ph, const T* Uin, const T* Uzero,
                            T* Uout, T* ONE, const T* M, unsigned char* LIM, DevState* st,
                            double* part, const DryMap& dm, const TilePlan& pl, hipStream_t s) {
-  if (plain_ty(g, PlainStage<T>::TY) == PLAIN_TY_SMALL)
+  if (pl.list || plain_ty(g, PlainStage<T>::TY) == PLAIN_TY_SMALL)
     launch_stage_ty<T, UNI, PLAIN_TY_SMALL>(io, g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, part, dm, pl, s);
   else
     launch_stage_ty<T, UNI, PlainStage<T>::TY>(io, g, ph, Uin, Uzero, Uout, ONE, M, LIM, st, part, dm, pl, s);
