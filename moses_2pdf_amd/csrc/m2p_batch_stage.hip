@@ -54,7 +54,7 @@ __device__ __forceinline__ void batch_tile(const BatchDev& B, T (*F)[TY + 1][64]
     stage_zero<T, TY>(m.g, Uout, (T*)m.ONE, m.dry, tx, ty);
     return;
   }
-  constexpr bool LB = !(std::is_same<T, float>::value && W >= 8);
+  constexpr bool LB = !(std::is_same<T, float>::value && W >= 7);
   const bool want_v2 = IO == 1 && st->vstop2 > 0.0;
   const double v2 = stage_tile<IO, T, TY, PLANNED, UNI, LB>(F, m.g, m.ph, Uin, (const T*)m.U0, Uout, (T*)m.ONE,
                                           (const T*)m.MT, m.LIM,
@@ -156,8 +156,10 @@ bool launch_batch_stage(const BatchDev& B, int ty, int w, int io, hipStream_t s)
     else if (ty == 7 && w == 4) stage_pair<float, 7, 4>(B, io, s);
     else if (ty == 3 && w == 6) stage_pair<float, 3, 6>(B, io, s);
     else if (ty == 15 && w == 4) stage_pair<float, 15, 4>(B, io, s);
-    else if (ty == 3 && w == 8) stage_pair<float, 3, 8>(B, io, s);
-    else if (ty == 7 && w == 8) stage_pair<float, 7, 8>(B, io, s);
+    // the 8-wave variants run their corrector at 7 waves (at 8 it spills 4 VGPRs):
+    // ensemble stepping 3% faster
+    else if (ty == 3 && w == 8) { if (io == 0) stage_pair<float, 3, 8>(B, io, s); else stage_pair<float, 3, 7>(B, io, s); }
+    else if (ty == 7 && w == 8) { if (io == 0) stage_pair<float, 7, 8>(B, io, s); else stage_pair<float, 7, 7>(B, io, s); }
     else return false;
   }
   return true;
