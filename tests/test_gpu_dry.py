@@ -116,6 +116,25 @@ def test_dry_skip_in_a_batch(precision):
         same(g, r)
 
 
+def test_stop_speed_rule_same_with_and_without_skipping():
+    # 520^2: the dense stages take 7-row tiles and the skipping ones 3-row tiles, so the
+    # flow-speed reduction behind the stop speed must read the partials of the tiling that
+    # ran; the flow (rows 248-278) lies in tiles that only the 3-row numbering reaches past
+    # the 7-row tile count.  At this stop speed the flow reaches it and falls below it within
+    # a few steps.
+    sc = S.landslide_plane(n=520)
+    got = []
+    for dry in (False, True):
+        with Simulation(sc, dry_skip=dry) as s:
+            s.set_stop(vstop=2.0, max_steps=3000)
+            s.step(3000)
+            got.append((s.status(), s.state()))
+    (sa, ua), (sb, ub) = got
+    say(f"stop speed: dense {sa}, skipping {sb}")
+    assert sa == sb and sa[2] and sa[1] < 3000
+    assert np.array_equal(ua.view(np.uint64), ub.view(np.uint64))
+
+
 def test_dry_skip_skips():
     sc = S.landslide_fractal(n=1024, seed=2)   # 0.05% of the cells start wet
     ms = {}
