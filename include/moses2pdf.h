@@ -264,6 +264,13 @@ int m2p_batch_create(int precision, m2p_batch** out);
 int m2p_batch_add(m2p_batch* b, m2p_ctx* ctx);
 int m2p_batch_remove(m2p_batch* b, m2p_ctx* ctx);
 int m2p_batch_size(const m2p_batch* b);
+/* Status of every member, in order of addition, with one device-to-host copy for the whole
+ * batch (no counterpart in the reference; for serving loops that poll many members): per
+ * member the scaled time, the steps done and whether it has halted (as m2p_status), and
+ * the non-finite flag and the step count of the first non-finite state, -1 if none (as
+ * m2p_nonfinite).  n must equal m2p_batch_size; any array may be null. */
+int m2p_batch_status(m2p_batch* b, int n, double* t, int* nt, int* halted, int* nonfinite,
+                     int* nonfinite_nt);
 /* Enqueues nsteps steps of every member (asynchronous; m2p_batch_sync waits). */
 int m2p_batch_step(m2p_batch* b, int nsteps);
 int m2p_batch_sync(m2p_batch* b);

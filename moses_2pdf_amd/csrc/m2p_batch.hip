@@ -56,6 +56,10 @@ struct m2p_batch {
   size_t capPart = 0;
   int* dPlan = nullptr;   // stage plans under dry-tile skipping
   size_t capPlan = 0;
+  // m2p_batch_status: the members' DevState pointers and a gathered copy of the states
+  DevState** dStPtr = nullptr;
+  DevState* dStAll = nullptr;
+  size_t capSt = 0;
   m2p::BatchDev B{};
   bool any_inflow = false;
   hipGraph_t graph = nullptr;
@@ -373,6 +377,38 @@ int m2p_batch_remove(m2p_batch* b, m2p_ctx* c) {
 
 int m2p_batch_size(const m2p_batch* b) { return b ? (int)b->mem.size() : M2P_ESTATE; }
 
+int m2p_batch_status(m2p_batch* b, int n, double* t, int* nt, int* halted, int* nonfinite,
+                     int* nonfinite_nt) {
+  if (!b || !b->s) return M2P_ESTATE;
+  if (n != (int)b->mem.size()) return bfail(b, M2P_EINVAL, "n must equal the number of members");
+  if (n == 0) return 0;
+  if ((size_t)n > b->capSt) {
+    if (b->dStPtr) (void)hipFree(b->dStPtr);
+    if (b->dStAll) (void)hipFree(b->dStAll);
+    b->dStPtr = nullptr;
+    b->dStAll = nullptr;
+    b->capSt = std::max((size_t)n, 2 * b->capSt);
+    BCHK(b, hipMalloc((void**)&b->dStPtr, b->capSt * sizeof(DevState*)));
+    BCHK(b, hipMalloc((void**)&b->dStAll, b->capSt * sizeof(DevState)));
+  }
+  std::vector<DevState*> ptr((size_t)n);
+  for (int k = 0; k < n; k++) ptr[k] = b->mem[k]->dSt;
+  std::vector<DevState> st((size_t)n);
+  BCHK(b, hipMemcpyAsync(b->dStPtr, ptr.data(), n * sizeof(DevState*), hipMemcpyHostToDevice, b->s));
+  m2p::launch_gather_states(b->dStPtr, b->dStAll, n, b->s);
+  BCHK(b, hipGetLastError());
+  BCHK(b, hipMemcpyAsync(st.data(), b->dStAll, n * sizeof(DevState), hipMemcpyDeviceToHost, b->s));
+  BCHK(b, hipStreamSynchronize(b->s));
+  for (int k = 0; k < n; k++) {
+    if (t) t[k] = st[k].time;
+    if (nt) nt[k] = st[k].nt;
+    if (halted) halted[k] = (st[k].skip || st[k].nt >= st[k].halt_at) ? 1 : 0;
+    if (nonfinite) nonfinite[k] = st[k].nonfinite;
+    if (nonfinite_nt) nonfinite_nt[k] = st[k].nonfinite ? st[k].nonfinite_nt : -1;
+  }
+  return 0;
+}
+
 int m2p_batch_step(m2p_batch* b, int nsteps) {
   if (!b || !b->s) return M2P_ESTATE;
   if (nsteps < 0) return bfail(b, M2P_EINVAL, "negative step count");
@@ -530,6 +566,8 @@ void m2p_batch_destroy(m2p_batch* b) {
   if (b->s) (void)hipStreamSynchronize(b->s);
   for (m2p_ctx* c : b->mem) c->batch = nullptr;
   drop_graph(b);
+  if (b->dStPtr) (void)hipFree(b->dStPtr);
+  if (b->dStAll) (void)hipFree(b->dStAll);
   if (b->dMem) (void)hipFree(b->dMem);
   if (b->dMap) (void)hipFree(b->dMap);
   if (b->dPart) (void)hipFree(b->dPart);

@@ -92,6 +92,8 @@ struct CopyItem {
 };
 constexpr int COPY_BLOCKS_PER_ITEM = 64;
 void launch_copy_list(const CopyItem* items, int nitems, hipStream_t s);
+// Copies n scenario states (*src[k] -> dst[k]) in one launch; src is in device memory.
+void launch_gather_states(const DevState* const* src, DevState* dst, int n, hipStream_t s);
 
 // Tile plan of one stage launch under dry-tile skipping.  A plan kernel tests every tile
 // (PLAN_LANES = 2 lanes per tile) and lists, in one of 8 lists, the tiles the stage must

@@ -518,6 +518,20 @@ __global__ void __launch_bounds__(256) k_copy_list(const CopyItem* items) {
   for (size_t i = done + lane; i < c.n; i += stride) c.dst[i] = c.src[i];
 }
 
+__global__ void __launch_bounds__(64) k_gather_states(const DevState* const* __restrict__ src,
+                                                     DevState* __restrict__ dst, int n) {
+  const int k = blockIdx.x;
+  if (k >= n) return;
+  static_assert(sizeof(DevState) % 4 == 0, "DevState is copied in 4-byte words");
+  const unsigned* a = (const unsigned*)src[k];
+  unsigned* d = (unsigned*)(dst + k);
+  for (int i = threadIdx.x; i < (int)(sizeof(DevState) / 4); i += 64) d[i] = a[i];
+}
+
+void launch_gather_states(const DevState* const* src, DevState* dst, int n, hipStream_t s) {
+  if (n > 0) k_gather_states<<<n, 64, 0, s>>>(src, dst, n);
+}
+
 void launch_copy_list(const CopyItem* items, int nitems, hipStream_t s) {
   if (nitems <= 0) return;
   k_copy_list<<<(unsigned)nitems * COPY_BLOCKS_PER_ITEM, 256, 0, s>>>(items);

@@ -29,7 +29,7 @@ EXPORTS = ["m2p_default_params", "m2p_open", "m2p_open_grid", "m2p_run", "m2p_st
            "m2p_get_field", "m2p_profile", "m2p_profile_get", "m2p_last_error", "m2p_close",
            "m2p_set_stop", "m2p_status", "m2p_footprint", "m2p_checkpoint", "m2p_restore",
            "m2p_run_steps", "m2p_set_dry_skip", "m2p_nonfinite", "m2p_mass_noalloc",
-           "m2p_batch_create", "m2p_batch_add", "m2p_batch_remove", "m2p_batch_size",
+           "m2p_batch_create", "m2p_batch_add", "m2p_batch_remove", "m2p_batch_size", "m2p_batch_status",
            "m2p_batch_step", "m2p_batch_sync", "m2p_batch_tune", "m2p_batch_set", "m2p_batch_get",
            "m2p_batch_last_error", "m2p_batch_destroy"]
 
@@ -95,6 +95,7 @@ def lib():
     L.m2p_batch_add.argtypes = [vp, vp]
     L.m2p_batch_remove.argtypes = [vp, vp]
     L.m2p_batch_size.argtypes = [vp]
+    L.m2p_batch_status.argtypes = [vp, ctypes.c_int, dp, ip, ip, ip, ip]
     L.m2p_batch_step.argtypes = [vp, ctypes.c_int]
     L.m2p_batch_sync.argtypes = [vp]
     L.m2p_batch_tune.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
@@ -112,6 +113,10 @@ def _dp(a):
     if a is None:
         return None
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
 
 
 def _f64(a):
@@ -360,6 +365,17 @@ class Batch:
 
     def __len__(self) -> int:
         return self._check(lib().m2p_batch_size(self._h), "m2p_batch_size")
+
+    def status(self):
+        """Per member, in order of addition (self.members): (scaled time, steps done, halted,
+        non-finite, first non-finite step or -1), with one device-to-host copy for the batch
+        (m2p_batch_status)."""
+        n = len(self.members)
+        t = np.zeros(max(n, 1), np.float64)
+        nt, h, bad, bad_nt = (np.zeros(max(n, 1), np.int32) for _ in range(4))
+        self._check(lib().m2p_batch_status(self._h, n, _dp(t), _ip(nt), _ip(h), _ip(bad), _ip(bad_nt)),
+                    "m2p_batch_status")
+        return [(float(t[k]), int(nt[k]), bool(h[k]), bool(bad[k]), int(bad_nt[k])) for k in range(n)]
 
     def step(self, n: int = 1) -> None:
         self._check(lib().m2p_batch_step(self._h, int(n)), "m2p_batch_step")

@@ -139,10 +139,11 @@ class Server:
                 continue
             b.step(self.tick_steps)
             b.sync()
-            for rid in list(active):
+            rid_of = {id(sim): rid for rid, (sim, _, _, _) in active.items()}
+            # every member's status with one copy (m2p_batch_status)
+            for sim, (t, nt, halted, bad, bad_nt) in zip(list(b.members), b.status()):
+                rid = rid_of[id(sim)]
                 sim, req, start, steady = active[rid]
-                t, nt, halted = sim.status()
-                bad, bad_nt = sim.nonfinite()
                 if halted or bad:
                     st = sim.state() if (self.keep_state and not bad) else None
                     results.append(Result(rid, req.arrival, start, now(), nt, t, st, bad, bad_nt))

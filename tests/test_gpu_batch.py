@@ -216,6 +216,12 @@ def test_members_halt_on_their_own_rules():
     b.step(20)
     b.sync()
     st = [s.status() for s in sims]
+    # the batch's one-copy status agrees with each member's own
+    for s, row in zip(sims, b.status()):
+        assert row[:3] == s.status() and row[3:] == s.nonfinite()
+    b.remove(sims[2])
+    assert [r[1] for r in b.status()] == [sims[k].status()[1] for k in (0, 1, 3)]
+    b.add(sims[2])
     assert st[0][1] == 5 and st[0][2]
     assert st[1][1] == 7 and st[1][0] == ref_t and st[1][2]
     assert st[2][1] == 20 and not st[2][2]
