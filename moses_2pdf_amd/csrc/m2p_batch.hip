@@ -345,7 +345,10 @@ int m2p_batch_create(int precision, m2p_batch** out) {
   if (!out || (precision != M2P_FP64 && precision != M2P_FP32)) return M2P_EINVAL;
   m2p_batch* b = new m2p_batch();
   b->prec = precision;
-  b->w = precision == M2P_FP32 ? 6 : 4;
+  // untuned default: 3-row tiles (the tuner's usual pick for sparse batches; a 64-request
+  // FP32 serving stream 4% faster than with 7-row tiles at 6 waves), 8 waves in FP32
+  b->ty = 3;
+  b->w = precision == M2P_FP32 ? 8 : 4;
   const hipError_t e = hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking);
   *out = b;
   if (e != hipSuccess) return bfail(b, M2P_EHIP, hipGetErrorString(e));
