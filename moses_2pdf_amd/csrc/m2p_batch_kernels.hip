@@ -83,7 +83,8 @@ __global__ void __launch_bounds__(256) kb_cfl(BatchDev B, int nbb) {
           st = -1;   // from the test
           const int lb = t - mp->box0;
           const DryRegion d = box_region(mp->g, lb % mp->nbx, lb / mp->nbx);
-          for (int e = q; e < d.n; e += CFL_LANES)
+          wet = dry_flags_wet3<7, CFL_LANES>(d, mp->dry.z0, mp->dry.nseg, q);
+          for (int e = d.nf + q; e < d.n; e += CFL_LANES)
             wet |= dry_item_wet(d, (const T*)mp->U0, mp->g, mp->dry.z0, mp->dry.nseg, e);
         }
       }

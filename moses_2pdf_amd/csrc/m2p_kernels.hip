@@ -253,8 +253,8 @@ __global__ void __launch_bounds__(256) k_cfl(Grid g, Phys ph, const T* __restric
       const int k = tid >> 5, t = t0 + k * nbb;
       if (t < nbox) {
         const DryRegion d = box_region(g, t % nbx, t / nbx);
-        int wet = 0;
-        for (int e = tid & 31; e < d.n; e += 32) wet |= dry_item_wet(d, U, g, dm.z0, dm.nseg, e);
+        int wet = dry_flags_wet3<7, 32>(d, dm.z0, dm.nseg, tid & 31);
+        for (int e = d.nf + (tid & 31); e < d.n; e += 32) wet |= dry_item_wet(d, U, g, dm.z0, dm.nseg, e);
         if (wet) atomicOr(&wetk[k], 1);
       }
     }

@@ -7,7 +7,8 @@ non-corner ghost cell a 0.5-weighted sum of interior cells two deep along the bo
 the same row or column (momenta 0).  This test restates the item enumeration and checks,
 for every tile of every compiled tile height and every CFL box of many grid shapes, that
 (a) every item is in bounds, and (b) every cell of the region is either listed (corner),
-covered by a listed flag (interior), or a ghost cell whose source cells are covered."""
+covered by a listed flag (interior), or a ghost cell whose source cells are covered.  It also
+checks the bounds dry_flags_wet3 (the same flag items, walked row by row) relies on."""
 MD = 3
 
 
@@ -30,6 +31,8 @@ def region_items(nx, ny, ra, rb, ca, cb):
         r, s = ia + e // ns, sa + e % ns
         assert MD <= r <= ymax and 0 <= s < nseg
         flags.add((r, s))
+    # what dry_flags_wet3 walks row by row: rows [ia, ib], segments [sa, sa + ns)
+    assert flags == {(r, sa + j) for r in range(ia, ib + 1) for j in range(ns)}
     for e in range(nrg * ncg):
         ir, ic = e // ncg, e % ncg
         r = rt + ir if ir < nrt else rbo + (ir - nrt)
@@ -40,8 +43,10 @@ def region_items(nx, ny, ra, rb, ca, cb):
     return (ra, rb, ca, cb), flags, corners
 
 
-def check_region(nx, ny, ra, rb, ca, cb):
+def check_region(nx, ny, ra, rb, ca, cb, walked=False):
     (ra, rb, ca, cb), flags, corners = region_items(nx, ny, ra, rb, ca, cb)
+    if walked:   # dry_flags_wet3's bounds (stage tiles: TY + 4 rows, CFL boxes 7): 3 segments
+        assert len({s for _, s in flags}) <= 3 and rb - ra + 1 <= 19
     xmax, ymax = nx + MD - 1, ny + MD - 1
     cov = lambda r, c: (r, (c - MD) // 63) in flags
     for r in range(ra, rb + 1):
@@ -68,12 +73,12 @@ def test_dry_regions_cover_their_stencils():
             for tx in range((nx + 62) // 63):
                 for ty in range((ny + TY - 1) // TY):
                     c0, r0 = MD + 63 * tx, MD + TY * ty
-                    check_region(nx, ny, r0 - 2, r0 + TY + 1, c0 - 2, c0 + 64)
+                    check_region(nx, ny, r0 - 2, r0 + TY + 1, c0 - 2, c0 + 64, True)
                     n += 1
         for bx in range((nx + 63) // 64):
             for by in range((ny + 3) // 4):
                 clo, rlo = MD + 64 * bx, MD + 4 * by
-                check_region(nx, ny, rlo - 1, rlo + 5, clo - 1, clo + 65)
+                check_region(nx, ny, rlo - 1, rlo + 5, clo - 1, clo + 65, True)
                 n += 1
     assert n > 1000
 
