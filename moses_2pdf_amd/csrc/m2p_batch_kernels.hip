@@ -52,8 +52,12 @@ __global__ void __launch_bounds__(GHOST_LINES) kb_ghost(BatchDev B, int post) {
 constexpr int CFL_GROUP = 64;
 constexpr int CFL_LANES = 256 / CFL_GROUP;
 
+// Waves per SIMD: the ring faces and the dry tests are chains of dependent loads, so
+// occupancy sets their pace (at the compiler's choice, 3 in FP64 and 4 in FP32, a 256-member
+// ensemble step was 4% slower in FP32, 2% in FP64).
 template <typename T, bool UNI>
-__global__ void __launch_bounds__(256) kb_cfl(BatchDev B, int nbb) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CFL_WAVES<T>, CFL_WAVES<T>)))
+    kb_cfl(BatchDev B, int nbb) {
   __shared__ int state[CFL_GROUP];   // 0: dry, 1: compute, 2: halted or past the end
   const int tid = threadIdx.y * 64 + threadIdx.x;
   if ((int)blockIdx.x >= nbb) {   // ring faces, one ring block per wave

@@ -113,6 +113,11 @@ constexpr int PLAN_COUNTERS = 16;   // the 8 list lengths of each of the two sta
 #define M2P_PLAN_CHUNK 32
 #endif
 constexpr int PLAN_CHUNK = M2P_PLAN_CHUNK;
+
+// Waves per SIMD of the CFL kernels: 128 VGPRs in FP64 (no spills), 64 in FP32 (12-28
+// bytes of spills, cheaper than the lost occupancy).
+template <typename T>
+constexpr int CFL_WAVES = sizeof(T) == 4 ? 8 : 4;
 struct TilePlan {
   int* list;   // 8 * per entries
   int* cnt;    // 8 list lengths

@@ -222,8 +222,11 @@ constexpr int CFL_BLOCKS = 4096;
 // slots (2048^2: skipping step FP64 -5%, FP32 -9%; the dense pass is fastest at 4096)
 constexpr int CFL_BLOCKS_DRY = 1024;
 
+// Waves per SIMD CFL_WAVES: with dry-tile skipping the box tests are chains of dependent
+// loads (at the compiler's choice of 3 waves the FP64 2048^2 skipping step was 6% slower).
 template <typename T, bool UNI>
-__global__ void __launch_bounds__(256) k_cfl(Grid g, Phys ph, const T* __restrict__ U,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CFL_WAVES<T>, CFL_WAVES<T>)))
+    k_cfl(Grid g, Phys ph, const T* __restrict__ U,
                                              const T* __restrict__ M, DevState* __restrict__ st,
                                              double* __restrict__ part, int nbx, int nbox,
                                              int nbb, DryMap dm) {
