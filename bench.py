@@ -231,8 +231,8 @@ def ensemble_measure(args, dense_steps=None) -> dict:
         scale = steps / dense_steps
         pd = E.run_plain(scs, dense_steps, "fp64", keep=False, dry_skip=False)
         bd = E.run_batch(scs, dense_steps, "fp32", keep=False, dry_skip=False)
-        wp = pd["open_s"] + pd["step_s"] * scale
-        wb = bd["open_s"] + bd["tune_s"] + bd["step_s"] * scale
+        wp = pd["wall_s"] + pd["step_s"] * (scale - 1.0)
+        wb = bd["wall_s"] + bd["step_s"] * (scale - 1.0)
         out["dense"] = {"timed_steps": dense_steps, "plain_fp64_scenarios_per_hour": per_h(wp),
                         "batched_fp32_scenarios_per_hour": per_h(wb), "speedup_vs_plain_fp64": wp / wb,
                         "note": f"every tile computed; stepping timed over {dense_steps} of the "
