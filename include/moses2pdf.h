@@ -274,12 +274,13 @@ int m2p_batch_status(m2p_batch* b, int n, double* t, int* nt, int* halted, int* 
 /* Enqueues nsteps steps of every member (asynchronous; m2p_batch_sync waits). */
 int m2p_batch_step(m2p_batch* b, int nsteps);
 int m2p_batch_sync(m2p_batch* b);
-/* Autotuner: times every compiled stage-kernel variant (tile height x waves per SIMD)
- * under each block schedule (XCD banding on/off, member order as added / largest first)
- * for trial_steps steps on the batch itself, then, under the fastest, the CFL launch's
- * block count (2048 .. 32768), and keeps the fastest.  The trials start from the state
- * one step on, restored after each; every member's state, time and step count are put
- * back at the end.  A text table of the timings goes to report (nullable). */
+/* Autotuner: times, for trial_steps steps on the batch itself, every compiled stage-kernel
+ * variant (tile height x waves per SIMD) with XCD banding and largest-first member order
+ * on; then, for the fastest, member order as added and banding off, each kept if faster;
+ * then the CFL launch's block count, doubling from 4096 and halving while faster
+ * (2048 .. 32768).  The trials start from the state one step on, restored after each;
+ * every member's state, time and step count are put back at the end, and a failed search
+ * leaves the schedule as it was.  A text table of the timings goes to report (nullable). */
 int m2p_batch_tune(m2p_batch* b, int trial_steps, char* report, int report_len);
 /* Manual schedule: keys "ty", "waves", "remap", "order", "graph_steps" (0: no graph),
  * "cfl_blocks" (>= 64; most blocks of the CFL launch, default 4096), "plan_cap" (>= 1;
