@@ -8,8 +8,10 @@ compared.
 """
 from __future__ import annotations
 
+import os
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -20,6 +22,33 @@ from .engine import Batch, Simulation
 
 def log(*a):
     print("#", *a, file=sys.stderr, flush=True)
+
+
+def open_all(scs, precision, dry_skip=True, workers=None):
+    """Opens every scenario, on up to `workers` host threads (default min(8, CPUs)).
+    Distinct contexts may be opened from different threads (the C ABI's contract), and
+    ctypes releases the GIL inside m2p_open_grid, so the members' host-side padding,
+    uploads and set-up kernels overlap.  Returns the simulations in the order of scs; if
+    one fails, the others are closed and the error raised."""
+    n = workers or min(8, os.cpu_count() or 1)
+    if n <= 1 or len(scs) <= 1:
+        return [Simulation(sc, precision=precision, dry_skip=dry_skip) for sc in scs]
+
+    def one(sc):
+        try:
+            return Simulation(sc, precision=precision, dry_skip=dry_skip)
+        except Exception as e:   # returned, so that the others still get closed
+            return e
+
+    with ThreadPoolExecutor(max_workers=n) as pool:
+        sims = list(pool.map(one, scs))
+    bad = [x for x in sims if isinstance(x, Exception)]
+    if bad:
+        for x in sims:
+            if not isinstance(x, Exception):
+                x.close()
+        raise bad[0]
+    return sims
 
 
 def population(count: int = 256, seed: int = 4):
@@ -61,7 +90,7 @@ def run_batch(scs, steps, precision, tune_steps=3, t_end=None, keep=True, graph_
     "wall_s" runs from the first open to the end of stepping ("open_s", "tune_s" and
     "step_s" are its parts; the rest is adding the members to the batch)."""
     t0 = time.perf_counter()
-    sims = [Simulation(sc, precision=precision, dry_skip=dry_skip) for sc in scs]
+    sims = open_all(scs, precision, dry_skip)
     t_open = time.perf_counter() - t0
     b = Batch(precision)
     for s in sims:
