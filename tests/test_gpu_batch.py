@@ -264,3 +264,24 @@ def test_stop_speed_rule_halts_a_settling_flow():
         s.set_stop(vstop=1e-30)        # reached at once, then never below it
         s.step(30)
         assert not s.status()[2]
+
+
+def test_members_opened_on_threads_match_opened_one_by_one():
+    # ensemble.open_all opens on several host threads (pooled streams are shared between
+    # contexts): each member's set-up must not see another's, so the batch ends with the
+    # same bits as members opened one after another
+    from moses_2pdf_amd import ensemble as EN
+    scs = population() * 3
+    ref = batched(scs, 20, "fp32")
+    sims = EN.open_all(scs, "fp32", workers=8)
+    b = Batch("fp32")
+    for s in sims:
+        b.add(s)
+    b.step(20)
+    b.sync()
+    got = [(s.state(), s.time_steps()) for s in sims]
+    b.close()
+    for s in sims:
+        s.close()
+    for (x, tx), (y, ty) in zip(got, ref):
+        assert np.array_equal(x.view(np.uint64), y.view(np.uint64)) and tx == ty
