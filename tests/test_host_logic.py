@@ -2,6 +2,7 @@
 import math
 
 import numpy as np
+import pytest
 
 from moses_2pdf_amd import scenarios as S
 from moses_2pdf_amd import serving as V
@@ -66,3 +67,27 @@ def test_ensemble_scenario_release_inside_domain():
         assert d[0].max() == 0 and d[-1].max() == 0 and d[:, 0].max() == 0 and d[:, -1].max() == 0
         vol = d.sum() * sc.cellsize ** 2
         assert abs(vol - m.volume) < 0.1 * m.volume
+
+
+def test_open_all_keeps_order_and_closes_the_others_when_one_fails(monkeypatch):
+    # ensemble.open_all opens on host threads: results come back in the scenarios' order,
+    # and when one open raises, every member opened is closed and that error raised
+    from moses_2pdf_amd import ensemble as EN
+
+    closed = []
+
+    class FakeSim:
+        def __init__(self, sc, precision="fp64", dry_skip=True):
+            if sc == "bad":
+                raise RuntimeError("open failed")
+            self.sc = sc
+
+        def close(self):
+            closed.append(self.sc)
+
+    monkeypatch.setattr(EN, "Simulation", FakeSim)
+    sims = EN.open_all(list(range(40)), "fp32", workers=8)
+    assert [s.sc for s in sims] == list(range(40))
+    with pytest.raises(RuntimeError, match="open failed"):
+        EN.open_all([0, 1, "bad", 3, 4], "fp32", workers=4)
+    assert sorted(closed) == [0, 1, 3, 4]
